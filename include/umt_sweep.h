@@ -1,0 +1,163 @@
+/*
+ * C-ABI of the MI355X UCB xyz sweep library (libumt_amd.so).
+ *
+ * Every entry point takes plain pointers and sizes. Arrays use the Teton
+ * layout: column-major shapes (first index fastest, groups fastest), index
+ * contents 1-based (corner, zone, face, boundary-element numbers).
+ * Functions returning int return 0 on success and a nonzero code otherwise;
+ * umt_last_error() gives the message of the calling thread's last failure.
+ * Nothing here aborts or exits the process.
+ */
+#ifndef UMT_SWEEP_H
+#define UMT_SWEEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Drop-in sweep, one angle of one phase-space set per call.
+ *
+ * Replaces the CUDA entry point of the same name and argument list,
+ * src/teton/gpu/GPU_SweepUCBxyz.cu:532-572, bound from Fortran with
+ * bind(c) in src/teton/gpu/SweepUCBxyzToGPU.F90:43-91 and called at
+ * SweepUCBxyzToGPU.F90:198-240. Semantics follow snac/SweepUCBxyz.F90:11-322:
+ * the boundary tail of Psi1 is loaded from PsiB(:,:,Angle), Psi1 receives the
+ * corner fluxes, Phi += quadwt * Psi1, exiting boundary fluxes go to PsiB, and
+ * Psi(:,:,Angle) = Psi1 when *savePsi != 0. Outputs are complete when the call
+ * returns (gpu_streamsynchronize is kept for the caller's sequence).
+ * Supported: 3D hex zones in the corner-face order setTetonZone produces,
+ * acyclic schedules (*numCycles == 0), no reflecting boundary
+ * (*nBdyElem == 0 or *Mref < 1). Anything else leaves the outputs untouched
+ * and sets a nonzero gpu_sweepucbxyz_status().
+ * ---------------------------------------------------------------------- */
+void gpu_sweepucbxyz(int *Angle, int *nHyperPlanes, int *nZonesInPlane, int *nextZ, int *nextC,
+                     double *STotal, double *tau, double *Psi, int *Groups, double *Volume,
+                     double *Sigt, int *nCFacesArray, int *ndim, int *maxcf, int *ncorner,
+                     double *A_fp, double *omega, int *cFP, double *Psi1, int *nbelem,
+                     double *A_ez, int *cEZ, int *NumAngles, double *quadwt, double *Phi,
+                     double *PsiB, int *maxCorner, int *mem0solve1, int *streamIdPtr,
+                     int *totalStreams, int *savePsi, int *numCycles, int *cycleOffSet,
+                     double *cyclePsi, int *cycleList, int *b0, int *nBdyElem, double *PsiBMref,
+                     int *Mref, int *Geom_numCorner, int *Geom_cOffSet);
+
+/* Replaces gpu_streamsynchronize, GPU_SweepUCBxyz.cu:520-523. */
+void gpu_streamsynchronize(int *streamId);
+
+/* Replaces gpu_devicesynchronize, GPU_SweepUCBxyz.cu:525-528. */
+void gpu_devicesynchronize(void);
+
+/* Status of the calling thread's last gpu_sweepucbxyz call (0 = success).
+ * No counterpart in the reference, whose wrapper exits the process on error
+ * (GPU_SweepUCBxyz.cu:18-27); the argument list above is kept unchanged. */
+int gpu_sweepucbxyz_status(void);
+
+/* Message of the calling thread's last failure, or "" if none. */
+const char *umt_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Problem construction on the host (mesh, geometry, quadrature, schedule).
+ * ---------------------------------------------------------------------- */
+typedef struct umt_problem umt_problem;
+
+/* Tiled hex mesh of [0,1]^3 (nx*ny*nz tiles of 24 zones, UMT's tile,
+ * driver/test_driver.cc:1787-1816), corner connectivity as
+ * aux/setTetonZone.F90:143-191, corner geometry as rt/geometryUCBxyz.F90,
+ * product quadrature (rt/quadProduct.F90, polar axis x) with
+ * npolar x nazimuthal directions per octant, and the per-angle schedule of
+ * rt/rtorder.F90 / snac/snnext.F90. kdtree != 0 orders zones by a k-d tree. */
+int umt_problem_tiled(int nx, int ny, int nz, int npolar, int nazimuthal, int kdtree,
+                      umt_problem **out);
+void umt_problem_destroy(umt_problem *p);
+
+/* dims[0..7] = nzones, ncorner, nbelem, nangles, maxPlanes, npoints,
+ *              total planes over all angles, maxcf (= 3). */
+int umt_problem_dims(const umt_problem *p, int64_t *dims);
+
+/* Mesh arrays (Geometry_mod names): numCorner(Z), cOffSet(Z), nCFacesArray(C),
+ * cFP(3,C), cEZ(3,C), CToFace(3,C), zoneOpp(6,Z), faceOpp(6,Z), px(3,C).
+ * Any output pointer may be NULL. */
+int umt_problem_mesh(const umt_problem *p, int *numCorner, int *cOffSet, int *nCFaces, int *cFP,
+                     int *cEZ, int *CToFace, int *zoneOpp, int *faceOpp, double *px);
+
+/* A_fp(3,3,C), A_ez(3,3,C), Volume(C), VolumeZone(Z). */
+int umt_problem_geometry(const umt_problem *p, double *A_fp, double *A_ez, double *Volume,
+                         double *VolumeZone);
+
+/* omega(3,A), weight(A) with sum(weight) = 4 pi. */
+int umt_problem_quadrature(const umt_problem *p, double *omega, double *weight);
+
+/* nHyperPlanes(A), zonesInPlane (all angles concatenated), nextZ(Z,A),
+ * nextC(C,A) (AngleSet_mod names). */
+int umt_problem_schedule(const umt_problem *p, int *nHyperPlanes, int *zonesInPlane, int *nextZ,
+                         int *nextC);
+
+/* Seeded cross sections per (zone, group), counter-based so any ordering of
+ * the draws gives the same values: sigt ~ log-uniform[sigt_lo, sigt_hi],
+ * sigs = c * sigt with c ~ U[c_lo, c_hi], q ~ U[q_lo, q_hi]. Arrays (G, Z). */
+int umt_draw_cross_sections(uint64_t seed, int nzones, int groups, double sigt_lo,
+                            double sigt_hi, double c_lo, double c_hi, double q_lo, double q_hi,
+                            double *sigt, double *sigs, double *q);
+
+/* ------------------------------------------------------------------------
+ * Device-resident sweep context: geometry, schedule (hyperplane zone
+ * batches over all angles), cross sections, psi for all angles and phi stay
+ * in HBM. precision = 64 (FP64 sweep) or 32 (FP32 sweep; phi, the
+ * source-iteration residual and balance stay FP64).
+ * ---------------------------------------------------------------------- */
+typedef struct umt_ctx umt_ctx;
+
+/* From Teton-layout arrays (as a Fortran host holds them). zonesInPlane is
+ * the concatenation over angles of each angle's nHyperPlanes entries. */
+int umt_ctx_create(int nzones, int ncorner, int nbelem, int groups, int nangles, int precision,
+                   const int *numCorner, const int *cOffSet, const int *nCFaces, const int *cFP,
+                   const int *cEZ, const double *A_fp, const double *A_ez, const double *Volume,
+                   const double *VolumeZone, const double *omega, const double *weight,
+                   const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ,
+                   umt_ctx **out);
+int umt_ctx_create_from_problem(const umt_problem *p, int groups, int precision, umt_ctx **out);
+void umt_ctx_destroy(umt_ctx *ctx);
+
+/* Sigt(G,Z) = total cross section seen by the sweep (setTotalOpacity with tau = 0). */
+int umt_ctx_set_sigt(umt_ctx *ctx, const double *sigt);
+/* STotal(G,C): source per steradian for the next sweep. */
+int umt_ctx_set_source(umt_ctx *ctx, const double *STotal);
+/* Scattering cross section sigs(G,Z) and fixed source q(G,Z) for source iteration. */
+int umt_ctx_set_scattering(umt_ctx *ctx, const double *sigs, const double *q);
+
+/* One sweep of all angles with the current source: psi for every angle and
+ * phi = sum_a w_a psi. */
+int umt_ctx_sweep(umt_ctx *ctx);
+int umt_ctx_get_phi(const umt_ctx *ctx, double *phi);   /* (G,C) */
+/* psi(G, C+nbelem, A): corner fluxes, then the boundary tail (incoming values
+ * for incoming boundary elements, exiting fluxes for exiting ones). */
+int umt_ctx_get_psi(const umt_ctx *ctx, double *psi);
+int umt_ctx_set_phi(umt_ctx *ctx, const double *phi);   /* (G,C) */
+
+/* Source iteration: S = (sigs phi + q)/(4 pi); sweep; phi from psi; residual =
+ * max over (zone, group) of |phibar_new - phibar_old| / phibar_new for
+ * phibar_new > 1e-6 * max(phibar_new) (rt/ConvergenceTest.F90:150-168), zone
+ * averages volume-weighted. Starts from the phi held by the context. Stops
+ * when residual < tol or after maxIters iterations; residual_history (may be
+ * NULL) receives up to maxIters values. Returns 0 when converged, 1 when
+ * maxIters was reached (the state is still valid). */
+int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
+                             double *residual, double *residual_history);
+
+/* Timing: warmup + reps sweeps of the current source, timed with HIP events
+ * on the context's stream. ms_sweep = mean wall time of one full sweep
+ * (level kernels + phi reduction); ms_level_kernels = mean summed duration
+ * of the sweep-level launches in one sweep; launches = level launches per sweep. */
+int umt_ctx_time_sweeps(umt_ctx *ctx, int warmup, int reps, double *ms_sweep,
+                        double *ms_level_kernels, int *launches);
+
+/* dims[0..6] = nzones, ncorner, nbelem, groups, nangles, levels, items. */
+int umt_ctx_dims(const umt_ctx *ctx, int64_t *dims);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UMT_SWEEP_H */

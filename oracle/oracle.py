@@ -1,0 +1,172 @@
+"""ORACLE — test infrastructure only. ctypes binding of the C restatement of
+the reference sweep path (oracle.c). Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_pi = C.POINTER(C.c_int)
+_pd = C.POINTER(C.c_double)
+_i, _d = C.c_int, C.c_double
+
+_SIG = {
+    "oracle_geometry": (_i, [_i, _i, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
+    "oracle_quadrature": (_i, [_i, _i, _i, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "oracle_schedule_angle": (_i, [_i, _i, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd,
+                                   _pi, _pi, _pi, _pi]),
+    "oracle_sweep_angle": (None, [_i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
+                                  _pd, _d, _pd, _pd, _d, _i, _pi, _pi, _pi, _pd, _pd, _pd]),
+    "oracle_sweep_all": (None, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
+                                _pd, _pd, _pd, _pi, _pi, _pi, _pi, _i, _pd, _pd]),
+    "oracle_source_iteration": (_i, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd,
+                                     _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, _i, _d, _i,
+                                     _pd, _pd, _pd]),
+}
+_libs = {}
+
+
+def load(fast=False):
+    """liboracle.so (parity checker) or liboracle_fast.so (timed CPU baseline)."""
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", _HERE, name], check=True, capture_output=True)
+    lib = C.CDLL(path)
+    for fn, (res, args) in _SIG.items():
+        f = getattr(lib, fn)
+        f.restype, f.argtypes = res, args
+    _libs[name] = lib
+    return lib
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_pi)
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_pd)
+
+
+def geometry(mesh):
+    """A_fp (C,3,3), A_ez (C,3,3), Volume (C) from mesh arrays."""
+    Cn = mesh["nCFaces"].shape[0]
+    Z = mesh["numCorner"].shape[0]
+    A_fp, A_ez, V = np.zeros((Cn, 3, 3)), np.zeros((Cn, 3, 3)), np.zeros(Cn)
+    load().oracle_geometry(Z, Cn, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]),
+                           _ip(mesh["zoneFaces"]), _ip(mesh["nCFaces"]), _ip(mesh["cFP"]),
+                           _ip(mesh["cEZ"]), _ip(mesh["CToFace"]), _ip(mesh["zoneOpp"]),
+                           _dp(mesh["px"]), _dp(A_fp), _dp(A_ez), _dp(V))
+    return A_fp, A_ez, V
+
+
+def quadrature(npolar, nazimuthal, cos_theta, w_theta, cos_phi, w_phi, polaraxis=1):
+    A = 8 * npolar * nazimuthal
+    omega, weight = np.zeros((A, 3)), np.zeros(A)
+    rc = load().oracle_quadrature(npolar, nazimuthal, polaraxis,
+                                  _dp(np.ascontiguousarray(cos_theta, np.float64)),
+                                  _dp(np.ascontiguousarray(w_theta, np.float64)),
+                                  _dp(np.ascontiguousarray(cos_phi, np.float64)),
+                                  _dp(np.ascontiguousarray(w_phi, np.float64)), _dp(omega),
+                                  _dp(weight))
+    assert rc == 0
+    return omega, weight
+
+
+def schedule_angle(mesh, A_fp, A_ez, omega):
+    """(rc, nextZ (Z), nextC (C), zonesInPlane (nHP)) for one direction."""
+    Z = mesh["numCorner"].shape[0]
+    Cn = mesh["nCFaces"].shape[0]
+    nextZ, nextC = np.zeros(Z, np.int32), np.zeros(Cn, np.int32)
+    zip_ = np.zeros(Z, np.int32)
+    nhp = C.c_int(0)
+    rc = load().oracle_schedule_angle(
+        Z, Cn, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["zoneFaces"]),
+        _ip(mesh["nCFaces"]), _ip(mesh["cEZ"]), _ip(mesh["CToFace"]), _ip(mesh["zoneOpp"]),
+        _ip(mesh["faceOpp"]), _dp(A_fp), _dp(A_ez),
+        _dp(np.ascontiguousarray(omega, np.float64)), _ip(nextZ), _ip(nextC), _ip(zip_),
+        C.byref(nhp))
+    return rc, nextZ, nextC, zip_[:nhp.value].copy()
+
+
+def schedule(mesh, A_fp, A_ez, omega_all):
+    """Per-angle schedule for every direction: dict like TiledProblem.schedule()."""
+    A = omega_all.shape[0]
+    nh, zips, nz, nc = [], [], [], []
+    for a in range(A):
+        rc, nextZ, nextC, zip_ = schedule_angle(mesh, A_fp, A_ez, omega_all[a])
+        if rc:
+            raise RuntimeError(f"oracle schedule failed for angle {a + 1}: code {rc}")
+        nh.append(len(zip_))
+        zips.append(zip_)
+        nz.append(nextZ)
+        nc.append(nextC)
+    return dict(nHyperPlanes=np.array(nh, np.int32), zonesInPlane=np.concatenate(zips),
+                nextZ=np.stack(nz), nextC=np.stack(nc))
+
+
+def sweep_angle(mesh, geom, STotal, Sigt, omega, quadwt, nextZ, nextC, zonesInPlane, Psi1, PsiB,
+                Phi, tau=0.0, Psi=None):
+    """One angle, in place on Psi1 (C+nb,G), PsiB (nb,G), Phi (C,G)."""
+    Z = mesh["numCorner"].shape[0]
+    Cn = mesh["nCFaces"].shape[0]
+    G = STotal.shape[1]
+    nb = PsiB.shape[0]
+    load().oracle_sweep_angle(
+        Z, Cn, nb, G, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["nCFaces"]),
+        _ip(mesh["cFP"]), _ip(mesh["cEZ"]), _dp(geom["A_fp"]), _dp(geom["A_ez"]),
+        _dp(geom["Volume"]), _dp(STotal), _dp(Sigt), tau, _dp(Psi),
+        _dp(np.ascontiguousarray(omega, np.float64)), quadwt, len(zonesInPlane),
+        _ip(np.ascontiguousarray(zonesInPlane, np.int32)), _ip(np.ascontiguousarray(nextZ, np.int32)),
+        _ip(np.ascontiguousarray(nextC, np.int32)), _dp(Psi1), _dp(PsiB), _dp(Phi))
+
+
+def sweep_all(mesh, geom, quad, sched, STotal, Sigt, nthreads=1, PsiB=None, fast=False):
+    """All angles (SetSweep semantics). Returns (Phi (C,G), PsiB (A,nb,G))."""
+    Z = mesh["numCorner"].shape[0]
+    Cn = mesh["nCFaces"].shape[0]
+    G = STotal.shape[1]
+    A = quad["weight"].shape[0]
+    nb = int(np.count_nonzero(mesh["cFP"] > Cn))
+    if PsiB is None:
+        PsiB = np.zeros((A, nb, G))
+    Phi = np.zeros((Cn, G))
+    load(fast).oracle_sweep_all(
+        Z, Cn, nb, G, A, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["nCFaces"]),
+        _ip(mesh["cFP"]), _ip(mesh["cEZ"]), _dp(geom["A_fp"]), _dp(geom["A_ez"]),
+        _dp(geom["Volume"]), _dp(np.ascontiguousarray(STotal)), _dp(np.ascontiguousarray(Sigt)),
+        _dp(quad["omega"]), _dp(quad["weight"]), _ip(sched["nHyperPlanes"]),
+        _ip(sched["zonesInPlane"]), _ip(sched["nextZ"]), _ip(sched["nextC"]), nthreads, _dp(PsiB),
+        _dp(Phi))
+    return Phi, PsiB
+
+
+def source_iteration(mesh, geom, quad, sched, Sigt, sigs, q, tol, max_iters, phi0=None,
+                     nthreads=1, fast=False):
+    """Returns (Phi (C,G), iterations, residual, history)."""
+    Z = mesh["numCorner"].shape[0]
+    Cn = mesh["nCFaces"].shape[0]
+    G = Sigt.shape[1]
+    A = quad["weight"].shape[0]
+    nb = int(np.count_nonzero(mesh["cFP"] > Cn))
+    Phi = np.zeros((Cn, G)) if phi0 is None else np.array(phi0, dtype=np.float64, order="C")
+    res = C.c_double(0.0)
+    hist = np.zeros(max(max_iters, 1))
+    it = load(fast).oracle_source_iteration(
+        Z, Cn, nb, G, A, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["nCFaces"]),
+        _ip(mesh["cFP"]), _ip(mesh["cEZ"]), _dp(geom["A_fp"]), _dp(geom["A_ez"]),
+        _dp(geom["Volume"]), _dp(geom["VolumeZone"]), _dp(np.ascontiguousarray(Sigt)),
+        _dp(np.ascontiguousarray(sigs)), _dp(np.ascontiguousarray(q)), _dp(quad["omega"]),
+        _dp(quad["weight"]), _ip(sched["nHyperPlanes"]), _ip(sched["zonesInPlane"]),
+        _ip(sched["nextZ"]), _ip(sched["nextC"]), nthreads, tol, max_iters, _dp(Phi),
+        C.byref(res), _dp(hist))
+    return Phi, it, res.value, hist[:it].copy()

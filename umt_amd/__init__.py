@@ -1,0 +1,10 @@
+"""MI355X-native UCB xyz Sn sweep (Teton / LLNL UMT hot path).
+
+The compute path is libumt_amd.so (hand-written HIP kernels for gfx950 behind
+the C-ABI of include/umt_sweep.h); this package is the Python host side.
+"""
+from ._lib import LIB_PATH, UmtError, load  # noqa: F401
+from .sweep import SweepContext, TiledProblem, draw_cross_sections, sweep_ucbxyz  # noqa: F401
+
+__all__ = ["SweepContext", "TiledProblem", "draw_cross_sections", "sweep_ucbxyz", "load",
+           "UmtError", "LIB_PATH"]

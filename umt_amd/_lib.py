@@ -1,0 +1,89 @@
+"""ctypes binding of libumt_amd.so (declarations in include/umt_sweep.h).
+
+The library is built in-tree (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C umt_amd/csrc``). Loading fails loudly when it is missing: there
+is no CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libumt_amd.so")
+
+_i = C.c_int
+_i64 = C.c_int64
+_u64 = C.c_uint64
+_d = C.c_double
+_pi = C.POINTER(C.c_int)
+_pi64 = C.POINTER(C.c_int64)
+_pd = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "umt_last_error": (C.c_char_p, []),
+    "gpu_sweepucbxyz": (None, [_pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi, _pd, _pd, _pi, _pi,
+                               _pi, _pi, _pd, _pd, _pi, _pd, _pi, _pd, _pi, _pi, _pd, _pd, _pd,
+                               _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pi, _pi, _pi, _pd, _pi,
+                               _pi, _pi]),
+    "gpu_streamsynchronize": (None, [_pi]),
+    "gpu_devicesynchronize": (None, []),
+    "gpu_sweepucbxyz_status": (_i, []),
+    "umt_problem_tiled": (_i, [_i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
+    "umt_problem_destroy": (None, [_vp]),
+    "umt_problem_dims": (_i, [_vp, _pi64]),
+    "umt_problem_mesh": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd]),
+    "umt_problem_geometry": (_i, [_vp, _pd, _pd, _pd, _pd]),
+    "umt_problem_quadrature": (_i, [_vp, _pd, _pd]),
+    "umt_problem_schedule": (_i, [_vp, _pi, _pi, _pi, _pi]),
+    "umt_draw_cross_sections": (_i, [_u64, _i, _i, _d, _d, _d, _d, _d, _d, _pd, _pd, _pd]),
+    "umt_ctx_create": (_i, [_i, _i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
+                            _pd, _pd, _pi, _pi, _pi, C.POINTER(_vp)]),
+    "umt_ctx_create_from_problem": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
+    "umt_ctx_destroy": (None, [_vp]),
+    "umt_ctx_set_sigt": (_i, [_vp, _pd]),
+    "umt_ctx_set_source": (_i, [_vp, _pd]),
+    "umt_ctx_set_scattering": (_i, [_vp, _pd, _pd]),
+    "umt_ctx_sweep": (_i, [_vp]),
+    "umt_ctx_get_phi": (_i, [_vp, _pd]),
+    "umt_ctx_get_psi": (_i, [_vp, _pd]),
+    "umt_ctx_set_phi": (_i, [_vp, _pd]),
+    "umt_ctx_source_iteration": (_i, [_vp, _d, _i, _pi, _pd, _pd]),
+    "umt_ctx_time_sweeps": (_i, [_vp, _i, _i, _pd, _pd, _pi]),
+    "umt_ctx_dims": (_i, [_vp, _pi64]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libumt_amd.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it first (make -C umt_amd/csrc). "
+            "The sweep has no CPU fallback.")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class UmtError(RuntimeError):
+    """A nonzero status from the C-ABI, with the library's message."""
+
+    def __init__(self, fn, code):
+        msg = load().umt_last_error().decode(errors="replace")
+        super().__init__(f"{fn} returned {code}: {msg}")
+        self.code = code
+
+
+def check(fn, code):
+    if code != 0:
+        raise UmtError(fn, code)
+    return code

@@ -1,0 +1,45 @@
+// Kernel argument blocks and launch wrappers for the UCB sweep kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace umt {
+
+template <typename Real>
+struct SweepLevelArgs {
+  const int2* items;        // (zone, angle) pairs of this level, 0-based
+  int nItems;
+  int G;                    // groups
+  int C;                    // corners; cFP values above C are boundary elements
+  const Real* Afp;          // (3, 3, C)
+  const Real* Aez;          // (3, 3, C)
+  const Real* vol;          // (C)
+  const int* cFP;           // (3, C), 1-based contents
+  const Real* S;            // (G, C) source per steradian
+  const Real* sigt;         // (G, Z)
+  const double* omega;      // (3, A)
+  Real* psi;                // (G, C + nbelem, A) all angles, or one angle
+  long psiAngleStride;      // (C + nbelem) * G
+  // Single-angle (drop-in) extras.
+  double* phi;              // (G, C), phi += quadwt * psi
+  double quadwt;
+  const Real* tauPsi;       // (G, C) or null: source += tau * Psi
+  Real tau;
+};
+
+template <typename Real>
+hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s);
+template <typename Real>
+hipError_t launch_phi_from_psi(const Real* psi, const double* w, int A, long stride, long n,
+                               double* phi, hipStream_t s);
+template <typename Real>
+hipError_t launch_source_update(const double* phi, const double* sigs, const double* q, int G,
+                                long n, Real* S, hipStream_t s);
+hipError_t launch_residual(const double* phi, const double* vol, const double* volZ, int G, long nzg,
+                           double* phibar, double* phibarOld, double cutoff,
+                           unsigned long long* bits, hipStream_t s);
+template <typename T>
+hipError_t launch_fill_zero(T* p, long n, hipStream_t s);
+template <typename Dst, typename Src>
+hipError_t launch_convert(const Src* s, Dst* d, long n, hipStream_t s_);
+
+}  // namespace umt

@@ -1,0 +1,703 @@
+// Host runtime of the MI355X UCB sweep: problem construction, the
+// device-resident sweep context and the drop-in entry points of
+// include/umt_sweep.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/umt_sweep.h"
+#include "host.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_dropin_status = 0;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(100 + (int)e_, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+// Canonical hex corner topology (1-based EZ neighbours per corner-face slot),
+// the layout the sweep kernels are specialised for.
+const int kCanonicalEZ[8][3] = {{4, 2, 5}, {1, 3, 6}, {2, 4, 7}, {3, 1, 8},
+                                {6, 8, 1}, {7, 5, 2}, {8, 6, 3}, {5, 7, 4}};
+
+int check_hex_topology(int Z, int C, const int* numCorner, const int* cOffSet,
+                       const int* nCFaces, const int* cEZ) {
+  if (C != 8 * Z) return fail(10, "ncorner (%d) != 8 * nzones (%d): only hex zones are supported", C, Z);
+  for (int z = 0; z < Z; ++z) {
+    if (numCorner[z] != 8 || cOffSet[z] != 8 * z)
+      return fail(11, "zone %d: %d corners at offset %d (need 8 corners at offset %d)", z + 1,
+                  numCorner[z], cOffSet[z], 8 * z);
+    for (int c = 0; c < 8; ++c) {
+      const long cc = 8L * z + c;
+      if (nCFaces[cc] != 3) return fail(12, "corner %ld has %d corner faces (need 3)", cc + 1, nCFaces[cc]);
+      for (int f = 0; f < 3; ++f)
+        if (cEZ[3 * cc + f] != kCanonicalEZ[c][f])
+          return fail(13, "zone %d corner %d: corner-face order differs from the hex layout the kernels use",
+                      z + 1, c + 1);
+    }
+  }
+  return 0;
+}
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return 0;
+  HIP_TRY(hipMalloc((void**)p, n * sizeof(T)));
+  return 0;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct umt_problem {
+  umt::Mesh m;
+  umt::Geometry g;
+  umt::Quadrature q;
+  umt::Schedule s;
+};
+
+struct umt_ctx {
+  int Z = 0, C = 0, nb = 0, G = 0, A = 0, precision = 64;
+  hipStream_t stream = nullptr;
+  // Geometry and schedule (sweep precision for geometry).
+  void* Afp = nullptr;
+  void* Aez = nullptr;
+  void* vol = nullptr;
+  int* cFP = nullptr;
+  double* omega = nullptr;
+  double* weight = nullptr;
+  double* vol64 = nullptr;
+  double* volZ = nullptr;
+  int2* items = nullptr;
+  std::vector<int> levelOffset;  // (levels + 1)
+  // Fields.
+  void* S = nullptr;     // (G, C)
+  void* sigt = nullptr;  // (G, Z)
+  double* sigs = nullptr;
+  double* q = nullptr;
+  void* psi = nullptr;  // (G, C+nb, A)
+  double* phi = nullptr;
+  double* phibar = nullptr;
+  double* phibarOld = nullptr;
+  unsigned long long* bits = nullptr;
+  bool haveScattering = false;
+  std::vector<hipEvent_t> evLevel;  // 2 per level
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  size_t rsize() const { return precision == 32 ? sizeof(float) : sizeof(double); }
+  long psiStride() const { return (long)(C + nb) * G; }
+};
+
+namespace {
+
+template <typename Real>
+umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, int level) {
+  umt::SweepLevelArgs<Real> a{};
+  a.items = x->items + x->levelOffset[level];
+  a.nItems = x->levelOffset[level + 1] - x->levelOffset[level];
+  a.G = x->G;
+  a.C = x->C;
+  a.Afp = (const Real*)x->Afp;
+  a.Aez = (const Real*)x->Aez;
+  a.vol = (const Real*)x->vol;
+  a.cFP = x->cFP;
+  a.S = (const Real*)x->S;
+  a.sigt = (const Real*)x->sigt;
+  a.omega = x->omega;
+  a.psi = (Real*)x->psi;
+  a.psiAngleStride = x->psiStride();
+  return a;
+}
+
+template <typename Real>
+int sweep_impl(umt_ctx* x, bool timeLevels) {
+  const int L = (int)x->levelOffset.size() - 1;
+  for (int h = 0; h < L; ++h) {
+    if (timeLevels) HIP_TRY(hipEventRecord(x->evLevel[2 * h], x->stream));
+    HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, h), false, x->stream));
+    if (timeLevels) HIP_TRY(hipEventRecord(x->evLevel[2 * h + 1], x->stream));
+  }
+  HIP_TRY(umt::launch_phi_from_psi<Real>((const Real*)x->psi, x->weight, x->A, x->psiStride(),
+                                         (long)x->C * x->G, x->phi, x->stream));
+  return 0;
+}
+
+int sweep(umt_ctx* x, bool timeLevels) {
+  return x->precision == 32 ? sweep_impl<float>(x, timeLevels) : sweep_impl<double>(x, timeLevels);
+}
+
+// Upload a host FP64 array into a device array of the sweep precision.
+int upload_real(umt_ctx* x, void* dst, const double* src, size_t n) {
+  if (n == 0) return 0;
+  if (x->precision == 64) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* umt_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Problem construction
+
+int umt_problem_tiled(int nx, int ny, int nz, int npolar, int nazimuthal, int kdtree,
+                      umt_problem** out) {
+  if (!out) return fail(1, "null output pointer");
+  *out = nullptr;
+  umt_problem* p = new umt_problem();
+  try {
+    umt::build_tiled_mesh(nx, ny, nz, kdtree != 0, p->m);
+    umt::compute_geometry(p->m, p->g);
+    umt::product_quadrature(npolar, nazimuthal, p->q);
+    std::string err;
+    int rc = umt::build_schedule(p->m, p->g, p->q, p->s, err);
+    if (rc) {
+      delete p;
+      return fail(20 + rc, "schedule: %s", err.c_str());
+    }
+  } catch (const std::exception& e) {
+    delete p;
+    return fail(2, "problem construction: %s", e.what());
+  }
+  *out = p;
+  return 0;
+}
+
+void umt_problem_destroy(umt_problem* p) { delete p; }
+
+int umt_problem_dims(const umt_problem* p, int64_t* d) {
+  if (!p || !d) return fail(1, "null argument");
+  d[0] = p->m.nzones;
+  d[1] = p->m.ncorner;
+  d[2] = p->m.nbelem;
+  d[3] = p->q.nangles;
+  d[4] = p->s.maxPlanes;
+  d[5] = p->m.npoints;
+  d[6] = (int64_t)p->s.zonesInPlane.size();
+  d[7] = umt::kMaxCF;
+  return 0;
+}
+
+static void copy_out(const std::vector<int>& v, int* dst) {
+  if (dst) std::copy(v.begin(), v.end(), dst);
+}
+static void copy_out(const std::vector<double>& v, double* dst) {
+  if (dst) std::copy(v.begin(), v.end(), dst);
+}
+
+int umt_problem_mesh(const umt_problem* p, int* numCorner, int* cOffSet, int* nCFaces, int* cFP,
+                     int* cEZ, int* CToFace, int* zoneOpp, int* faceOpp, double* px) {
+  if (!p) return fail(1, "null problem");
+  copy_out(p->m.numCorner, numCorner);
+  copy_out(p->m.cOffSet, cOffSet);
+  copy_out(p->m.nCFaces, nCFaces);
+  copy_out(p->m.cFP, cFP);
+  copy_out(p->m.cEZ, cEZ);
+  copy_out(p->m.CToFace, CToFace);
+  copy_out(p->m.zoneOpp, zoneOpp);
+  copy_out(p->m.faceOpp, faceOpp);
+  copy_out(p->m.px, px);
+  return 0;
+}
+
+int umt_problem_geometry(const umt_problem* p, double* A_fp, double* A_ez, double* Volume,
+                         double* VolumeZone) {
+  if (!p) return fail(1, "null problem");
+  copy_out(p->g.A_fp, A_fp);
+  copy_out(p->g.A_ez, A_ez);
+  copy_out(p->g.Volume, Volume);
+  copy_out(p->g.VolumeZone, VolumeZone);
+  return 0;
+}
+
+int umt_problem_quadrature(const umt_problem* p, double* omega, double* weight) {
+  if (!p) return fail(1, "null problem");
+  copy_out(p->q.omega, omega);
+  copy_out(p->q.weight, weight);
+  return 0;
+}
+
+int umt_problem_schedule(const umt_problem* p, int* nHyperPlanes, int* zonesInPlane, int* nextZ,
+                         int* nextC) {
+  if (!p) return fail(1, "null problem");
+  copy_out(p->s.nHyperPlanes, nHyperPlanes);
+  copy_out(p->s.zonesInPlane, zonesInPlane);
+  copy_out(p->s.nextZ, nextZ);
+  copy_out(p->s.nextC, nextC);
+  return 0;
+}
+
+int umt_draw_cross_sections(uint64_t seed, int nzones, int groups, double sigt_lo, double sigt_hi,
+                            double c_lo, double c_hi, double q_lo, double q_hi, double* sigt,
+                            double* sigs, double* q) {
+  if (nzones < 0 || groups < 1) return fail(1, "bad sizes");
+  if (!(sigt_lo > 0.0) || !(sigt_hi >= sigt_lo)) return fail(1, "need 0 < sigt_lo <= sigt_hi");
+  const double l0 = std::log(sigt_lo), l1 = std::log(sigt_hi);
+  for (int z = 0; z < nzones; ++z)
+    for (int g = 0; g < groups; ++g) {
+      const size_t i = (size_t)z * groups + g;
+      const double st = std::exp(l0 + umt::draw_uniform(seed, z, g, 0) * (l1 - l0));
+      const double c = c_lo + umt::draw_uniform(seed, z, g, 1) * (c_hi - c_lo);
+      if (sigt) sigt[i] = st;
+      if (sigs) sigs[i] = c * st;
+      if (q) q[i] = q_lo + umt::draw_uniform(seed, z, g, 2) * (q_hi - q_lo);
+    }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident context
+
+void umt_ctx_destroy(umt_ctx* x) {
+  if (!x) return;
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  dfree(x->Afp); dfree(x->Aez); dfree(x->vol); dfree(x->cFP); dfree(x->omega); dfree(x->weight);
+  dfree(x->vol64); dfree(x->volZ); dfree(x->items); dfree(x->S); dfree(x->sigt); dfree(x->sigs);
+  dfree(x->q); dfree(x->psi); dfree(x->phi); dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
+  for (auto e : x->evLevel) (void)hipEventDestroy(e);
+  if (x->ev0) (void)hipEventDestroy(x->ev0);
+  if (x->ev1) (void)hipEventDestroy(x->ev1);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+}
+
+static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, const int* numCorner,
+                           const int* cOffSet, const int* nCFaces, const int* cFP, const int* cEZ,
+                           const double* A_fp, const double* A_ez, const double* Volume,
+                           const double* VolumeZone, const double* omega, const double* weight,
+                           const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ,
+                           umt_ctx* x) {
+  if (Z < 1 || C < 1 || nb < 0 || G < 1 || A < 1) return fail(1, "bad sizes");
+  if (precision != 32 && precision != 64) return fail(1, "precision must be 32 or 64");
+  if (int rc = check_hex_topology(Z, C, numCorner, cOffSet, nCFaces, cEZ)) return rc;
+  for (long i = 0; i < 3L * C; ++i)
+    if (cFP[i] < 1 || cFP[i] > C + nb) return fail(14, "cFP entry %ld out of range", i + 1);
+
+  // Hyperplane zone batches: level h holds plane h of every angle.
+  std::vector<int> planeStart(A + 1, 0);
+  int maxPlanes = 0;
+  for (int a = 0; a < A; ++a) {
+    planeStart[a + 1] = planeStart[a] + nHyperPlanes[a];
+    maxPlanes = std::max(maxPlanes, nHyperPlanes[a]);
+  }
+  std::vector<std::vector<int2>> lv(maxPlanes);
+  for (int a = 0; a < A; ++a) {
+    long done = 0;
+    for (int h = 0; h < nHyperPlanes[a]; ++h) {
+      const int n = zonesInPlane[planeStart[a] + h];
+      for (int i = 0; i < n; ++i) {
+        const int z = nextZ[(long)a * Z + done + i];
+        if (z <= 0 || z > Z) return fail(15, "angle %d: nextZ entry %d (cyclic or out of range)", a + 1, z);
+        lv[h].push_back(make_int2(z - 1, a));
+      }
+      done += n;
+    }
+    if (done != Z) return fail(16, "angle %d: hyperplanes hold %ld zones, mesh has %d", a + 1, done, Z);
+  }
+  std::vector<int2> items;
+  items.reserve((size_t)Z * A);
+  x->levelOffset.assign(1, 0);
+  for (int h = 0; h < maxPlanes; ++h) {
+    items.insert(items.end(), lv[h].begin(), lv[h].end());
+    x->levelOffset.push_back((int)items.size());
+  }
+
+  x->Z = Z; x->C = C; x->nb = nb; x->G = G; x->A = A; x->precision = precision;
+  HIP_TRY(hipStreamCreate(&x->stream));
+  HIP_TRY(hipEventCreate(&x->ev0));
+  HIP_TRY(hipEventCreate(&x->ev1));
+  x->evLevel.resize(2 * (size_t)maxPlanes);
+  for (auto& e : x->evLevel) HIP_TRY(hipEventCreate(&e));
+
+  const size_t rs = x->rsize();
+  const size_t nPsi = (size_t)(C + nb) * G * A;
+  int rc = 0;
+  if ((rc = dalloc((char**)&x->Afp, 9 * (size_t)C * rs)) || (rc = dalloc((char**)&x->Aez, 9 * (size_t)C * rs)) ||
+      (rc = dalloc((char**)&x->vol, (size_t)C * rs)) || (rc = dalloc(&x->cFP, 3 * (size_t)C)) ||
+      (rc = dalloc(&x->omega, 3 * (size_t)A)) || (rc = dalloc(&x->weight, (size_t)A)) ||
+      (rc = dalloc(&x->vol64, (size_t)C)) || (rc = dalloc(&x->volZ, (size_t)Z)) ||
+      (rc = dalloc(&x->items, items.size())) || (rc = dalloc((char**)&x->S, (size_t)G * C * rs)) ||
+      (rc = dalloc((char**)&x->sigt, (size_t)G * Z * rs)) || (rc = dalloc(&x->sigs, (size_t)G * Z)) ||
+      (rc = dalloc(&x->q, (size_t)G * Z)) || (rc = dalloc((char**)&x->psi, nPsi * rs)) ||
+      (rc = dalloc(&x->phi, (size_t)G * C)) || (rc = dalloc(&x->phibar, (size_t)G * Z)) ||
+      (rc = dalloc(&x->phibarOld, (size_t)G * Z)) || (rc = dalloc(&x->bits, 2)))
+    return rc;
+
+  if ((rc = upload_real(x, x->Afp, A_fp, 9 * (size_t)C)) || (rc = upload_real(x, x->Aez, A_ez, 9 * (size_t)C)) ||
+      (rc = upload_real(x, x->vol, Volume, (size_t)C)))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(x->cFP, cFP, 3 * (size_t)C * sizeof(int), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(x->omega, omega, 3 * (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(x->weight, weight, (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(x->vol64, Volume, (size_t)C * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(x->volZ, VolumeZone, (size_t)Z * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(x->items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, x->stream));
+  if (precision == 32) {
+    HIP_TRY(umt::launch_fill_zero<float>((float*)x->psi, (long)nPsi, x->stream));
+    HIP_TRY(umt::launch_fill_zero<float>((float*)x->S, (long)G * C, x->stream));
+    HIP_TRY(umt::launch_fill_zero<float>((float*)x->sigt, (long)G * Z, x->stream));
+  } else {
+    HIP_TRY(umt::launch_fill_zero<double>((double*)x->psi, (long)nPsi, x->stream));
+    HIP_TRY(umt::launch_fill_zero<double>((double*)x->S, (long)G * C, x->stream));
+    HIP_TRY(umt::launch_fill_zero<double>((double*)x->sigt, (long)G * Z, x->stream));
+  }
+  HIP_TRY(umt::launch_fill_zero<double>(x->phi, (long)G * C, x->stream));
+  HIP_TRY(umt::launch_fill_zero<double>(x->phibarOld, (long)G * Z, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_create(int nzones, int ncorner, int nbelem, int groups, int nangles, int precision,
+                   const int* numCorner, const int* cOffSet, const int* nCFaces, const int* cFP,
+                   const int* cEZ, const double* A_fp, const double* A_ez, const double* Volume,
+                   const double* VolumeZone, const double* omega, const double* weight,
+                   const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ,
+                   umt_ctx** out) {
+  if (!out) return fail(1, "null output pointer");
+  *out = nullptr;
+  umt_ctx* x = new umt_ctx();
+  int rc = ctx_create_impl(nzones, ncorner, nbelem, groups, nangles, precision, numCorner, cOffSet,
+                           nCFaces, cFP, cEZ, A_fp, A_ez, Volume, VolumeZone, omega, weight,
+                           nHyperPlanes, zonesInPlane, nextZ, x);
+  if (rc) {
+    std::string keep = g_err;
+    umt_ctx_destroy(x);
+    g_err = keep;
+    return rc;
+  }
+  *out = x;
+  return 0;
+}
+
+int umt_ctx_create_from_problem(const umt_problem* p, int groups, int precision, umt_ctx** out) {
+  if (!p) return fail(1, "null problem");
+  return umt_ctx_create(p->m.nzones, p->m.ncorner, p->m.nbelem, groups, p->q.nangles, precision,
+                        p->m.numCorner.data(), p->m.cOffSet.data(), p->m.nCFaces.data(),
+                        p->m.cFP.data(), p->m.cEZ.data(), p->g.A_fp.data(), p->g.A_ez.data(),
+                        p->g.Volume.data(), p->g.VolumeZone.data(), p->q.omega.data(),
+                        p->q.weight.data(), p->s.nHyperPlanes.data(), p->s.zonesInPlane.data(),
+                        p->s.nextZ.data(), out);
+}
+
+int umt_ctx_dims(const umt_ctx* x, int64_t* d) {
+  if (!x || !d) return fail(1, "null argument");
+  d[0] = x->Z; d[1] = x->C; d[2] = x->nb; d[3] = x->G; d[4] = x->A;
+  d[5] = (int64_t)x->levelOffset.size() - 1;
+  d[6] = x->levelOffset.back();
+  return 0;
+}
+
+int umt_ctx_set_sigt(umt_ctx* x, const double* sigt) {
+  if (!x || !sigt) return fail(1, "null argument");
+  for (long i = 0; i < (long)x->G * x->Z; ++i)
+    if (!(sigt[i] > 0.0) || !std::isfinite(sigt[i])) return fail(30, "Sigt entry %ld is not positive and finite", i + 1);
+  int rc = upload_real(x, x->sigt, sigt, (size_t)x->G * x->Z);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_set_source(umt_ctx* x, const double* S) {
+  if (!x || !S) return fail(1, "null argument");
+  for (long i = 0; i < (long)x->G * x->C; ++i)
+    if (!std::isfinite(S[i])) return fail(31, "STotal entry %ld is not finite", i + 1);
+  int rc = upload_real(x, x->S, S, (size_t)x->G * x->C);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_set_scattering(umt_ctx* x, const double* sigs, const double* q) {
+  if (!x || !sigs || !q) return fail(1, "null argument");
+  const long n = (long)x->G * x->Z;
+  for (long i = 0; i < n; ++i)
+    if (!(sigs[i] >= 0.0) || !std::isfinite(sigs[i]) || !std::isfinite(q[i]))
+      return fail(32, "sigs/q entry %ld is negative or not finite", i + 1);
+  HIP_TRY(hipMemcpyAsync(x->sigs, sigs, n * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(x->q, q, n * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  x->haveScattering = true;
+  return 0;
+}
+
+int umt_ctx_sweep(umt_ctx* x) {
+  if (!x) return fail(1, "null context");
+  if (int rc = sweep(x, false)) return rc;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_get_phi(const umt_ctx* x, double* phi) {
+  if (!x || !phi) return fail(1, "null argument");
+  HIP_TRY(hipMemcpyAsync(phi, x->phi, (size_t)x->G * x->C * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_set_phi(umt_ctx* x, const double* phi) {
+  if (!x || !phi) return fail(1, "null argument");
+  HIP_TRY(hipMemcpyAsync(x->phi, phi, (size_t)x->G * x->C * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_get_psi(const umt_ctx* x, double* psi) {
+  if (!x || !psi) return fail(1, "null argument");
+  const size_t n = (size_t)x->psiStride() * x->A;
+  if (x->precision == 64) {
+    HIP_TRY(hipMemcpyAsync(psi, x->psi, n * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), x->psi, n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+    for (size_t i = 0; i < n; ++i) psi[i] = tmp[i];
+  }
+  return 0;
+}
+
+int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, double* residual,
+                             double* hist) {
+  if (!x) return fail(1, "null context");
+  if (!x->haveScattering) return fail(33, "source iteration needs umt_ctx_set_scattering first");
+  const long nGC = (long)x->G * x->C, nGZ = (long)x->G * x->Z;
+  double res = 1.0;
+  int it = 0;
+  // Zone averages of the starting phi are the "old" values of iteration 1.
+  HIP_TRY(umt::launch_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld, 1e-6,
+                               x->bits, x->stream));
+  while (it < maxIters) {
+    if (x->precision == 32)
+      HIP_TRY(umt::launch_source_update<float>(x->phi, x->sigs, x->q, x->G, nGC, (float*)x->S, x->stream));
+    else
+      HIP_TRY(umt::launch_source_update<double>(x->phi, x->sigs, x->q, x->G, nGC, (double*)x->S, x->stream));
+    if (int rc = sweep(x, false)) return rc;
+    HIP_TRY(umt::launch_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld, 1e-6,
+                                 x->bits, x->stream));
+    unsigned long long b[2];
+    HIP_TRY(hipMemcpyAsync(b, x->bits, sizeof b, hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+    double r;
+    std::memcpy(&r, &b[1], sizeof r);
+    if (!std::isfinite(r)) return fail(34, "source iteration residual is not finite at iteration %d", it + 1);
+    res = r;
+    if (hist) hist[it] = r;
+    ++it;
+    if (res < tol) break;
+  }
+  if (iters) *iters = it;
+  if (residual) *residual = res;
+  return res < tol ? 0 : 1;
+}
+
+int umt_ctx_time_sweeps(umt_ctx* x, int warmup, int reps, double* msSweep, double* msLevels,
+                        int* launches) {
+  if (!x || reps < 1) return fail(1, "bad argument");
+  for (int i = 0; i < warmup; ++i)
+    if (int rc = sweep(x, false)) return rc;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  const int L = (int)x->levelOffset.size() - 1;
+  double sumLevels = 0.0;
+  HIP_TRY(hipEventRecord(x->ev0, x->stream));
+  for (int r = 0; r < reps; ++r) {
+    if (int rc = sweep(x, true)) return rc;
+    HIP_TRY(hipStreamSynchronize(x->stream));
+    for (int h = 0; h < L; ++h) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, x->evLevel[2 * h], x->evLevel[2 * h + 1]));
+      sumLevels += ms;
+    }
+  }
+  HIP_TRY(hipEventRecord(x->ev1, x->stream));
+  HIP_TRY(hipEventSynchronize(x->ev1));
+  float total = 0.f;
+  HIP_TRY(hipEventElapsedTime(&total, x->ev0, x->ev1));
+  if (msSweep) *msSweep = total / reps;
+  if (msLevels) *msLevels = sumLevels / reps;
+  if (launches) *launches = L;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in per-angle sweep
+
+namespace {
+
+struct DropIn {
+  hipStream_t stream = nullptr;
+  double *Afp = nullptr, *Aez = nullptr, *vol = nullptr, *S = nullptr, *sigt = nullptr,
+         *omega = nullptr, *psi1 = nullptr, *phi = nullptr, *tauPsi = nullptr;
+  int* cFP = nullptr;
+  int2* items = nullptr;
+  size_t capC = 0, capZ = 0, capG = 0, capNb = 0, capItems = 0;
+};
+
+constexpr int kMaxStreams = 80;
+std::mutex g_dropMutex;
+DropIn g_drop[kMaxStreams];
+
+int dropin_alloc(DropIn& d, size_t C, size_t Z, size_t G, size_t nb) {
+  if (C <= d.capC && Z <= d.capZ && G <= d.capG && nb <= d.capNb) return 0;
+  dfree(d.Afp); dfree(d.Aez); dfree(d.vol); dfree(d.S); dfree(d.sigt); dfree(d.omega);
+  dfree(d.psi1); dfree(d.phi); dfree(d.tauPsi); dfree(d.cFP); dfree(d.items);
+  d.capC = std::max(C, d.capC); d.capZ = std::max(Z, d.capZ);
+  d.capG = std::max(G, d.capG); d.capNb = std::max(nb, d.capNb);
+  C = d.capC; Z = d.capZ; G = d.capG; nb = d.capNb;
+  int rc;
+  if ((rc = dalloc(&d.Afp, 9 * C)) || (rc = dalloc(&d.Aez, 9 * C)) || (rc = dalloc(&d.vol, C)) ||
+      (rc = dalloc(&d.S, G * C)) || (rc = dalloc(&d.sigt, G * Z)) || (rc = dalloc(&d.omega, 3)) ||
+      (rc = dalloc(&d.psi1, G * (C + nb))) || (rc = dalloc(&d.phi, G * C)) ||
+      (rc = dalloc(&d.tauPsi, G * C)) || (rc = dalloc(&d.cFP, 3 * C)) || (rc = dalloc(&d.items, Z))) {
+    d.capC = d.capZ = d.capG = d.capNb = 0;
+    return rc;
+  }
+  d.capItems = Z;
+  return 0;
+}
+
+int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const double* STotal,
+                 double tau, double* Psi, int G, const double* Volume, const double* Sigt,
+                 const int* nCFaces, int ndim, int maxcf, int C, const double* A_fp,
+                 const double* omega, const int* cFP, double* Psi1, int nb, const double* A_ez,
+                 const int* cEZ, double quadwt, double* Phi, double* PsiB, int maxCorner,
+                 int streamId, int savePsi, int numCycles, int nBdyElem, int Mref,
+                 const int* numCorner, const int* cOffSet) {
+  if (ndim != 3) return fail(40, "only 3D (ndim = 3) is supported, got %d", ndim);
+  if (maxcf != 3 || maxCorner != 8) return fail(41, "maxcf = %d, maxCorner = %d (need 3 and 8)", maxcf, maxCorner);
+  if (numCycles != 0) return fail(42, "angle %d has %d cycle corners (unsupported)", Angle, numCycles);
+  if (nBdyElem > 0 && Mref >= 1) return fail(43, "reflecting boundaries are not supported");
+  if (nHP < 1 || G < 1 || C < 1 || nb < 0) return fail(44, "bad sizes");
+  long Zl = 0;
+  for (int h = 0; h < nHP; ++h) Zl += zip[h];
+  const int Z = (int)Zl;
+  if (int rc = check_hex_topology(Z, C, numCorner, cOffSet, nCFaces, cEZ)) return rc;
+  std::vector<int2> items(Z);
+  for (int i = 0; i < Z; ++i) {
+    if (nextZ[i] <= 0 || nextZ[i] > Z) return fail(45, "nextZ(%d) = %d (cyclic or out of range)", i + 1, nextZ[i]);
+    items[i] = make_int2(nextZ[i] - 1, 0);
+  }
+  for (long i = 0; i < 3L * C; ++i)
+    if (cFP[i] < 1 || cFP[i] > C + nb) return fail(46, "cFP entry %ld out of range", i + 1);
+
+  DropIn& d = g_drop[streamId % kMaxStreams];
+  if (!d.stream) HIP_TRY(hipStreamCreate(&d.stream));
+  if (int rc = dropin_alloc(d, C, Z, G, nb)) return rc;
+  hipStream_t s = d.stream;
+  const size_t GC = (size_t)G * C;
+  HIP_TRY(hipMemcpyAsync(d.Afp, A_fp, 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.Aez, A_ez, 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.vol, Volume, (size_t)C * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.cFP, cFP, 3 * (size_t)C * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.S, STotal, GC * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.sigt, Sigt, (size_t)G * Z * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.omega, omega, 3 * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.phi, Phi, GC * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.items, items.data(), (size_t)Z * sizeof(int2), hipMemcpyHostToDevice, s));
+  // The boundary tail of Psi1 starts from PsiB(:,:,Angle) (SweepUCBxyz.F90:95-97).
+  if (nb > 0) std::memcpy(Psi1 + GC, PsiB, (size_t)G * nb * 8);
+  HIP_TRY(hipMemcpyAsync(d.psi1, Psi1, (GC + (size_t)G * nb) * 8, hipMemcpyHostToDevice, s));
+  const bool useTau = tau != 0.0;
+  if (useTau) HIP_TRY(hipMemcpyAsync(d.tauPsi, Psi, GC * 8, hipMemcpyHostToDevice, s));
+
+  umt::SweepLevelArgs<double> a{};
+  a.G = G;
+  a.C = C;
+  a.Afp = d.Afp;
+  a.Aez = d.Aez;
+  a.vol = d.vol;
+  a.cFP = d.cFP;
+  a.S = d.S;
+  a.sigt = d.sigt;
+  a.omega = d.omega;
+  a.psi = d.psi1;
+  a.psiAngleStride = 0;
+  a.phi = d.phi;
+  a.quadwt = quadwt;
+  a.tauPsi = useTau ? d.tauPsi : nullptr;
+  a.tau = tau;
+  long done = 0;
+  for (int h = 0; h < nHP; ++h) {
+    a.items = d.items + done;
+    a.nItems = zip[h];
+    HIP_TRY(umt::launch_sweep_level<double>(a, true, s));
+    done += zip[h];
+  }
+  HIP_TRY(hipMemcpyAsync(Psi1, d.psi1, GC * 8, hipMemcpyDeviceToHost, s));
+  if (nb > 0) HIP_TRY(hipMemcpyAsync(PsiB, d.psi1 + GC, (size_t)G * nb * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(Phi, d.phi, GC * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (savePsi) std::memcpy(Psi, Psi1, GC * 8);
+  return 0;
+}
+
+}  // namespace
+
+void gpu_sweepucbxyz(int* Angle, int* nHyperPlanes, int* nZonesInPlane, int* nextZ, int* nextC,
+                     double* STotal, double* tau, double* Psi, int* Groups, double* Volume,
+                     double* Sigt, int* nCFacesArray, int* ndim, int* maxcf, int* ncorner,
+                     double* A_fp, double* omega, int* cFP, double* Psi1, int* nbelem, double* A_ez,
+                     int* cEZ, int* NumAngles, double* quadwt, double* Phi, double* PsiB,
+                     int* maxCorner, int* mem0solve1, int* streamIdPtr, int* totalStreams,
+                     int* savePsi, int* numCycles, int* cycleOffSet, double* cyclePsi,
+                     int* cycleList, int* b0, int* nBdyElem, double* PsiBMref, int* Mref,
+                     int* Geom_numCorner, int* Geom_cOffSet) {
+  (void)nextC; (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)cycleOffSet;
+  (void)cyclePsi; (void)cycleList; (void)b0; (void)PsiBMref;
+  std::lock_guard<std::mutex> lock(g_dropMutex);
+  g_dropin_status = dropin_sweep(*Angle, *nHyperPlanes, nZonesInPlane, nextZ, STotal, *tau, Psi,
+                                 *Groups, Volume, Sigt, nCFacesArray, *ndim, *maxcf, *ncorner, A_fp,
+                                 omega, cFP, Psi1, *nbelem, A_ez, cEZ, *quadwt, Phi, PsiB,
+                                 *maxCorner, *streamIdPtr, *savePsi, *numCycles, *nBdyElem, *Mref,
+                                 Geom_numCorner, Geom_cOffSet);
+  if (g_dropin_status) fprintf(stderr, "gpu_sweepucbxyz: %s\n", g_err.c_str());
+}
+
+void gpu_streamsynchronize(int* streamId) {
+  DropIn& d = g_drop[*streamId % kMaxStreams];
+  if (d.stream) (void)hipStreamSynchronize(d.stream);
+}
+
+void gpu_devicesynchronize(void) { (void)hipDeviceSynchronize(); }
+
+int gpu_sweepucbxyz_status(void) { return g_dropin_status; }
+
+}  // extern "C"

@@ -1,0 +1,350 @@
+// CDNA4 kernels for the upstream-corner-balance (UCB) xyz sweep.
+//
+// One lane owns one (zone, angle, group) triple and solves all eight corners
+// of that hex zone for that direction and group; consecutive lanes hold
+// consecutive groups, so every load of a corner's group vector
+// (S, psi, phi: groups fastest) is one contiguous run per zone. Work is handed
+// out per hyperplane level: a level holds every (zone, angle) pair whose
+// upstream zones lie in earlier levels, for all angles at once, so one launch
+// per level keeps the whole GPU busy.
+//
+// The corner arithmetic follows snac/SweepUCBxyz.F90:119-306: volume terms,
+// incoming FP-face fluxes, the EZ coupling term sez with fouralpha = 1.82,
+// and the per-corner solve psi_c = src_c / (sum of outgoing areas + sigma V_c).
+// The within-zone corner solve does not need the nextC order: corners are
+// relaxed in a fixed order, each pulling the EZ couplings of its upstream
+// neighbours, until every corner has been computed from final upstream
+// values (the result of a triangular solve does not depend on which
+// topological order produced it).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace umt {
+
+// Canonical hex topology produced by the tiled-mesh generator: EZ neighbour of
+// each local corner (0-based) for corner-face slots 0..2, following the
+// corner-face chaining of aux/setTetonZone.F90:172-189. The runtime checks
+// every zone against this table before launching these kernels.
+__device__ constexpr int kEZ[8][3] = {{3, 1, 4}, {0, 2, 5}, {1, 3, 6}, {2, 0, 7},
+                                      {5, 7, 0}, {6, 4, 1}, {7, 5, 2}, {4, 6, 3}};
+// The twelve hex edges, as (lower corner, slot in the lower corner).
+__device__ constexpr int kEdgeLo[12] = {0, 0, 0, 1, 1, 2, 2, 3, 4, 4, 5, 6};
+__device__ constexpr int kEdgeSlot[12] = {0, 1, 2, 1, 2, 1, 2, 2, 0, 1, 0, 0};
+// Edge id and orientation seen from (corner, slot): +1 if the corner is the
+// lower end (its A_ez is the stored one), -1 if it is the upper end.
+__device__ constexpr int kEdgeOf[8][3] = {{0, 1, 2}, {1, 3, 4},  {3, 5, 6},  {5, 0, 7},
+                                          {8, 9, 2}, {10, 8, 4}, {11, 10, 6}, {9, 11, 7}};
+__device__ constexpr int kEdgeSign[8][3] = {{1, 1, 1},   {-1, 1, 1},  {-1, 1, 1},
+                                            {-1, -1, 1}, {1, 1, -1},  {1, -1, -1},
+                                            {1, -1, -1}, {-1, -1, -1}};
+
+template <typename Real>
+__device__ __forceinline__ Real dot3(Real ox, Real oy, Real oz, const Real* a) {
+  return ox * a[0] + oy * a[1] + oz * a[2];
+}
+
+template <typename Real, bool kSingleAngle>
+__global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int item = (int)(tid / p.G);
+  const int g = (int)(tid - (long)item * p.G);
+  if (item >= p.nItems) return;
+  const int2 it = p.items[item];
+  const int z = it.x, a = it.y;
+  const int G = p.G;
+  const long c0 = 8L * z;
+  const Real ox = (Real)p.omega[3 * a + 0];
+  const Real oy = (Real)p.omega[3 * a + 1];
+  const Real oz = (Real)p.omega[3 * a + 2];
+  const long angleBase = kSingleAngle ? 0L : (long)a * p.psiAngleStride;
+  Real* __restrict__ psiA = p.psi + angleBase;
+
+  const Real sig = p.sigt[(long)z * G + g];
+  Real Q[8], src[8], den[8], vol[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    vol[c] = p.vol[c0 + c];
+    Real q = p.S[(c0 + c) * G + g];
+    if (kSingleAngle && p.tauPsi) q += p.tau * p.tauPsi[(c0 + c) * G + g];
+    Q[c] = q;
+    src[c] = vol[c] * q;
+  }
+  Real ed[12];
+#pragma unroll
+  for (int e = 0; e < 12; ++e)
+    ed[e] = dot3(ox, oy, oz, p.Aez + 3 * (3 * (c0 + kEdgeLo[e]) + kEdgeSlot[e]));
+
+  const Real fouralpha = (Real)1.82;
+  const Real half = (Real)0.5;
+  unsigned exitMask = 0;  // bit 3c+f: outgoing corner face on the problem boundary
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    Real afp[3], psifp[3];
+    Real sumArea = 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      afp[f] = dot3(ox, oy, oz, p.Afp + 3 * (3 * (c0 + c) + f));
+      psifp[f] = 0;
+      const int cfp = p.cFP[3 * (c0 + c) + f] - 1;
+      if (afp[f] > 0) {
+        sumArea += afp[f];
+        if (cfp >= p.C) exitMask |= 1u << (3 * c + f);
+      } else if (afp[f] < 0) {
+        psifp[f] = psiA[(long)cfp * G + g];
+        src[c] = src[c] - afp[f] * psifp[f];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const Real aez = kEdgeSign[c][f] > 0 ? ed[kEdgeOf[c][f]] : -ed[kEdgeOf[c][f]];
+      if (aez > 0) {
+        const int cez = kEZ[c][f];
+        sumArea += aez;
+        const int ifp = (f + 1) % 3;
+        Real sez;
+        if (afp[ifp] < 0) {
+          const Real psiOpp = psifp[ifp];
+          const Real aez2 = aez * aez;
+          const Real v = vol[c];
+          const Real sigv = sig * v;
+          const Real sigv2 = sigv * sigv;
+          const Real gnum = aez2 * (fouralpha * sigv2 + aez * ((Real)4 * sigv + (Real)3 * aez));
+          const Real gden = v * ((Real)4 * sigv * sigv2 +
+                                 aez * ((Real)6 * sigv2 + (Real)2 * aez * ((Real)2 * sigv + aez)));
+          sez = (v * gnum * (sig * psiOpp - Q[c]) + half * aez * gden * (Q[c] - Q[cez])) /
+                (gnum + gden * sig);
+        } else {
+          const Real sigInv = (Real)1 / sig;
+          sez = half * aez * sigInv * (Q[c] - Q[cez]);
+        }
+        src[c] += sez;
+        src[cez] -= sez;
+      }
+    }
+    den[c] = sumArea + sig * vol[c];
+  }
+
+  // Corner solve: relax in static order until every corner saw final
+  // upstream values. A corner with no upstream EZ neighbour is final after
+  // its first visit; a chain of k backward edges needs k+1 passes (<= 8).
+  Real psi[8], rden[8];
+  unsigned fin = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    psi[c] = 0;
+    rden[c] = (Real)1 / den[c];
+  }
+  for (int pass = 0; pass < 8 && fin != 0xffu; ++pass) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      Real acc = src[c];
+      bool ok = true;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        const Real aez = kEdgeSign[c][f] > 0 ? ed[kEdgeOf[c][f]] : -ed[kEdgeOf[c][f]];
+        if (aez < 0) {
+          const int nb = kEZ[c][f];
+          acc += (-aez) * psi[nb];
+          ok = ok && ((fin >> nb) & 1u);
+        }
+      }
+      psi[c] = acc * rden[c];
+      fin |= ok ? (1u << c) : 0u;
+    }
+  }
+  // Every upstream value is final now: one more visit with the division
+  // psi_c = src_c / (sumArea_c + sigma V_c) of SweepUCBxyz.F90:267.
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    Real acc = src[c];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const Real aez = kEdgeSign[c][f] > 0 ? ed[kEdgeOf[c][f]] : -ed[kEdgeOf[c][f]];
+      if (aez < 0) acc += (-aez) * psi[kEZ[c][f]];
+    }
+    psi[c] = acc / den[c];
+  }
+
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    psiA[(c0 + c) * G + g] = psi[c];
+    if (kSingleAngle) p.phi[(c0 + c) * G + g] += p.quadwt * (double)psi[c];
+  }
+  // Exiting boundary fluxes go to the boundary tail of this angle's block.
+  if (exitMask) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        if (exitMask & (1u << (3 * c + f))) {
+          const int cfp = p.cFP[3 * (c0 + c) + f] - 1;
+          psiA[(long)cfp * G + g] = psi[c];
+        }
+  }
+}
+
+// phi(g,c) = sum_a w_a psi(g,c,a), accumulated in FP64 in angle order.
+template <typename Real>
+__global__ __launch_bounds__(256) void phi_from_psi(const Real* __restrict__ psi,
+                                                    const double* __restrict__ weight, int A,
+                                                    long angleStride, long n,
+                                                    double* __restrict__ phi) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int a = 0; a < A; ++a) acc += weight[a] * (double)psi[(long)a * angleStride + i];
+  phi[i] = acc;
+}
+
+// Source for the next sweep: S = (sigma_s phi + q) / (4 pi), within-group
+// isotropic scattering (getCollisionRate semantics for a steady problem).
+template <typename Real>
+__global__ __launch_bounds__(256) void source_update(const double* __restrict__ phi,
+                                                     const double* __restrict__ sigs,
+                                                     const double* __restrict__ q, int G,
+                                                     long n, Real* __restrict__ S) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long c = i / G;
+  const int g = (int)(i - c * G);
+  const long zg = (c >> 3) * G + g;
+  const double wtiso = 0.07957747154594767;  // 1/(4 pi)
+  S[i] = (Real)(wtiso * (sigs[zg] * phi[i] + q[zg]));
+}
+
+// Zone averages phibar(g,z) = sum_c V_c phi(g,c) / V_z and the running max.
+__global__ __launch_bounds__(256) void zone_average(const double* __restrict__ phi,
+                                                    const double* __restrict__ vol,
+                                                    const double* __restrict__ volZ, int G,
+                                                    long n, double* __restrict__ phibar,
+                                                    unsigned long long* __restrict__ maxBits) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    const long z = i / G;
+    const int g = (int)(i - z * G);
+    double acc = 0.0;
+    for (int c = 0; c < 8; ++c) acc += phi[(8 * z + c) * G + g] * vol[8 * z + c];
+    v = acc / volZ[z];
+    phibar[i] = v;
+  }
+  // Wave-level max of non-negative values, then one atomic per wave.
+  double m = v > 0.0 ? v : 0.0;
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxBits, (unsigned long long)__double_as_longlong(m));
+}
+
+// Max relative change of zone averages over entries above cutoff * max
+// (rt/ConvergenceTest.F90:150-168 semantics per zone and group), and copy of
+// the new averages into the old.
+__global__ __launch_bounds__(256) void residual_max(const double* __restrict__ phibar,
+                                                    double* __restrict__ phibarOld, long n,
+                                                    const unsigned long long* __restrict__ maxBits,
+                                                    double cutoff,
+                                                    unsigned long long* __restrict__ resBits) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (i < n) {
+    const double thr = cutoff * __longlong_as_double((long long)*maxBits);
+    const double v = phibar[i];
+    if (v > thr) r = fabs((v - phibarOld[i]) / v);
+    phibarOld[i] = v;
+  }
+  for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_down(r, off, 64));
+  if ((threadIdx.x & 63) == 0 && r > 0.0) atomicMax(resBits, (unsigned long long)__double_as_longlong(r));
+}
+
+__global__ void fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_zero(T* p, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+
+template <typename Dst, typename Src>
+__global__ __launch_bounds__(256) void convert_copy(const Src* __restrict__ s, Dst* __restrict__ d,
+                                                    long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (Dst)s[i];
+}
+
+// ----------------------------------------------------------------------------
+// Launch wrappers (host side).
+
+static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename Real>
+hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
+  const long threads = (long)a.nItems * a.G;
+  if (threads == 0) return hipSuccess;
+  if (singleAngle)
+    hipLaunchKernelGGL((ucb_sweep_level<Real, true>), dim3(blocks_for(threads, 256)), dim3(256), 0,
+                       s, a);
+  else
+    hipLaunchKernelGGL((ucb_sweep_level<Real, false>), dim3(blocks_for(threads, 256)), dim3(256),
+                       0, s, a);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_phi_from_psi(const Real* psi, const double* w, int A, long stride, long n,
+                               double* phi, hipStream_t s) {
+  hipLaunchKernelGGL((phi_from_psi<Real>), dim3(blocks_for(n, 256)), dim3(256), 0, s, psi, w, A,
+                     stride, n, phi);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_source_update(const double* phi, const double* sigs, const double* q, int G,
+                                long n, Real* S, hipStream_t s) {
+  hipLaunchKernelGGL((source_update<Real>), dim3(blocks_for(n, 256)), dim3(256), 0, s, phi, sigs,
+                     q, G, n, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const double* phi, const double* vol, const double* volZ, int G, long nzg,
+                           double* phibar, double* phibarOld, double cutoff,
+                           unsigned long long* bits, hipStream_t s) {
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, bits, 2, 0ull);
+  hipLaunchKernelGGL(zone_average, dim3(blocks_for(nzg, 256)), dim3(256), 0, s, phi, vol, volZ, G,
+                     nzg, phibar, bits);
+  hipLaunchKernelGGL(residual_max, dim3(blocks_for(nzg, 256)), dim3(256), 0, s, phibar, phibarOld,
+                     nzg, bits, cutoff, bits + 1);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill_zero(T* p, long n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((fill_zero<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+template <typename Dst, typename Src>
+hipError_t launch_convert(const Src* s_, Dst* d, long n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((convert_copy<Dst, Src>), dim3(blocks_for(n, 256)), dim3(256), 0, s, s_, d, n);
+  return hipGetLastError();
+}
+
+template hipError_t launch_sweep_level<double>(const SweepLevelArgs<double>&, bool, hipStream_t);
+template hipError_t launch_sweep_level<float>(const SweepLevelArgs<float>&, bool, hipStream_t);
+template hipError_t launch_phi_from_psi<double>(const double*, const double*, int, long, long,
+                                                double*, hipStream_t);
+template hipError_t launch_phi_from_psi<float>(const float*, const double*, int, long, long, double*,
+                                               hipStream_t);
+template hipError_t launch_source_update<double>(const double*, const double*, const double*, int,
+                                                 long, double*, hipStream_t);
+template hipError_t launch_source_update<float>(const double*, const double*, const double*, int,
+                                                long, float*, hipStream_t);
+template hipError_t launch_fill_zero<double>(double*, long, hipStream_t);
+template hipError_t launch_fill_zero<float>(float*, long, hipStream_t);
+template hipError_t launch_convert<float, double>(const double*, float*, long, hipStream_t);
+template hipError_t launch_convert<double, double>(const double*, double*, long, hipStream_t);
+
+}  // namespace umt

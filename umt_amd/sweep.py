@@ -1,0 +1,252 @@
+"""Python host side of the MI355X UCB sweep: problem construction, the
+device-resident sweep context and the per-angle drop-in entry point.
+
+This mirrors the reference's sweep interface for the parity tests:
+``SweepContext.sweep`` is one pass of snac/SetSweep.F90:100-170 (all angles
+of a single domain, vacuum boundaries), ``sweep_ucbxyz`` is
+snac/SweepUCBxyz.F90:11 for one angle through the C-ABI that replaces
+gpu/GPU_SweepUCBxyz.cu:532, and ``SweepContext.source_iteration`` is the
+steady source iteration of rt/rtmainsn.F90:167-235 with the convergence
+measure of rt/ConvergenceTest.F90:150-168.
+
+Numpy shapes are the Teton Fortran shapes reversed (row-major): Teton
+``Phi(G, C)`` is a numpy array of shape ``(C, G)``; index contents stay
+1-based.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+_pi = C.POINTER(C.c_int)
+_pd = C.POINTER(C.c_double)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_pi) if a is not None else None
+
+
+def _dp(a):
+    return a.ctypes.data_as(_pd) if a is not None else None
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class TiledProblem:
+    """Tiled hex mesh of [0,1]^3 with geometry, quadrature and schedule."""
+
+    def __init__(self, nx, ny, nz, npolar, nazimuthal, kdtree=True):
+        lib = _lib.load()
+        h = C.c_void_p()
+        check("umt_problem_tiled",
+              lib.umt_problem_tiled(nx, ny, nz, npolar, nazimuthal, int(kdtree), C.byref(h)))
+        self._h = h
+        d = np.zeros(8, dtype=np.int64)
+        check("umt_problem_dims", lib.umt_problem_dims(h, d.ctypes.data_as(C.POINTER(C.c_int64))))
+        (self.nzones, self.ncorner, self.nbelem, self.nangles, self.max_planes, self.npoints,
+         self.total_planes, self.maxcf) = (int(x) for x in d)
+        self.tiles = (nx, ny, nz)
+        self.npolar, self.nazimuthal = npolar, nazimuthal
+        self._mesh = self._geom = self._quad = self._sched = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().umt_problem_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def mesh(self):
+        if self._mesh is None:
+            Z, Cn = self.nzones, self.ncorner
+            m = dict(numCorner=np.zeros(Z, np.int32), cOffSet=np.zeros(Z, np.int32),
+                     nCFaces=np.zeros(Cn, np.int32), cFP=np.zeros((Cn, 3), np.int32),
+                     cEZ=np.zeros((Cn, 3), np.int32), CToFace=np.zeros((Cn, 3), np.int32),
+                     zoneOpp=np.zeros((Z, 6), np.int32), faceOpp=np.zeros((Z, 6), np.int32),
+                     px=np.zeros((Cn, 3), np.float64))
+            check("umt_problem_mesh", _lib.load().umt_problem_mesh(
+                self._h, _ip(m["numCorner"]), _ip(m["cOffSet"]), _ip(m["nCFaces"]), _ip(m["cFP"]),
+                _ip(m["cEZ"]), _ip(m["CToFace"]), _ip(m["zoneOpp"]), _ip(m["faceOpp"]),
+                _dp(m["px"])))
+            m["zoneFaces"] = np.full(Z, 6, np.int32)
+            self._mesh = m
+        return self._mesh
+
+    def geometry(self):
+        if self._geom is None:
+            Z, Cn = self.nzones, self.ncorner
+            g = dict(A_fp=np.zeros((Cn, 3, 3)), A_ez=np.zeros((Cn, 3, 3)), Volume=np.zeros(Cn),
+                     VolumeZone=np.zeros(Z))
+            check("umt_problem_geometry", _lib.load().umt_problem_geometry(
+                self._h, _dp(g["A_fp"]), _dp(g["A_ez"]), _dp(g["Volume"]), _dp(g["VolumeZone"])))
+            self._geom = g
+        return self._geom
+
+    def quadrature(self):
+        if self._quad is None:
+            q = dict(omega=np.zeros((self.nangles, 3)), weight=np.zeros(self.nangles))
+            check("umt_problem_quadrature", _lib.load().umt_problem_quadrature(
+                self._h, _dp(q["omega"]), _dp(q["weight"])))
+            self._quad = q
+        return self._quad
+
+    def schedule(self):
+        if self._sched is None:
+            A, Z, Cn = self.nangles, self.nzones, self.ncorner
+            s = dict(nHyperPlanes=np.zeros(A, np.int32),
+                     zonesInPlane=np.zeros(self.total_planes, np.int32),
+                     nextZ=np.zeros((A, Z), np.int32), nextC=np.zeros((A, Cn), np.int32))
+            check("umt_problem_schedule", _lib.load().umt_problem_schedule(
+                self._h, _ip(s["nHyperPlanes"]), _ip(s["zonesInPlane"]), _ip(s["nextZ"]),
+                _ip(s["nextC"])))
+            self._sched = s
+        return self._sched
+
+    def unknowns(self, groups):
+        """Unknowns per sweep: corners x angles x groups (driver/test_driver.cc:1895)."""
+        return self.ncorner * self.nangles * groups
+
+
+def draw_cross_sections(seed, nzones, groups, sigt=(0.1, 10.0), c=(0.0, 0.9), q=(1.0, 1.0)):
+    """Seeded (zone, group) draws: returns sigt, sigs, q as (Z, G) arrays."""
+    st = np.zeros((nzones, groups))
+    ss = np.zeros((nzones, groups))
+    qq = np.zeros((nzones, groups))
+    check("umt_draw_cross_sections", _lib.load().umt_draw_cross_sections(
+        seed, nzones, groups, sigt[0], sigt[1], c[0], c[1], q[0], q[1], _dp(st), _dp(ss), _dp(qq)))
+    return st, ss, qq
+
+
+class SweepContext:
+    """Device-resident sweep of all angles (precision 64 or 32)."""
+
+    def __init__(self, problem, groups, precision=64):
+        lib = _lib.load()
+        h = C.c_void_p()
+        check("umt_ctx_create_from_problem",
+              lib.umt_ctx_create_from_problem(problem.handle, groups, precision, C.byref(h)))
+        self._h = h
+        self.problem = problem
+        self.G = groups
+        self.precision = precision
+        d = np.zeros(7, dtype=np.int64)
+        check("umt_ctx_dims", lib.umt_ctx_dims(h, d.ctypes.data_as(C.POINTER(C.c_int64))))
+        self.levels = int(d[5])
+        self.items = int(d[6])
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().umt_ctx_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_sigt(self, sigt):
+        a = _f64(sigt)
+        check("umt_ctx_set_sigt", _lib.load().umt_ctx_set_sigt(self._h, _dp(a)))
+
+    def set_source(self, S):
+        a = _f64(S)
+        check("umt_ctx_set_source", _lib.load().umt_ctx_set_source(self._h, _dp(a)))
+
+    def set_scattering(self, sigs, q):
+        a, b = _f64(sigs), _f64(q)
+        check("umt_ctx_set_scattering", _lib.load().umt_ctx_set_scattering(self._h, _dp(a), _dp(b)))
+
+    def sweep(self):
+        check("umt_ctx_sweep", _lib.load().umt_ctx_sweep(self._h))
+
+    def phi(self):
+        out = np.zeros((self.problem.ncorner, self.G))
+        check("umt_ctx_get_phi", _lib.load().umt_ctx_get_phi(self._h, _dp(out)))
+        return out
+
+    def set_phi(self, phi):
+        a = _f64(phi)
+        check("umt_ctx_set_phi", _lib.load().umt_ctx_set_phi(self._h, _dp(a)))
+
+    def psi(self):
+        """psi(G, C+nbelem, A) as a numpy array of shape (A, C+nbelem, G)."""
+        p = self.problem
+        out = np.zeros((p.nangles, p.ncorner + p.nbelem, self.G))
+        check("umt_ctx_get_psi", _lib.load().umt_ctx_get_psi(self._h, _dp(out)))
+        return out
+
+    def source_iteration(self, tol=1e-8, max_iters=1000):
+        """Returns (converged, iterations, residual, residual_history)."""
+        it = C.c_int(0)
+        res = C.c_double(0.0)
+        hist = np.zeros(max(max_iters, 1))
+        rc = _lib.load().umt_ctx_source_iteration(self._h, tol, max_iters, C.byref(it),
+                                                  C.byref(res), _dp(hist))
+        if rc not in (0, 1):
+            check("umt_ctx_source_iteration", rc)
+        return rc == 0, it.value, res.value, hist[:it.value].copy()
+
+    def time_sweeps(self, warmup=2, reps=10):
+        """Returns (ms per sweep, ms of level kernels per sweep, level launches per sweep)."""
+        ms = C.c_double(0.0)
+        msl = C.c_double(0.0)
+        n = C.c_int(0)
+        check("umt_ctx_time_sweeps", _lib.load().umt_ctx_time_sweeps(
+            self._h, warmup, reps, C.byref(ms), C.byref(msl), C.byref(n)))
+        return ms.value, msl.value, n.value
+
+
+def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, Psi_angle=None,
+                 save_psi=False, stream_id=0):
+    """One angle through the drop-in C-ABI ``gpu_sweepucbxyz``.
+
+    ``angle`` is 1-based. Psi1 (C+nbelem, G), PsiB_angle (nbelem, G) and Phi
+    (C, G) are updated in place, as the Fortran caller's arrays would be.
+    Returns the library status (0 on success).
+    """
+    lib = _lib.load()
+    m, g, q, s = problem.mesh(), problem.geometry(), problem.quadrature(), problem.schedule()
+    a = angle - 1
+    start = int(np.sum(s["nHyperPlanes"][:a]))
+    nhp = int(s["nHyperPlanes"][a])
+    zip_ = _i32(s["zonesInPlane"][start:start + nhp])
+    nextZ = _i32(s["nextZ"][a])
+    nextC = _i32(s["nextC"][a])
+    G = STotal.shape[1]
+    omega = _f64(q["omega"][a])
+    ints = {k: C.c_int(v) for k, v in dict(
+        Angle=angle, nHP=nhp, Groups=G, ndim=3, maxcf=3, ncorner=problem.ncorner,
+        nbelem=problem.nbelem, NumAngles=problem.nangles, maxCorner=8, mem0solve1=0,
+        streamId=stream_id, totalStreams=1, savePsi=int(save_psi), numCycles=0, cycleOffSet=0,
+        b0=0, nBdyElem=0, Mref=-1).items()}
+    tau_c = C.c_double(tau)
+    wt = C.c_double(q["weight"][a])
+    Psi = Psi_angle if Psi_angle is not None else np.zeros((problem.ncorner, G))
+    dummy_d = np.zeros(1)
+    dummy_i = np.zeros(1, np.int32)
+    for arr in (STotal, Sigt, Psi1, PsiB_angle, Phi, Psi):
+        assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"]
+    lib.gpu_sweepucbxyz(
+        C.byref(ints["Angle"]), C.byref(ints["nHP"]), _ip(zip_), _ip(nextZ), _ip(nextC),
+        _dp(STotal), C.byref(tau_c), _dp(Psi), C.byref(ints["Groups"]), _dp(g["Volume"]),
+        _dp(Sigt), _ip(m["nCFaces"]), C.byref(ints["ndim"]), C.byref(ints["maxcf"]),
+        C.byref(ints["ncorner"]), _dp(g["A_fp"]), _dp(omega), _ip(m["cFP"]), _dp(Psi1),
+        C.byref(ints["nbelem"]), _dp(g["A_ez"]), _ip(m["cEZ"]), C.byref(ints["NumAngles"]),
+        C.byref(wt), _dp(Phi), _dp(PsiB_angle), C.byref(ints["maxCorner"]),
+        C.byref(ints["mem0solve1"]), C.byref(ints["streamId"]), C.byref(ints["totalStreams"]),
+        C.byref(ints["savePsi"]), C.byref(ints["numCycles"]), C.byref(ints["cycleOffSet"]),
+        _dp(dummy_d), _ip(dummy_i), C.byref(ints["b0"]), C.byref(ints["nBdyElem"]),
+        _dp(dummy_d), C.byref(ints["Mref"]), _ip(m["numCorner"]), _ip(m["cOffSet"]))
+    sid = C.c_int(stream_id)
+    lib.gpu_streamsynchronize(C.byref(sid))
+    return lib.gpu_sweepucbxyz_status()
