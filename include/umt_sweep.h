@@ -153,6 +153,19 @@ int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
 int umt_ctx_time_sweeps(umt_ctx *ctx, int warmup, int reps, double *ms_sweep,
                         double *ms_level_kernels, int *launches);
 
+/* n sweeps back to back on the context's stream, one synchronisation at the end. */
+int umt_ctx_sweep_n(umt_ctx *ctx, int n);
+
+/* Mean incoming interior FP corner faces per (corner, angle): the n_in of
+ * the algorithmic-bytes model (Omega.A_fp < 0 towards a corner, not a
+ * boundary element), counted over all corners and angles. */
+int umt_ctx_incoming_faces(const umt_ctx *ctx, double *n_in);
+
+/* Calibration probe for HBM byte counters (no counterpart in the
+ * reference): copies n doubles device-to-device reps times with a plain
+ * streaming kernel (umt_copy_probe) and returns the mean ms per copy. */
+int umt_hbm_copy_probe(int64_t n, int reps, double *ms);
+
 /* dims[0..6] = nzones, ncorner, nbelem, groups, nangles, levels, items. */
 int umt_ctx_dims(const umt_ctx *ctx, int64_t *dims);
 

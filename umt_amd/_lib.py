@@ -51,6 +51,9 @@ SIGNATURES = {
     "umt_ctx_source_iteration": (_i, [_vp, _d, _i, _pi, _pd, _pd]),
     "umt_ctx_time_sweeps": (_i, [_vp, _i, _i, _pd, _pd, _pi]),
     "umt_ctx_dims": (_i, [_vp, _pi64]),
+    "umt_ctx_sweep_n": (_i, [_vp, _i]),
+    "umt_ctx_incoming_faces": (_i, [_vp, _pd]),
+    "umt_hbm_copy_probe": (_i, [_i64, _i, _pd]),
 }
 
 _lib = None

@@ -39,6 +39,7 @@ hipError_t launch_residual(const double* phi, const double* vol, const double* v
                            unsigned long long* bits, hipStream_t s);
 template <typename T>
 hipError_t launch_fill_zero(T* p, long n, hipStream_t s);
+hipError_t launch_copy_probe(const double* src, double* dst, long n, hipStream_t s);
 template <typename Dst, typename Src>
 hipError_t launch_convert(const Src* s, Dst* d, long n, hipStream_t s_);
 

@@ -111,6 +111,7 @@ struct umt_ctx {
   double* phibarOld = nullptr;
   unsigned long long* bits = nullptr;
   bool haveScattering = false;
+  double nIn = 0.0;  // mean incoming interior FP faces per (corner, angle)
   std::vector<hipEvent_t> evLevel;  // 2 per level
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -333,6 +334,19 @@ static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, co
     }
     if (done != Z) return fail(16, "angle %d: hyperplanes hold %ld zones, mesh has %d", a + 1, done, Z);
   }
+  // n_in of the algorithmic-bytes model, counted exactly.
+  {
+    long long cnt = 0;
+#pragma omp parallel for reduction(+ : cnt) schedule(static)
+    for (int a = 0; a < A; ++a) {
+      const double* o = omega + 3L * a;
+      for (long i = 0; i < 3L * C; ++i) {
+        const double* v = A_fp + 3 * i;
+        if (o[0] * v[0] + o[1] * v[1] + o[2] * v[2] < 0.0 && cFP[i] <= C) ++cnt;
+      }
+    }
+    x->nIn = (double)cnt / ((double)C * A);
+  }
   std::vector<int2> items;
   items.reserve((size_t)Z * A);
   x->levelOffset.assign(1, 0);
@@ -464,6 +478,49 @@ int umt_ctx_sweep(umt_ctx* x) {
   if (int rc = sweep(x, false)) return rc;
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
+}
+
+int umt_ctx_sweep_n(umt_ctx* x, int n) {
+  if (!x || n < 0) return fail(1, "bad argument");
+  for (int i = 0; i < n; ++i)
+    if (int rc = sweep(x, false)) return rc;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_incoming_faces(const umt_ctx* x, double* n_in) {
+  if (!x || !n_in) return fail(1, "null argument");
+  *n_in = x->nIn;
+  return 0;
+}
+
+int umt_hbm_copy_probe(int64_t n, int reps, double* ms) {
+  if (n < 1 || reps < 1 || !ms) return fail(1, "bad argument");
+  double *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = dalloc(&a, (size_t)n);
+  if (!rc) rc = dalloc(&b, (size_t)n);
+  if (!rc) {
+    HIP_TRY(hipStreamCreate(&s));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(umt::launch_fill_zero<double>(a, (long)n, s));
+    HIP_TRY(umt::launch_copy_probe(a, b, (long)n, s));
+    HIP_TRY(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) HIP_TRY(umt::launch_copy_probe(a, b, (long)n, s));
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  dfree(a);
+  dfree(b);
+  return rc;
 }
 
 int umt_ctx_get_phi(const umt_ctx* x, double* phi) {

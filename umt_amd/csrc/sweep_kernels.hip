@@ -18,6 +18,8 @@
 // topological order produced it).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace umt {
@@ -266,6 +268,13 @@ __global__ __launch_bounds__(256) void fill_zero(T* p, long n) {
   if (i < n) p[i] = T(0);
 }
 
+// Streaming copy used only to calibrate HBM byte counters.
+__global__ __launch_bounds__(256) void umt_copy_probe(const double* __restrict__ s,
+                                                      double* __restrict__ d, long n) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) d[i] = s[i];
+}
+
 template <typename Dst, typename Src>
 __global__ __launch_bounds__(256) void convert_copy(const Src* __restrict__ s, Dst* __restrict__ d,
                                                     long n) {
@@ -322,6 +331,12 @@ template <typename T>
 hipError_t launch_fill_zero(T* p, long n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL((fill_zero<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_probe(const double* src, double* dst, long n, hipStream_t s) {
+  const long blocks = std::min<long>(blocks_for(n, 256), 256L * 32);
+  hipLaunchKernelGGL(umt_copy_probe, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
   return hipGetLastError();
 }
 

@@ -169,6 +169,14 @@ class SweepContext:
     def sweep(self):
         check("umt_ctx_sweep", _lib.load().umt_ctx_sweep(self._h))
 
+    def sweep_n(self, n):
+        check("umt_ctx_sweep_n", _lib.load().umt_ctx_sweep_n(self._h, n))
+
+    def incoming_faces(self):
+        v = C.c_double(0.0)
+        check("umt_ctx_incoming_faces", _lib.load().umt_ctx_incoming_faces(self._h, C.byref(v)))
+        return v.value
+
     def phi(self):
         out = np.zeros((self.problem.ncorner, self.G))
         check("umt_ctx_get_phi", _lib.load().umt_ctx_get_phi(self._h, _dp(out)))
@@ -204,6 +212,13 @@ class SweepContext:
         check("umt_ctx_time_sweeps", _lib.load().umt_ctx_time_sweeps(
             self._h, warmup, reps, C.byref(ms), C.byref(msl), C.byref(n)))
         return ms.value, msl.value, n.value
+
+
+def hbm_copy_probe(n_doubles, reps=10):
+    """Mean ms of one device-to-device copy of n_doubles FP64 values."""
+    ms = C.c_double(0.0)
+    check("umt_hbm_copy_probe", _lib.load().umt_hbm_copy_probe(n_doubles, reps, C.byref(ms)))
+    return ms.value
 
 
 def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, Psi_angle=None,
