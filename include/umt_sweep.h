@@ -153,6 +153,17 @@ int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
 int umt_ctx_time_sweeps(umt_ctx *ctx, int warmup, int reps, double *ms_sweep,
                         double *ms_level_kernels, int *launches);
 
+/* Groups each lane of the sweep kernels carries (K = 1, 2 or 4, dividing
+ * the group count; default 1). No counterpart in the reference. */
+int umt_ctx_set_groups_per_lane(umt_ctx *ctx, int K);
+int umt_ctx_get_groups_per_lane(const umt_ctx *ctx, int *K);
+
+/* Autotuner: times reps sweeps with each supported groups-per-lane value on
+ * the context's own problem and keeps the fastest. msPerK[0..2] receives the
+ * mean ms per sweep for K = 1, 2, 4 (-1 when unsupported); may be NULL.
+ * No counterpart in the reference. */
+int umt_ctx_autotune(umt_ctx *ctx, int reps, int *bestK, double *msPerK);
+
 /* n sweeps back to back on the context's stream, one synchronisation at the end. */
 int umt_ctx_sweep_n(umt_ctx *ctx, int n);
 

@@ -55,6 +55,28 @@ def test_sweep_fp64_matches_oracle(tiles, quad, G):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("K", [1, 2, 4])
+def test_groups_per_lane_variants_agree(K, precision):
+    """Every groups-per-lane kernel variant (the autotuner's choices) gives the
+    oracle's answer: 1e-12 in FP64, 2e-5 in FP32."""
+    p = U.TiledProblem(2, 3, 2, 3, 2)
+    G = 8
+    st, _, _, S = _inputs(p, G, sigt=(1e-2, 1e3), c=(0.0, 0.95), q=(0.0, 1.0))
+    print(f"case K={K} precision={precision}", flush=True)
+    ctx = U.SweepContext(p, G, precision=precision)
+    ctx.set_groups_per_lane(K)
+    assert ctx.groups_per_lane() == K
+    ctx.set_sigt(st)
+    ctx.set_source(S)
+    ctx.sweep()
+    ref, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st)
+    assert rel_err(ctx.phi(), ref) < (1e-12 if precision == 64 else 2e-5)
+    best, times = ctx.autotune(1)
+    assert best in times and set(times) == {1, 2, 4}
+    ctx.close()
+
+
 def test_sweep_fp64_mixed_thin_thick():
     """sigma_t log-uniform over [1e-2, 1e3] (the 20^3 configuration's range)."""
     p = U.TiledProblem(4, 4, 2, 2, 2)
@@ -93,7 +115,7 @@ def test_dropin_per_angle_matches_oracle(small_problem):
     with nonzero incident boundary fluxes and the tau*Psi source term."""
     p = small_problem
     m, g, q, s = p.mesh(), p.geometry(), p.quadrature(), p.schedule()
-    G = 3
+    G = 4  # even: the drop-in carries two groups per lane
     st, _, _, S = _inputs(p, G)
     rng = np.random.default_rng(7)
     starts = np.concatenate([[0], np.cumsum(s["nHyperPlanes"])])

@@ -54,6 +54,9 @@ SIGNATURES = {
     "umt_ctx_sweep_n": (_i, [_vp, _i]),
     "umt_ctx_incoming_faces": (_i, [_vp, _pd]),
     "umt_hbm_copy_probe": (_i, [_i64, _i, _pd]),
+    "umt_ctx_set_groups_per_lane": (_i, [_vp, _i]),
+    "umt_ctx_get_groups_per_lane": (_i, [_vp, _pi]),
+    "umt_ctx_autotune": (_i, [_vp, _i, _pi, _pd]),
 }
 
 _lib = None

@@ -9,6 +9,7 @@ struct SweepLevelArgs {
   const int2* items;        // (zone, angle) pairs of this level, 0-based
   int nItems;
   int G;                    // groups
+  int groupsPerLane;        // K: groups one lane carries (1, 2 or 4; divides G)
   int C;                    // corners; cFP values above C are boundary elements
   const Real* Afp;          // (3, 3, C)
   const Real* Aez;          // (3, 3, C)
@@ -28,6 +29,8 @@ struct SweepLevelArgs {
 
 template <typename Real>
 hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s);
+// Nonzero when the sweep kernels can carry K groups per lane for G groups.
+int groups_per_lane_supported(int G, int K);
 template <typename Real>
 hipError_t launch_phi_from_psi(const Real* psi, const double* w, int A, long stride, long n,
                                double* phi, hipStream_t s);

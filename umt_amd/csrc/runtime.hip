@@ -111,6 +111,7 @@ struct umt_ctx {
   double* phibarOld = nullptr;
   unsigned long long* bits = nullptr;
   bool haveScattering = false;
+  int groupsPerLane = 1;  // K of the sweep kernels
   double nIn = 0.0;  // mean incoming interior FP faces per (corner, angle)
   std::vector<hipEvent_t> evLevel;  // 2 per level
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -127,6 +128,7 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, int level) {
   a.items = x->items + x->levelOffset[level];
   a.nItems = x->levelOffset[level + 1] - x->levelOffset[level];
   a.G = x->G;
+  a.groupsPerLane = x->groupsPerLane;
   a.C = x->C;
   a.Afp = (const Real*)x->Afp;
   a.Aez = (const Real*)x->Aez;
@@ -488,6 +490,49 @@ int umt_ctx_sweep_n(umt_ctx* x, int n) {
   return 0;
 }
 
+int umt_ctx_set_groups_per_lane(umt_ctx* x, int K) {
+  if (!x) return fail(1, "null context");
+  if (!umt::groups_per_lane_supported(x->G, K))
+    return fail(35, "groups per lane %d not supported for %d groups (1, 2 or 4, dividing G)", K, x->G);
+  x->groupsPerLane = K;
+  return 0;
+}
+
+int umt_ctx_get_groups_per_lane(const umt_ctx* x, int* K) {
+  if (!x || !K) return fail(1, "null argument");
+  *K = x->groupsPerLane;
+  return 0;
+}
+
+int umt_ctx_autotune(umt_ctx* x, int reps, int* bestK, double* msPerK) {
+  if (!x || reps < 1) return fail(1, "bad argument");
+  static const int cand[3] = {1, 2, 4};
+  double best = 1e300;
+  int bk = x->groupsPerLane;
+  for (int i = 0; i < 3; ++i) {
+    if (msPerK) msPerK[i] = -1.0;
+    if (!umt::groups_per_lane_supported(x->G, cand[i])) continue;
+    x->groupsPerLane = cand[i];
+    if (int rc = sweep(x, false)) return rc;  // warm-up
+    HIP_TRY(hipEventRecord(x->ev0, x->stream));
+    for (int r = 0; r < reps; ++r)
+      if (int rc = sweep(x, false)) return rc;
+    HIP_TRY(hipEventRecord(x->ev1, x->stream));
+    HIP_TRY(hipEventSynchronize(x->ev1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, x->ev0, x->ev1));
+    const double ms = t / reps;
+    if (msPerK) msPerK[i] = ms;
+    if (ms < best) {
+      best = ms;
+      bk = cand[i];
+    }
+  }
+  x->groupsPerLane = bk;
+  if (bestK) *bestK = bk;
+  return 0;
+}
+
 int umt_ctx_incoming_faces(const umt_ctx* x, double* n_in) {
   if (!x || !n_in) return fail(1, "null argument");
   *n_in = x->nIn;
@@ -697,6 +742,7 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
 
   umt::SweepLevelArgs<double> a{};
   a.G = G;
+  a.groupsPerLane = umt::groups_per_lane_supported(G, 2) ? 2 : 1;
   a.C = C;
   a.Afp = d.Afp;
   a.Aez = d.Aez;

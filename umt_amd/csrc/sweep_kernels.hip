@@ -1,12 +1,14 @@
 // CDNA4 kernels for the upstream-corner-balance (UCB) xyz sweep.
 //
-// One lane owns one (zone, angle, group) triple and solves all eight corners
-// of that hex zone for that direction and group; consecutive lanes hold
-// consecutive groups, so every load of a corner's group vector
-// (S, psi, phi: groups fastest) is one contiguous run per zone. Work is handed
-// out per hyperplane level: a level holds every (zone, angle) pair whose
-// upstream zones lie in earlier levels, for all angles at once, so one launch
-// per level keeps the whole GPU busy.
+// One lane owns one corner of one (zone, angle, group) triple: eight
+// consecutive lanes are the eight corners of a hex zone for one direction and
+// group, and a wave holds eight such (item, group) pairs, groups of one item
+// next to each other, so the loads of a corner's group vector (S, psi: groups
+// fastest) are contiguous across the pairs. The corners of a zone exchange
+// values with DPP lane moves inside their 8-lane group. Work is handed out
+// per hyperplane level: a level holds every (zone, angle) pair whose upstream
+// zones lie in earlier levels, for all angles at once, so one launch per
+// level keeps the whole GPU busy.
 //
 // The corner arithmetic follows snac/SweepUCBxyz.F90:119-306: volume terms,
 // incoming FP-face fluxes, the EZ coupling term sez with fouralpha = 1.82,
@@ -24,165 +26,216 @@
 
 namespace umt {
 
-// Canonical hex topology produced by the tiled-mesh generator: EZ neighbour of
-// each local corner (0-based) for corner-face slots 0..2, following the
-// corner-face chaining of aux/setTetonZone.F90:172-189. The runtime checks
-// every zone against this table before launching these kernels.
-__device__ constexpr int kEZ[8][3] = {{3, 1, 4}, {0, 2, 5}, {1, 3, 6}, {2, 0, 7},
-                                      {5, 7, 0}, {6, 4, 1}, {7, 5, 2}, {4, 6, 3}};
-// The twelve hex edges, as (lower corner, slot in the lower corner).
-__device__ constexpr int kEdgeLo[12] = {0, 0, 0, 1, 1, 2, 2, 3, 4, 4, 5, 6};
-__device__ constexpr int kEdgeSlot[12] = {0, 1, 2, 1, 2, 1, 2, 2, 0, 1, 0, 0};
-// Edge id and orientation seen from (corner, slot): +1 if the corner is the
-// lower end (its A_ez is the stored one), -1 if it is the upper end.
-__device__ constexpr int kEdgeOf[8][3] = {{0, 1, 2}, {1, 3, 4},  {3, 5, 6},  {5, 0, 7},
-                                          {8, 9, 2}, {10, 8, 4}, {11, 10, 6}, {9, 11, 7}};
-__device__ constexpr int kEdgeSign[8][3] = {{1, 1, 1},   {-1, 1, 1},  {-1, 1, 1},
-                                            {-1, -1, 1}, {1, 1, -1},  {1, -1, -1},
-                                            {1, -1, -1}, {-1, -1, -1}};
-
+// The kernels assume the canonical hex topology of the tiled-mesh generator:
+// the EZ neighbour of local corner c (0-based) in corner-face slots 0..2 is
+//   {3,1,4} {0,2,5} {1,3,6} {2,0,7} {5,7,0} {6,4,1} {7,5,2} {4,6,3}
+// for c = 0..7 (corner-face chaining of aux/setTetonZone.F90:172-189). The
+// runtime checks every zone against this table before launching them.
 template <typename Real>
 __device__ __forceinline__ Real dot3(Real ox, Real oy, Real oz, const Real* a) {
   return ox * a[0] + oy * a[1] + oz * a[2];
 }
 
-template <typename Real, bool kSingleAngle>
+// Cross-lane moves inside the 8-lane group of one (item, group) pair, lane
+// c = corner c. kEZ[c][0] is the previous corner of the bottom quad
+// (quad_perm [3,0,1,2]) but the next corner of the top quad
+// (quad_perm [1,2,3,0]); kEZ[c][1] is the inverse; kEZ[c][2] = c ^ 4
+// (row_shl:4 for the bottom quad, row_shr:4 for the top quad).
+template <int kCtrl>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_move(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_i32<kCtrl>((int)b), hi = dpp_i32<kCtrl>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_move(float x) {
+  return __int_as_float(dpp_i32<kCtrl>(__float_as_int(x)));
+}
+template <int kCtrl>
+__device__ __forceinline__ int dpp_move(int x) {
+  return dpp_i32<kCtrl>(x);
+}
+constexpr int kQuadPrev = 0x93;  // quad_perm [3,0,1,2]
+constexpr int kQuadNext = 0x39;  // quad_perm [1,2,3,0]
+constexpr int kRowShl4 = 0x104;
+constexpr int kRowShr4 = 0x114;
+
+// Value held by this lane's EZ neighbour in corner-face slot F.
+template <int F, typename T>
+__device__ __forceinline__ T from_ez(T x, bool bottom) {
+  if constexpr (F == 0) {
+    const T prev = dpp_move<kQuadPrev>(x), next = dpp_move<kQuadNext>(x);
+    return bottom ? prev : next;
+  } else if constexpr (F == 1) {
+    const T prev = dpp_move<kQuadPrev>(x), next = dpp_move<kQuadNext>(x);
+    return bottom ? next : prev;
+  } else {
+    const T up = dpp_move<kRowShl4>(x), down = dpp_move<kRowShr4>(x);
+    return bottom ? up : down;
+  }
+}
+
+template <typename Real>
+__device__ __forceinline__ Real sez_term(Real A, Real sig, Real v, Real halfSigInv, Real Qu,
+                                         Real Qd, Real areaOpp, Real psiOpp) {
+  const Real sigv = sig * v;
+  const Real sigv2 = sigv * sigv;
+  const Real A2 = A * A;
+  const Real gnum = A2 * ((Real)1.82 * sigv2 + A * ((Real)4 * sigv + (Real)3 * A));
+  const Real gden =
+      v * ((Real)4 * sigv * sigv2 + A * ((Real)6 * sigv2 + (Real)2 * A * ((Real)2 * sigv + A)));
+  const Real withOpp =
+      (v * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd)) / (gnum + gden * sig);
+  const Real noOpp = A * halfSigInv * (Qu - Qd);
+  return areaOpp > 0 ? withOpp : noOpp;
+}
+
+// K consecutive groups of one corner as one aligned vector access.
+template <typename Real, int K>
+struct GroupVec {
+  Real v[K];
+};
+template <typename Real, int K>
+__device__ __forceinline__ GroupVec<Real, K> load_groups(const Real* p) {
+  GroupVec<Real, K> r;
+  __builtin_memcpy(&r, __builtin_assume_aligned(p, K * sizeof(Real)), sizeof r);
+  return r;
+}
+template <typename Real, int K>
+__device__ __forceinline__ void store_groups(Real* p, const GroupVec<Real, K>& r) {
+  __builtin_memcpy(__builtin_assume_aligned(p, K * sizeof(Real)), &r, sizeof r);
+}
+
+// One lane: one corner c of one zone for one direction and K consecutive
+// groups. Geometry, dot products and the zone's EZ orientation are shared by
+// the K groups; everything group-dependent is a K-vector in registers.
+template <typename Real, int K, bool kSingleAngle>
 __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
-  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int item = (int)(tid / p.G);
-  const int g = (int)(tid - (long)item * p.G);
-  if (item >= p.nItems) return;
+  const int c = threadIdx.x & 7;
+  const bool bottom = c < 4;
+  const long pair = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const int G = p.G;
+  const int blocksPerItem = G / K;
+  int item = (int)(pair / blocksPerItem);
+  const int g0 = K * (int)(pair - (long)item * blocksPerItem);
+  // Lanes past the last item still take part in the cross-lane moves and the
+  // wave vote; they work on item 0 and store nothing.
+  const bool valid = item < p.nItems;
+  if (!valid) item = 0;
   const int2 it = p.items[item];
   const int z = it.x, a = it.y;
-  const int G = p.G;
-  const long c0 = 8L * z;
+  const long cc = 8L * z + c;
   const Real ox = (Real)p.omega[3 * a + 0];
   const Real oy = (Real)p.omega[3 * a + 1];
   const Real oz = (Real)p.omega[3 * a + 2];
   const long angleBase = kSingleAngle ? 0L : (long)a * p.psiAngleStride;
   Real* __restrict__ psiA = p.psi + angleBase;
 
-  const Real sig = p.sigt[(long)z * G + g];
-  Real Q[8], src[8], den[8], vol[8];
+  const Real v = p.vol[cc];
+  Real afp[3], aez[3];
+  int cfp[3];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    vol[c] = p.vol[c0 + c];
-    Real q = p.S[(c0 + c) * G + g];
-    if (kSingleAngle && p.tauPsi) q += p.tau * p.tauPsi[(c0 + c) * G + g];
-    Q[c] = q;
-    src[c] = vol[c] * q;
+  for (int f = 0; f < 3; ++f) {
+    afp[f] = dot3(ox, oy, oz, p.Afp + 3 * (3 * cc + f));
+    aez[f] = dot3(ox, oy, oz, p.Aez + 3 * (3 * cc + f));
+    cfp[f] = p.cFP[3 * cc + f] - 1;
   }
-  Real ed[12];
+  Real outArea = 0;
 #pragma unroll
-  for (int e = 0; e < 12; ++e)
-    ed[e] = dot3(ox, oy, oz, p.Aez + 3 * (3 * (c0 + kEdgeLo[e]) + kEdgeSlot[e]));
-
-  const Real fouralpha = (Real)1.82;
-  const Real half = (Real)0.5;
-  unsigned exitMask = 0;  // bit 3c+f: outgoing corner face on the problem boundary
+  for (int f = 0; f < 3; ++f) {
+    if (afp[f] > 0) outArea += afp[f];
+    if (aez[f] > 0) outArea += aez[f];
+  }
+  const GroupVec<Real, K> sigv_ = load_groups<Real, K>(p.sigt + (long)z * G + g0);
+  GroupVec<Real, K> Q = load_groups<Real, K>(p.S + cc * G + g0);
+  if (kSingleAngle && p.tauPsi) {
+    const GroupVec<Real, K> tp = load_groups<Real, K>(p.tauPsi + cc * G + g0);
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    Real afp[3], psifp[3];
-    Real sumArea = 0;
+    for (int k = 0; k < K; ++k) Q.v[k] += p.tau * tp.v[k];
+  }
+  // Incoming FP faces (SweepUCBxyz.F90:139-161).
+  GroupVec<Real, K> psf[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    if (afp[f] < 0) {
+      psf[f] = load_groups<Real, K>(psiA + (long)cfp[f] * G + g0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) psf[f].v[k] = 0;
+    }
+  }
+  Real src[K], rden[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const Real sig = sigv_.v[k];
+    Real s = v * Q.v[k];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
+    src[k] = s;
+    rden[k] = (Real)1 / (outArea + sig * v);
+  }
+  // EZ faces: coupling terms computed by the upwind corner
+  // (SweepUCBxyz.F90:182-252), pulled with the opposite sign by the downwind one.
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const Real sig = sigv_.v[k];
+    const Real halfSigInv = (Real)0.5 * ((Real)1 / sig);
+    const Real q = Q.v[k];
+    const Real Qn[3] = {from_ez<0>(q, bottom), from_ez<1>(q, bottom), from_ez<2>(q, bottom)};
+    Real sezOut[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      afp[f] = dot3(ox, oy, oz, p.Afp + 3 * (3 * (c0 + c) + f));
-      psifp[f] = 0;
-      const int cfp = p.cFP[3 * (c0 + c) + f] - 1;
-      if (afp[f] > 0) {
-        sumArea += afp[f];
-        if (cfp >= p.C) exitMask |= 1u << (3 * c + f);
-      } else if (afp[f] < 0) {
-        psifp[f] = psiA[(long)cfp * G + g];
-        src[c] = src[c] - afp[f] * psifp[f];
-      }
+      const int fo = (f + 1) % 3;
+      const Real areaOpp = afp[fo] < 0 ? -afp[fo] : (Real)0;
+      const Real t = sez_term<Real>(aez[f], sig, v, halfSigInv, q, Qn[f], areaOpp, psf[fo].v[k]);
+      sezOut[f] = aez[f] > 0 ? t : (Real)0;
     }
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const Real aez = kEdgeSign[c][f] > 0 ? ed[kEdgeOf[c][f]] : -ed[kEdgeOf[c][f]];
-      if (aez > 0) {
-        const int cez = kEZ[c][f];
-        sumArea += aez;
-        const int ifp = (f + 1) % 3;
-        Real sez;
-        if (afp[ifp] < 0) {
-          const Real psiOpp = psifp[ifp];
-          const Real aez2 = aez * aez;
-          const Real v = vol[c];
-          const Real sigv = sig * v;
-          const Real sigv2 = sigv * sigv;
-          const Real gnum = aez2 * (fouralpha * sigv2 + aez * ((Real)4 * sigv + (Real)3 * aez));
-          const Real gden = v * ((Real)4 * sigv * sigv2 +
-                                 aez * ((Real)6 * sigv2 + (Real)2 * aez * ((Real)2 * sigv + aez)));
-          sez = (v * gnum * (sig * psiOpp - Q[c]) + half * aez * gden * (Q[c] - Q[cez])) /
-                (gnum + gden * sig);
-        } else {
-          const Real sigInv = (Real)1 / sig;
-          sez = half * aez * sigInv * (Q[c] - Q[cez]);
-        }
-        src[c] += sez;
-        src[cez] -= sez;
-      }
-    }
-    den[c] = sumArea + sig * vol[c];
+    Real s = src[k];
+    s += sezOut[0];
+    s += sezOut[1];
+    s += sezOut[2];
+    // The neighbour in slot f sees this edge in slot 1, 0, 2 for f = 0, 1, 2.
+    s -= from_ez<0>(sezOut[1], bottom);
+    s -= from_ez<1>(sezOut[0], bottom);
+    s -= from_ez<2>(sezOut[2], bottom);
+    src[k] = s;
   }
 
-  // Corner solve: relax in static order until every corner saw final
-  // upstream values. A corner with no upstream EZ neighbour is final after
-  // its first visit; a chain of k backward edges needs k+1 passes (<= 8).
-  Real psi[8], rden[8];
-  unsigned fin = 0;
+  // Corner solve psi_c = (src_c + sum over upstream EZ neighbours of
+  // |aez| psi_n) / den_c, relaxed until every corner saw final upstream values.
+  const Real k0 = aez[0] < 0 ? -aez[0] : (Real)0;
+  const Real k1 = aez[1] < 0 ? -aez[1] : (Real)0;
+  const Real k2 = aez[2] < 0 ? -aez[2] : (Real)0;
+  GroupVec<Real, K> psi;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    psi[c] = 0;
-    rden[c] = (Real)1 / den[c];
-  }
-  for (int pass = 0; pass < 8 && fin != 0xffu; ++pass) {
+  for (int k = 0; k < K; ++k) psi.v[k] = src[k] * rden[k];
+  int fin = (k0 == 0 && k1 == 0 && k2 == 0) ? 1 : 0;
+  for (int iter = 0; iter < 8; ++iter) {
+    if (__all(fin)) break;
+    const int f0 = from_ez<0>(fin, bottom), f1 = from_ez<1>(fin, bottom),
+              f2 = from_ez<2>(fin, bottom);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      Real acc = src[c];
-      bool ok = true;
-#pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        const Real aez = kEdgeSign[c][f] > 0 ? ed[kEdgeOf[c][f]] : -ed[kEdgeOf[c][f]];
-        if (aez < 0) {
-          const int nb = kEZ[c][f];
-          acc += (-aez) * psi[nb];
-          ok = ok && ((fin >> nb) & 1u);
-        }
-      }
-      psi[c] = acc * rden[c];
-      fin |= ok ? (1u << c) : 0u;
+    for (int k = 0; k < K; ++k) {
+      const Real x = psi.v[k];
+      const Real n0 = from_ez<0>(x, bottom), n1 = from_ez<1>(x, bottom),
+                 n2 = from_ez<2>(x, bottom);
+      psi.v[k] = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
     }
-  }
-  // Every upstream value is final now: one more visit with the division
-  // psi_c = src_c / (sumArea_c + sigma V_c) of SweepUCBxyz.F90:267.
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    Real acc = src[c];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const Real aez = kEdgeSign[c][f] > 0 ? ed[kEdgeOf[c][f]] : -ed[kEdgeOf[c][f]];
-      if (aez < 0) acc += (-aez) * psi[kEZ[c][f]];
-    }
-    psi[c] = acc / den[c];
+    fin = (k0 == 0 || f0) && (k1 == 0 || f1) && (k2 == 0 || f2);
   }
 
+  if (valid) {
+    store_groups<Real, K>(psiA + cc * G + g0, psi);
+    if (kSingleAngle)
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    psiA[(c0 + c) * G + g] = psi[c];
-    if (kSingleAngle) p.phi[(c0 + c) * G + g] += p.quadwt * (double)psi[c];
-  }
-  // Exiting boundary fluxes go to the boundary tail of this angle's block.
-  if (exitMask) {
+      for (int k = 0; k < K; ++k) p.phi[cc * G + g0 + k] += p.quadwt * (double)psi.v[k];
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-#pragma unroll
-      for (int f = 0; f < 3; ++f)
-        if (exitMask & (1u << (3 * c + f))) {
-          const int cfp = p.cFP[3 * (c0 + c) + f] - 1;
-          psiA[(long)cfp * G + g] = psi[c];
-        }
+    for (int f = 0; f < 3; ++f)
+      if (afp[f] > 0 && cfp[f] >= p.C) store_groups<Real, K>(psiA + (long)cfp[f] * G + g0, psi);
   }
 }
 
@@ -287,17 +340,28 @@ __global__ __launch_bounds__(256) void convert_copy(const Src* __restrict__ s, D
 
 static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-template <typename Real>
-hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
-  const long threads = (long)a.nItems * a.G;
+template <typename Real, int K>
+static hipError_t launch_k(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
+  const long threads = 8L * a.nItems * (a.G / K);  // one lane per corner of each (item, K groups)
   if (threads == 0) return hipSuccess;
   if (singleAngle)
-    hipLaunchKernelGGL((ucb_sweep_level<Real, true>), dim3(blocks_for(threads, 256)), dim3(256), 0,
-                       s, a);
-  else
-    hipLaunchKernelGGL((ucb_sweep_level<Real, false>), dim3(blocks_for(threads, 256)), dim3(256),
+    hipLaunchKernelGGL((ucb_sweep_level<Real, K, true>), dim3(blocks_for(threads, 256)), dim3(256),
                        0, s, a);
+  else
+    hipLaunchKernelGGL((ucb_sweep_level<Real, K, false>), dim3(blocks_for(threads, 256)),
+                       dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+int groups_per_lane_supported(int G, int K) { return (K == 1 || K == 2 || K == 4) && G % K == 0; }
+
+template <typename Real>
+hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
+  switch (a.groupsPerLane) {
+    case 4: return launch_k<Real, 4>(a, singleAngle, s);
+    case 2: return launch_k<Real, 2>(a, singleAngle, s);
+    default: return launch_k<Real, 1>(a, singleAngle, s);
+  }
 }
 
 template <typename Real>

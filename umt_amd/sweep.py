@@ -172,6 +172,22 @@ class SweepContext:
     def sweep_n(self, n):
         check("umt_ctx_sweep_n", _lib.load().umt_ctx_sweep_n(self._h, n))
 
+    def set_groups_per_lane(self, K):
+        check("umt_ctx_set_groups_per_lane", _lib.load().umt_ctx_set_groups_per_lane(self._h, K))
+
+    def groups_per_lane(self):
+        k = C.c_int(0)
+        check("umt_ctx_get_groups_per_lane", _lib.load().umt_ctx_get_groups_per_lane(self._h, C.byref(k)))
+        return k.value
+
+    def autotune(self, reps=2):
+        """Time each supported groups-per-lane value; keep the fastest.
+        Returns (best K, {K: ms per sweep})."""
+        k = C.c_int(0)
+        ms = np.zeros(3)
+        check("umt_ctx_autotune", _lib.load().umt_ctx_autotune(self._h, reps, C.byref(k), _dp(ms)))
+        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4), ms) if t >= 0}
+
     def incoming_faces(self):
         v = C.c_double(0.0)
         check("umt_ctx_incoming_faces", _lib.load().umt_ctx_incoming_faces(self._h, C.byref(v)))
