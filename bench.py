@@ -102,6 +102,7 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
     ctx = U.SweepContext(p, G, precision=precision)
     ctx.set_sigt(st)
     ctx.set_source(S)
+    best_k, tune_ms = ctx.autotune(2)
     ctx.sweep_n(warmup)
     lib.gpu_devicesynchronize()
     t0 = time.perf_counter()
@@ -112,7 +113,8 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
     n_in = ctx.incoming_faces()
     ctx.close()
     return dict(dt=dt, ms_levels=ms_levels, launches=launches, n_in=n_in,
-                ms_sweep_events=ms_sweep_ev)
+                ms_sweep_events=ms_sweep_ev, groups_per_lane=best_k,
+                autotune_ms_per_sweep={str(k): v for k, v in tune_ms.items()})
 
 
 def converge(p, G, precision, st, ss, qq, tol, max_iters):
@@ -121,12 +123,14 @@ def converge(p, G, precision, st, ss, qq, tol, max_iters):
     ctx = U.SweepContext(p, G, precision=precision)
     ctx.set_sigt(st)
     ctx.set_scattering(ss, qq)
+    best_k, _ = ctx.autotune(1)
     t0 = time.perf_counter()
     ok, it, res, _ = ctx.source_iteration(tol=tol, max_iters=max_iters)
     dt = time.perf_counter() - t0
     phi = ctx.phi()
     ctx.close()
-    return dict(converged=ok, iterations=it, residual=res, seconds=dt), phi
+    return dict(converged=ok, iterations=it, residual=res, seconds=dt,
+                groups_per_lane=best_k), phi
 
 
 def main():
@@ -182,11 +186,13 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": ach64 / HBM_PEAK_GBS, "traffic": traffic,
-        "kernel": "ucb_sweep_level<double,false>",
+        "kernel": "ucb_sweep_level<double,K,false>",
         "launches_per_sweep": r64["launches"],
         "mean_launch_ms": r64["ms_levels"] / r64["launches"],
         "algorithmic_bytes_per_unknown": B64, "n_in": r64["n_in"],
         "unknowns_per_launch": unknowns / r64["launches"],
+        "groups_per_lane": r64["groups_per_lane"],
+        "autotune_ms_per_sweep": r64["autotune_ms_per_sweep"],
     }
     out = {
         "metric": "sweep unknowns/s (zone-corner x angle x group); wall time to converged phi at 1e-8",
@@ -223,8 +229,9 @@ def main():
             "ms_per_step": r32["dt"] * 1e3 / args.steps,
             "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS,
-                         "kernel": "ucb_sweep_level<float,false>",
-                         "algorithmic_bytes_per_unknown": B32},
+                         "kernel": "ucb_sweep_level<float,K,false>",
+                         "algorithmic_bytes_per_unknown": B32,
+                         "groups_per_lane": r32["groups_per_lane"]},
         }
 
     if not args.no_converge:

@@ -1,0 +1,58 @@
+"""The Fortran host (umt_amd/fortran, ISO_C_BINDING over the C-ABI) on the
+GPU against the oracle: source iteration through the device-resident
+context built from Fortran-held Teton-layout arrays, and one angle through
+the drop-in gpu_sweepucbxyz called with the reference's argument list."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import umt_amd as U
+from oracle import oracle as O
+
+from conftest import ROOT, rel_err
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(ROOT, "umt_amd", "fortran", "umt_si_driver")
+
+
+def _run(args, out, tmp_path):
+    path = str(tmp_path / out)
+    cmd = [DRIVER] + [str(a) for a in args] + [path]
+    print("run:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return np.fromfile(path, dtype=np.float64), r.stdout
+
+
+def test_fortran_source_iteration_matches_oracle(tmp_path):
+    seed, G = 20261203, 2
+    phi, out = _run(["si", 4, 4, 4, 2, 2, G, 64, 10, 0.0, seed, 0.1, 10.0, 0.0, 0.9, 1.0, 1.0, 1],
+                    "phi_si.bin", tmp_path)
+    assert "si iterations=    10" in out
+    p = U.TiledProblem(4, 4, 4, 2, 2)
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (0.1, 10.0), (0.0, 0.9), (1.0, 1.0))
+    ref, it, _, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), p.schedule(),
+                                       st, ss, qq, 0.0, 10)
+    assert rel_err(phi.reshape(p.ncorner, G), ref) < 1e-12
+
+
+def test_fortran_dropin_angle_matches_oracle(tmp_path):
+    seed, G, angle = 7, 4, 11
+    phi, _ = _run(["dropin", 2, 2, 2, 2, 2, G, 64, 0, 0.0, seed, 0.01, 100.0, 0.0, 0.9, 0.0, 1.0,
+                   angle], "phi_dropin.bin", tmp_path)
+    p = U.TiledProblem(2, 2, 2, 2, 2)
+    m, g, q, s = p.mesh(), p.geometry(), p.quadrature(), p.schedule()
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (0.01, 100.0), (0.0, 0.9), (0.0, 1.0))
+    S = np.repeat(ss + qq, 8, axis=0) / (4 * np.pi)
+    a = angle - 1
+    starts = np.concatenate([[0], np.cumsum(s["nHyperPlanes"])])
+    Psi1 = np.zeros((p.ncorner + p.nbelem, G))
+    PsiB = np.zeros((p.nbelem, G))
+    ref = np.zeros((p.ncorner, G))
+    O.sweep_angle(m, g, S, st, q["omega"][a], q["weight"][a], s["nextZ"][a], s["nextC"][a],
+                  s["zonesInPlane"][starts[a]:starts[a + 1]], Psi1, PsiB, ref)
+    assert rel_err(phi.reshape(p.ncorner, G), ref) < 1e-12

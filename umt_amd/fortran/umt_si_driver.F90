@@ -1,0 +1,159 @@
+!***********************************************************************
+!  umt_si_driver - Fortran host driver for the MI355X UCB sweep.
+!
+!  Builds a tiled hex problem, holds every array in the Teton layout
+!  (Geometry_mod / AngleSet_mod / GroupSet_mod names and shapes) and then
+!    mode "si":     hands the arrays to a device-resident context and runs
+!                   steady source iteration on the GPU (rt/rtmainsn.F90
+!                   semantics for tau = 0, within-group scattering);
+!    mode "dropin": sweeps one angle through gpu_sweepucbxyz with the
+!                   argument list of src/teton/gpu/SweepUCBxyzToGPU.F90:198-240.
+!  Writes the resulting Phi(G,C) as raw FP64 to the output file and prints a
+!  one-line summary.
+!
+!  usage: umt_si_driver mode nx ny nz npolar nazimuthal groups precision
+!                       maxIters tol seed sigt_lo sigt_hi c_lo c_hi q_lo q_hi
+!                       angle outfile
+!***********************************************************************
+program umt_si_driver
+
+  use iso_c_binding
+  use umt_sweep_mod
+
+  implicit none
+
+  character(len=256) :: arg, mode, outfile
+  integer(c_int)     :: nx, ny, nz, npolar, nazimu, G, precision, maxIters, angle
+  real(c_double)     :: tol, sigt_lo, sigt_hi, c_lo, c_hi, q_lo, q_hi
+  integer(c_int64_t) :: seed
+  integer(c_int64_t) :: dims(8)
+  type(c_ptr)        :: prob, ctx
+  integer            :: Z, C, nb, A, totalPlanes, rc, i, u, start
+  integer(c_int)     :: iters, bestK, status
+
+  integer(c_int), allocatable :: numCorner(:), cOffSet(:), nCFaces(:), cFP(:,:), cEZ(:,:)
+  integer(c_int), allocatable :: CToFace(:,:), zoneOpp(:,:), faceOpp(:,:)
+  integer(c_int), allocatable :: nHyperPlanes(:), zonesInPlane(:), nextZ(:,:), nextC(:,:)
+  real(c_double), allocatable :: px(:,:), A_fp(:,:,:), A_ez(:,:,:), Volume(:), VolumeZone(:)
+  real(c_double), allocatable :: omega(:,:), weight(:)
+  real(c_double), allocatable :: Sigt(:,:), sigs(:,:), q(:,:), Phi(:,:), STotal(:,:)
+  real(c_double), allocatable :: Psi1(:,:), PsiB(:,:), Psi(:,:), hist(:)
+  real(c_double)              :: residual, msPerK(3), tau, quadwt
+  real(c_double)              :: dummy(1)
+  integer(c_int)              :: idummy(1)
+  integer(c_int)              :: ndim, maxcf, maxCorner, zero, one, minusOne, streamId, nHP
+
+  call get_command_argument(1, mode)
+  call int_arg(2, nx); call int_arg(3, ny); call int_arg(4, nz)
+  call int_arg(5, npolar); call int_arg(6, nazimu); call int_arg(7, G)
+  call int_arg(8, precision); call int_arg(9, maxIters)
+  call real_arg(10, tol)
+  call get_command_argument(11, arg); read(arg, *) seed
+  call real_arg(12, sigt_lo); call real_arg(13, sigt_hi)
+  call real_arg(14, c_lo); call real_arg(15, c_hi)
+  call real_arg(16, q_lo); call real_arg(17, q_hi)
+  call int_arg(18, angle)
+  call get_command_argument(19, outfile)
+
+  rc = umt_problem_tiled(nx, ny, nz, npolar, nazimu, 1, prob)
+  call check(rc, "umt_problem_tiled")
+  rc = umt_problem_dims(prob, dims)
+  Z = int(dims(1)); C = int(dims(2)); nb = int(dims(3)); A = int(dims(4))
+  totalPlanes = int(dims(7))
+
+  allocate(numCorner(Z), cOffSet(Z), nCFaces(C), cFP(3,C), cEZ(3,C), CToFace(3,C))
+  allocate(zoneOpp(6,Z), faceOpp(6,Z), px(3,C), A_fp(3,3,C), A_ez(3,3,C), Volume(C))
+  allocate(VolumeZone(Z), omega(3,A), weight(A), nHyperPlanes(A), zonesInPlane(totalPlanes))
+  allocate(nextZ(Z,A), nextC(C,A), Sigt(G,Z), sigs(G,Z), q(G,Z), Phi(G,C))
+
+  rc = umt_problem_mesh(prob, numCorner, cOffSet, nCFaces, cFP, cEZ, CToFace, zoneOpp, faceOpp, px)
+  call check(rc, "umt_problem_mesh")
+  rc = umt_problem_geometry(prob, A_fp, A_ez, Volume, VolumeZone)
+  call check(rc, "umt_problem_geometry")
+  rc = umt_problem_quadrature(prob, omega, weight)
+  call check(rc, "umt_problem_quadrature")
+  rc = umt_problem_schedule(prob, nHyperPlanes, zonesInPlane, nextZ, nextC)
+  call check(rc, "umt_problem_schedule")
+  rc = umt_draw_cross_sections(seed, Z, G, sigt_lo, sigt_hi, c_lo, c_hi, q_lo, q_hi, Sigt, sigs, q)
+  call check(rc, "umt_draw_cross_sections")
+
+  if (trim(mode) == "si") then
+
+    rc = umt_ctx_create(Z, C, nb, G, A, precision, numCorner, cOffSet, nCFaces, cFP, cEZ, &
+                        A_fp, A_ez, Volume, VolumeZone, omega, weight, nHyperPlanes,       &
+                        zonesInPlane, nextZ, ctx)
+    call check(rc, "umt_ctx_create")
+    call check(umt_ctx_set_sigt(ctx, Sigt), "umt_ctx_set_sigt")
+    call check(umt_ctx_set_scattering(ctx, sigs, q), "umt_ctx_set_scattering")
+    call check(umt_ctx_autotune(ctx, 1, bestK, msPerK), "umt_ctx_autotune")
+    allocate(hist(max(maxIters, 1)))
+    rc = umt_ctx_source_iteration(ctx, tol, maxIters, iters, residual, hist)
+    if (rc /= 0 .and. rc /= 1) call check(rc, "umt_ctx_source_iteration")
+    call check(umt_ctx_get_phi(ctx, Phi), "umt_ctx_get_phi")
+    call umt_ctx_destroy(ctx)
+    write(*, '(a,i6,a,es12.5,a,i2,a,es22.15)') "si iterations=", iters, " residual=", residual, &
+         " groupsPerLane=", bestK, " sumPhi=", sum(Phi)
+
+  elseif (trim(mode) == "dropin") then
+
+    ! Source for one sweep: STotal = (sigs * phi0 + q) / (4 pi) with phi0 = 1.
+    allocate(STotal(G,C), Psi1(G,C+nb), PsiB(G,nb), Psi(G,C))
+    do i = 1, C
+      STotal(:,i) = (sigs(:, (i-1)/8 + 1) + q(:, (i-1)/8 + 1)) / (16.0d0*atan(1.0d0))
+    enddo
+    Psi1 = 0.0d0; PsiB = 0.0d0; Psi = 0.0d0; Phi = 0.0d0
+    start = 0
+    do i = 1, angle - 1
+      start = start + nHyperPlanes(i)
+    enddo
+    nHP = nHyperPlanes(angle)
+    tau = 0.0d0; quadwt = weight(angle)
+    ndim = 3; maxcf = 3; maxCorner = 8; zero = 0; one = 1; minusOne = -1; streamId = 0
+    call gpu_sweepucbxyz(angle, nHP, zonesInPlane(start+1), nextZ(1,angle), nextC(1,angle), &
+                         STotal, tau, Psi, G, Volume, Sigt, nCFaces, ndim, maxcf, C,          &
+                         A_fp, omega(1,angle), cFP, Psi1, nb, A_ez, cEZ, A, quadwt, Phi,      &
+                         PsiB, maxCorner, one, streamId, one, zero, zero, zero, dummy,        &
+                         idummy, zero, zero, dummy, minusOne, numCorner, cOffSet)
+    call gpu_streamsynchronize(streamId)
+    status = gpu_sweepucbxyz_status()
+    if (status /= 0) call check(status, "gpu_sweepucbxyz")
+    write(*, '(a,i4,a,es22.15)') "dropin angle=", angle, " sumPhi=", sum(Phi)
+
+  else
+    write(*, *) "unknown mode ", trim(mode)
+    stop 2
+  endif
+
+  open(newunit=u, file=trim(outfile), access="stream", form="unformatted", status="replace")
+  write(u) Phi
+  close(u)
+  call umt_problem_destroy(prob)
+
+contains
+
+  subroutine int_arg(k, v)
+    integer, intent(in) :: k
+    integer(c_int), intent(out) :: v
+    character(len=64) :: s
+    call get_command_argument(k, s)
+    read(s, *) v
+  end subroutine int_arg
+
+  subroutine real_arg(k, v)
+    integer, intent(in) :: k
+    real(c_double), intent(out) :: v
+    character(len=64) :: s
+    call get_command_argument(k, s)
+    read(s, *) v
+  end subroutine real_arg
+
+  subroutine check(code, what)
+    integer(c_int), intent(in) :: code
+    character(len=*), intent(in) :: what
+    if (code /= 0) then
+      write(*, '(a,a,i5,a,a)') what, " failed with status ", code, ": ", umt_error_message()
+      stop 1
+    endif
+  end subroutine check
+
+end program umt_si_driver

@@ -1,0 +1,230 @@
+!***********************************************************************
+!  umt_sweep_mod - Fortran host binding of the MI355X UCB xyz sweep
+!                  library (libumt_amd.so, C-ABI in include/umt_sweep.h).
+!
+!  The drop-in entry point gpu_sweepucbxyz keeps the argument list the
+!  reference binds in src/teton/gpu/SweepUCBxyzToGPU.F90:43-91 (scalars by
+!  reference, arrays by first element), so a Teton build links this library
+!  in place of the CUDA object without touching that file. The remaining
+!  interfaces expose the device-resident context: a Fortran host hands over
+!  its Teton-layout arrays once (Geometry_mod / AngleSet_mod names and
+!  shapes) and then sweeps or runs source iteration on the GPU.
+!***********************************************************************
+module umt_sweep_mod
+
+  use iso_c_binding
+
+  implicit none
+
+  private
+
+  public :: gpu_sweepucbxyz, gpu_streamsynchronize, gpu_devicesynchronize, &
+            gpu_sweepucbxyz_status, umt_problem_tiled, umt_problem_destroy, &
+            umt_problem_dims, umt_problem_mesh, umt_problem_geometry,       &
+            umt_problem_quadrature, umt_problem_schedule,                   &
+            umt_draw_cross_sections, umt_ctx_create, umt_ctx_destroy,       &
+            umt_ctx_set_sigt, umt_ctx_set_source, umt_ctx_set_scattering,   &
+            umt_ctx_sweep, umt_ctx_get_phi, umt_ctx_source_iteration,       &
+            umt_ctx_autotune, umt_error_message
+
+  interface
+
+!   gpu/GPU_SweepUCBxyz.cu:532-572 argument list
+    subroutine gpu_sweepucbxyz(Angle, nHyperPlanes, nZonesInPlane, nextZ, nextC, STotal, tau, &
+                               Psi, Groups, Volume, Sigt, nCFacesArray, ndim, maxcf, ncorner,  &
+                               A_fp, omega, cFP, Psi1, nbelem, A_ez, cEZ, NumAngles, quadwt,   &
+                               Phi, PsiB, maxCorner, mem0solve1, streamId, totalStreams,       &
+                               savePsi, numCycles, cycleOffSet, cyclePsi, cycleList, b0,       &
+                               nBdyElem, PsiBMref, Mref, Geom_numCorner, Geom_cOffSet)         &
+                               bind(c, name="gpu_sweepucbxyz")
+      import :: c_int, c_double
+      integer(c_int) :: Angle, nHyperPlanes, nZonesInPlane(*), nextZ(*), nextC(*)
+      real(c_double) :: STotal(*), tau, Psi(*)
+      integer(c_int) :: Groups
+      real(c_double) :: Volume(*), Sigt(*)
+      integer(c_int) :: nCFacesArray(*), ndim, maxcf, ncorner
+      real(c_double) :: A_fp(*), omega(*)
+      integer(c_int) :: cFP(*)
+      real(c_double) :: Psi1(*)
+      integer(c_int) :: nbelem
+      real(c_double) :: A_ez(*)
+      integer(c_int) :: cEZ(*), NumAngles
+      real(c_double) :: quadwt, Phi(*), PsiB(*)
+      integer(c_int) :: maxCorner, mem0solve1, streamId, totalStreams, savePsi, numCycles
+      integer(c_int) :: cycleOffSet
+      real(c_double) :: cyclePsi(*)
+      integer(c_int) :: cycleList(*), b0, nBdyElem
+      real(c_double) :: PsiBMref(*)
+      integer(c_int) :: Mref, Geom_numCorner(*), Geom_cOffSet(*)
+    end subroutine gpu_sweepucbxyz
+
+!   gpu/GPU_SweepUCBxyz.cu:520-528
+    subroutine gpu_streamsynchronize(streamId) bind(c, name="gpu_streamsynchronize")
+      import :: c_int
+      integer(c_int) :: streamId
+    end subroutine gpu_streamsynchronize
+
+    subroutine gpu_devicesynchronize() bind(c, name="gpu_devicesynchronize")
+    end subroutine gpu_devicesynchronize
+
+    integer(c_int) function gpu_sweepucbxyz_status() bind(c, name="gpu_sweepucbxyz_status")
+      import :: c_int
+    end function gpu_sweepucbxyz_status
+
+    type(c_ptr) function umt_last_error() bind(c, name="umt_last_error")
+      import :: c_ptr
+    end function umt_last_error
+
+    integer(c_int) function umt_problem_tiled(nx, ny, nz, npolar, nazimuthal, kdtree, prob) &
+        bind(c, name="umt_problem_tiled")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nx, ny, nz, npolar, nazimuthal, kdtree
+      type(c_ptr) :: prob
+    end function umt_problem_tiled
+
+    subroutine umt_problem_destroy(prob) bind(c, name="umt_problem_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: prob
+    end subroutine umt_problem_destroy
+
+    integer(c_int) function umt_problem_dims(prob, dims) bind(c, name="umt_problem_dims")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: prob
+      integer(c_int64_t) :: dims(8)
+    end function umt_problem_dims
+
+    integer(c_int) function umt_problem_mesh(prob, numCorner, cOffSet, nCFaces, cFP, cEZ, &
+                                             CToFace, zoneOpp, faceOpp, px)              &
+        bind(c, name="umt_problem_mesh")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: prob
+      integer(c_int) :: numCorner(*), cOffSet(*), nCFaces(*), cFP(*), cEZ(*), CToFace(*)
+      integer(c_int) :: zoneOpp(*), faceOpp(*)
+      real(c_double) :: px(*)
+    end function umt_problem_mesh
+
+    integer(c_int) function umt_problem_geometry(prob, A_fp, A_ez, Volume, VolumeZone) &
+        bind(c, name="umt_problem_geometry")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: prob
+      real(c_double) :: A_fp(*), A_ez(*), Volume(*), VolumeZone(*)
+    end function umt_problem_geometry
+
+    integer(c_int) function umt_problem_quadrature(prob, omega, weight) &
+        bind(c, name="umt_problem_quadrature")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: prob
+      real(c_double) :: omega(*), weight(*)
+    end function umt_problem_quadrature
+
+    integer(c_int) function umt_problem_schedule(prob, nHyperPlanes, zonesInPlane, nextZ, nextC) &
+        bind(c, name="umt_problem_schedule")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: prob
+      integer(c_int) :: nHyperPlanes(*), zonesInPlane(*), nextZ(*), nextC(*)
+    end function umt_problem_schedule
+
+    integer(c_int) function umt_draw_cross_sections(seed, nzones, groups, sigt_lo, sigt_hi, &
+                                                    c_lo, c_hi, q_lo, q_hi, sigt, sigs, q)  &
+        bind(c, name="umt_draw_cross_sections")
+      import :: c_int, c_double, c_int64_t
+      integer(c_int64_t), value :: seed
+      integer(c_int), value :: nzones, groups
+      real(c_double), value :: sigt_lo, sigt_hi, c_lo, c_hi, q_lo, q_hi
+      real(c_double) :: sigt(*), sigs(*), q(*)
+    end function umt_draw_cross_sections
+
+    integer(c_int) function umt_ctx_create(nzones, ncorner, nbelem, groups, nangles, precision, &
+                                           numCorner, cOffSet, nCFaces, cFP, cEZ, A_fp, A_ez,   &
+                                           Volume, VolumeZone, omega, weight, nHyperPlanes,     &
+                                           zonesInPlane, nextZ, ctx)                            &
+        bind(c, name="umt_ctx_create")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: nzones, ncorner, nbelem, groups, nangles, precision
+      integer(c_int) :: numCorner(*), cOffSet(*), nCFaces(*), cFP(*), cEZ(*)
+      real(c_double) :: A_fp(*), A_ez(*), Volume(*), VolumeZone(*), omega(*), weight(*)
+      integer(c_int) :: nHyperPlanes(*), zonesInPlane(*), nextZ(*)
+      type(c_ptr) :: ctx
+    end function umt_ctx_create
+
+    subroutine umt_ctx_destroy(ctx) bind(c, name="umt_ctx_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine umt_ctx_destroy
+
+    integer(c_int) function umt_ctx_set_sigt(ctx, sigt) bind(c, name="umt_ctx_set_sigt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: sigt(*)
+    end function umt_ctx_set_sigt
+
+    integer(c_int) function umt_ctx_set_source(ctx, STotal) bind(c, name="umt_ctx_set_source")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: STotal(*)
+    end function umt_ctx_set_source
+
+    integer(c_int) function umt_ctx_set_scattering(ctx, sigs, q) &
+        bind(c, name="umt_ctx_set_scattering")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: sigs(*), q(*)
+    end function umt_ctx_set_scattering
+
+    integer(c_int) function umt_ctx_sweep(ctx) bind(c, name="umt_ctx_sweep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function umt_ctx_sweep
+
+    integer(c_int) function umt_ctx_get_phi(ctx, phi) bind(c, name="umt_ctx_get_phi")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: phi(*)
+    end function umt_ctx_get_phi
+
+    integer(c_int) function umt_ctx_source_iteration(ctx, tol, maxIters, iters, residual, hist) &
+        bind(c, name="umt_ctx_source_iteration")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tol
+      integer(c_int), value :: maxIters
+      integer(c_int) :: iters
+      real(c_double) :: residual, hist(*)
+    end function umt_ctx_source_iteration
+
+    integer(c_int) function umt_ctx_autotune(ctx, reps, bestK, msPerK) &
+        bind(c, name="umt_ctx_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
+      integer(c_int) :: bestK
+      real(c_double) :: msPerK(3)
+    end function umt_ctx_autotune
+
+    integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+    end function c_strlen
+
+  end interface
+
+contains
+
+! Message of the calling thread's last failure in the library.
+  function umt_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    p = umt_last_error()
+    n = int(c_strlen(p))
+    allocate(character(len=n) :: msg)
+    if (n > 0) then
+      call c_f_pointer(p, chars, [n])
+      do i = 1, n
+        msg(i:i) = chars(i)
+      enddo
+    endif
+  end function umt_error_message
+
+end module umt_sweep_mod
