@@ -94,6 +94,13 @@ int umt_problem_quadrature(const umt_problem *p, double *omega, double *weight);
 int umt_problem_schedule(const umt_problem *p, int *nHyperPlanes, int *zonesInPlane, int *nextZ,
                          int *nextC);
 
+/* Grey-acceleration quadrature (level-symmetric S2, rt/quadxyz.F90 order 2,
+ * the GTA order of driver/test_driver.cc:1206) and its schedule. Faces that
+ * contain an S2 direction (the tile's diagonal faces) carry no dependency
+ * for it. nangles receives 8; array sizes as umt_problem_schedule. */
+int umt_problem_gta(const umt_problem *p, int *nangles, double *omega, double *weight,
+                    int *nHyperPlanes, int *zonesInPlane, int *nextZ);
+
 /* Seeded cross sections per (zone, group), counter-based so any ordering of
  * the draws gives the same values: sigt ~ log-uniform[sigt_lo, sigt_hi],
  * sigs = c * sigt with c ~ U[c_lo, c_hi], q ~ U[q_lo, q_hi]. Arrays (G, Z). */
@@ -152,6 +159,21 @@ int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
  * of the sweep-level launches in one sweep; launches = level launches per sweep. */
 int umt_ctx_time_sweeps(umt_ctx *ctx, int warmup, int reps, double *ms_sweep,
                         double *ms_level_kernels, int *launches);
+
+/* Grey-transport acceleration of source iteration (rt/GTASolver.F90): per
+ * group, the scattering-rate change after each sweep drives a BiCGStab solve
+ * of the S2 grey-correction problem, each step one S2 sweep plus one
+ * stretched-TSA sweep (rt/GreySweep.F90), and the correction is added to
+ * phi. umt_ctx_enable_gta takes the S2 directions and their schedule
+ * (umt_problem_gta gives them; umt_ctx_create_from_problem enables GTA
+ * itself). Replaces the GTA path of rt/LinearSolver.F90:87-121, which the
+ * mini-app build compiles out. */
+int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const double *weight,
+                       const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ);
+/* mode 0: plain source iteration (default); 1: GTA-accelerated. */
+int umt_ctx_set_acceleration(umt_ctx *ctx, int mode);
+/* Cumulative BiCGStab iterations and S2 sweeps done by GTA on this context. */
+int umt_ctx_gta_stats(const umt_ctx *ctx, int64_t *greyIterations, int64_t *greySweeps);
 
 /* Groups each lane of the sweep kernels carries (K = 1, 2 or 4, dividing
  * the group count; default 1). No counterpart in the reference. */

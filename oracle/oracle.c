@@ -24,6 +24,13 @@
 
 static double dot3(const double *o, const double *a) { return o[0] * a[0] + o[1] * a[1] + o[2] * a[2]; }
 
+/* Omega.A, zero when |Omega.A| <= tol |A|_1 (face parallel to Omega). */
+static double dot3_tol(const double *o, const double *a, double tol) {
+  const double d = dot3(o, a);
+  if (tol > 0.0 && fabs(d) <= tol * (fabs(a[0]) + fabs(a[1]) + fabs(a[2]))) return 0.0;
+  return d;
+}
+
 /* ------------------------------------------------------------------------ */
 /* rt/geometryUCBxyz.F90:63-181; centres from mods/Geometry_mod.F90:344-441 */
 int oracle_geometry(int nzones, int ncorner, const int *numCorner, const int *cOffSet,
@@ -187,6 +194,17 @@ int oracle_schedule_angle(int nzones, int ncorner, const int *numCorner, const i
                           const int *CToFace, const int *zoneOpp, const int *faceOpp,
                           const double *A_fp, const double *A_ez, const double *omega,
                           int *nextZ, int *nextC, int *zonesInPlane, int *nHyperPlanesOut) {
+  return oracle_schedule_angle_tol(nzones, ncorner, numCorner, cOffSet, zoneFaces, nCFaces, cEZ,
+                                   CToFace, zoneOpp, faceOpp, A_fp, A_ez, omega, 0.0, nextZ,
+                                   nextC, zonesInPlane, nHyperPlanesOut);
+}
+
+int oracle_schedule_angle_tol(int nzones, int ncorner, const int *numCorner, const int *cOffSet,
+                              const int *zoneFaces, const int *nCFaces, const int *cEZ,
+                              const int *CToFace, const int *zoneOpp, const int *faceOpp,
+                              const double *A_fp, const double *A_ez, const double *omega,
+                              double parTol, int *nextZ, int *nextC, int *zonesInPlane,
+                              int *nHyperPlanesOut) {
   (void)ncorner;
   int *needZ = calloc(nzones, sizeof(int));
   int *listZone = calloc(nzones, sizeof(int));
@@ -206,7 +224,7 @@ int oracle_schedule_angle(int nzones, int ncorner, const int *numCorner, const i
         const int face = I2(CToFace, cface, cc, MAXCF);
         const int zOpp = I2(zoneOpp, face, zone, MAXFACES);
         if (zOpp > zone) {
-          const double afpm = dot3(omega, &I3(A_fp, 1, cface, cc, 3, MAXCF));
+          const double afpm = dot3_tol(omega, &I3(A_fp, 1, cface, cc, 3, MAXCF), parTol);
           afpm_Face[face - 1] += afpm;
           if (afpm < 0.0) nInc[face - 1] += 1;
           else if (afpm > 0.0) nExit[face - 1] += 1;
@@ -252,7 +270,7 @@ int oracle_schedule_angle(int nzones, int ncorner, const int *numCorner, const i
       for (int cface = 1; cface <= nCFaces[cc - 1]; ++cface) {
         const int cez = I2(cEZ, cface, cc, MAXCF);
         if (cez > c) {
-          const double aez = dot3(omega, &I3(A_ez, 1, cface, cc, 3, MAXCF));
+          const double aez = dot3_tol(omega, &I3(A_ez, 1, cface, cc, 3, MAXCF), parTol);
           if (aez < 0.0) {
             need[c - 1] += 1;
             DownStreamC[cez - 1][nDSC[cez - 1]++] = c;

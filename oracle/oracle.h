@@ -10,15 +10,16 @@
  * routine it restates (paths relative to src/teton/).
  *
  * How each function is pinned (the reference has no tests or golden vectors
- * for this path, and its Fortran cannot be built here: see oracle/README.md):
+ * for this path, and its Fortran cannot be built here: SweepUCBxyz.F90 uses
+ * Size_mod, which pulls in Conduit, MPI and a CMake-generated module):
  *   oracle_geometry          closure, exact volumes, equal-and-opposite areas,
- *                            unit-cube closed forms (tests/test_oracle_geometry.py)
+ *                            box-zone closed forms (tests/test_oracle_geometry.py)
  *   oracle_quadrature        QRS45 table fixture parsed from the reference
- *                            text, sum w = 4 pi, moment identities
+ *                            text, sum w = 4 pi, octant symmetry
  *   oracle_schedule_angle    brute-force longest-path layering, brute-force
  *                            lexicographically-smallest corner order
  *   oracle_sweep_angle       zone balance, constant solution, thick limit,
- *                            single-zone dense solve, grey-form cross-check,
+ *                            corner balance with the grey-form coupling,
  *                            analytic pure-absorber convergence
  *   oracle_source_iteration  infinite-medium fixed point, global balance
  */
@@ -53,6 +54,17 @@ int oracle_schedule_angle(int nzones, int ncorner, const int *numCorner, const i
                           const int *CToFace, const int *zoneOpp, const int *faceOpp,
                           const double *A_fp, const double *A_ez, const double *omega,
                           int *nextZ, int *nextC, int *zonesInPlane, int *nHyperPlanes);
+
+/* The same with corner faces whose |Omega.A| <= parTol * |A|_1 taken as
+ * parallel to Omega (no dependency): directions lying in mesh faces, as the
+ * S2 grey-acceleration directions do on the tile's diagonal faces. parTol = 0
+ * is oracle_schedule_angle. */
+int oracle_schedule_angle_tol(int nzones, int ncorner, const int *numCorner, const int *cOffSet,
+                              const int *zoneFaces, const int *nCFaces, const int *cEZ,
+                              const int *CToFace, const int *zoneOpp, const int *faceOpp,
+                              const double *A_fp, const double *A_ez, const double *omega,
+                              double parTol, int *nextZ, int *nextC, int *zonesInPlane,
+                              int *nHyperPlanes);
 
 /* snac/SweepUCBxyz.F90:82-318 for one angle, in nextZ / nextC order.
  * Psi1(G, C+nbelem) work array; PsiB(G, nbelem) of this angle (in: incident,

@@ -18,6 +18,8 @@ _SIG = {
     "oracle_quadrature": (_i, [_i, _i, _i, _pd, _pd, _pd, _pd, _pd, _pd]),
     "oracle_schedule_angle": (_i, [_i, _i, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd,
                                    _pi, _pi, _pi, _pi]),
+    "oracle_schedule_angle_tol": (_i, [_i, _i, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd,
+                                       _pd, _d, _pi, _pi, _pi, _pi]),
     "oracle_sweep_angle": (None, [_i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
                                   _pd, _d, _pd, _pd, _d, _i, _pi, _pi, _pi, _pd, _pd, _pd]),
     "oracle_sweep_all": (None, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
@@ -82,28 +84,28 @@ def quadrature(npolar, nazimuthal, cos_theta, w_theta, cos_phi, w_phi, polaraxis
     return omega, weight
 
 
-def schedule_angle(mesh, A_fp, A_ez, omega):
+def schedule_angle(mesh, A_fp, A_ez, omega, par_tol=0.0):
     """(rc, nextZ (Z), nextC (C), zonesInPlane (nHP)) for one direction."""
     Z = mesh["numCorner"].shape[0]
     Cn = mesh["nCFaces"].shape[0]
     nextZ, nextC = np.zeros(Z, np.int32), np.zeros(Cn, np.int32)
     zip_ = np.zeros(Z, np.int32)
     nhp = C.c_int(0)
-    rc = load().oracle_schedule_angle(
+    rc = load().oracle_schedule_angle_tol(
         Z, Cn, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["zoneFaces"]),
         _ip(mesh["nCFaces"]), _ip(mesh["cEZ"]), _ip(mesh["CToFace"]), _ip(mesh["zoneOpp"]),
         _ip(mesh["faceOpp"]), _dp(A_fp), _dp(A_ez),
-        _dp(np.ascontiguousarray(omega, np.float64)), _ip(nextZ), _ip(nextC), _ip(zip_),
+        _dp(np.ascontiguousarray(omega, np.float64)), par_tol, _ip(nextZ), _ip(nextC), _ip(zip_),
         C.byref(nhp))
     return rc, nextZ, nextC, zip_[:nhp.value].copy()
 
 
-def schedule(mesh, A_fp, A_ez, omega_all):
+def schedule(mesh, A_fp, A_ez, omega_all, par_tol=0.0):
     """Per-angle schedule for every direction: dict like TiledProblem.schedule()."""
     A = omega_all.shape[0]
     nh, zips, nz, nc = [], [], [], []
     for a in range(A):
-        rc, nextZ, nextC, zip_ = schedule_angle(mesh, A_fp, A_ez, omega_all[a])
+        rc, nextZ, nextC, zip_ = schedule_angle(mesh, A_fp, A_ez, omega_all[a], par_tol)
         if rc:
             raise RuntimeError(f"oracle schedule failed for angle {a + 1}: code {rc}")
         nh.append(len(zip_))

@@ -1,0 +1,87 @@
+"""Grey-transport acceleration on the GPU (per-group one-group GTA: S2 grey
+sweeps, stretched-TSA sweeps, BiCGStab) against the oracle's restatement
+(oracle/gta.py) and against plain source iteration's fixed point."""
+import numpy as np
+import pytest
+
+import umt_amd as U
+from oracle import gta as OG
+from oracle import oracle as O
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261205
+
+
+@pytest.fixture(scope="module")
+def thick():
+    """c ~ U[0.9, 0.999], sigma_t over five decades: plain SI needs ~10^3 iterations."""
+    p = U.TiledProblem(2, 2, 2, 2, 2)
+    G = 2
+    st, ss, qq = U.draw_cross_sections(SEED, p.nzones, G, (1e-2, 1e3), (0.9, 0.999), (0.0, 1.0))
+    m, g = p.mesh(), p.geometry()
+    gq = p.gta()
+    s2 = O.schedule(m, g["A_fp"], g["A_ez"], gq["omega"], par_tol=1e-12)
+    q2 = dict(omega=gq["omega"], weight=gq["weight"])
+    return p, G, st, ss, qq, q2, s2
+
+
+def _ctx(p, G, st, ss, qq, precision=64):
+    ctx = U.SweepContext(p, G, precision=precision)
+    ctx.set_sigt(st)
+    ctx.set_scattering(ss, qq)
+    ctx.set_acceleration("gta")
+    return ctx
+
+
+def test_gta_s2_schedule_matches_oracle(thick):
+    p, G, st, ss, qq, q2, s2 = thick
+    gq = p.gta()
+    for k in ("nHyperPlanes", "zonesInPlane", "nextZ"):
+        np.testing.assert_array_equal(gq[k], s2[k])
+
+
+def test_gta_iterates_match_oracle(thick):
+    """Three accelerated iterations, compared iterate by iterate."""
+    p, G, st, ss, qq, q2, s2 = thick
+    ctx = _ctx(p, G, st, ss, qq)
+    _, it, _, hist = ctx.source_iteration(tol=0.0, max_iters=3)
+    phi = ctx.phi()
+    ref, rit, _, rhist, _ = OG.source_iteration_gta(p.mesh(), p.geometry(), p.quadrature(),
+                                                    p.schedule(), q2, s2, st, ss, qq, 0.0, 3)
+    assert it == rit == 3
+    assert rel_err(phi, ref) < 1e-9
+    np.testing.assert_allclose(hist, rhist, rtol=1e-6)
+    ctx.close()
+
+
+def test_gta_converges_to_the_plain_fixed_point_in_few_iterations(thick):
+    p, G, st, ss, qq, q2, s2 = thick
+    ctx = _ctx(p, G, st, ss, qq)
+    ok, it, res, _ = ctx.source_iteration(tol=1e-10, max_iters=200)
+    assert ok and res < 1e-10
+    grey_its, grey_sweeps = ctx.gta_stats()
+    phi = ctx.phi()
+    ref, rit, rres, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), p.schedule(),
+                                           st, ss, qq, 1e-13, 20000, nthreads=4)
+    print(f"GTA {it} iterations ({grey_its} BiCGStab steps, {grey_sweeps} S2 sweeps); "
+          f"plain {rit} iterations")
+    assert rres < 1e-13
+    assert it * 10 < rit
+    assert rel_err(phi, ref) < 1e-7
+    ctx.close()
+
+
+def test_gta_with_fp32_sweeps(thick):
+    p, G, st, ss, qq, q2, s2 = thick
+    ctx = _ctx(p, G, st, ss, qq, precision=32)
+    ok, it, res, _ = ctx.source_iteration(tol=1e-8, max_iters=200)
+    assert ok
+    ref, _, _, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), p.schedule(),
+                                      st, ss, qq, 1e-13, 20000, nthreads=4)
+    err = rel_err(ctx.phi(), ref)
+    print(f"FP32 sweeps + GTA: {it} iterations, max rel phi error vs FP64 oracle {err:.2e}")
+    assert err < 1e-4
+    ctx.close()

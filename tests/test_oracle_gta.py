@@ -1,0 +1,85 @@
+"""The oracle's GTA restatement (oracle/gta.py) against what the mathematics
+fixes: one-group grey opacities, the grey-correction equation solved by the
+BiCGStab, no correction without scattering, and the accelerated iteration
+reaching plain source iteration's fixed point in far fewer sweeps."""
+import numpy as np
+import pytest
+
+import umt_amd as U
+from oracle import gta as OG
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def prob():
+    p = U.TiledProblem(2, 2, 1, 2, 2)
+    m, g = p.mesh(), p.geometry()
+    gq = p.gta()
+    s2 = O.schedule(m, g["A_fp"], g["A_ez"], gq["omega"], par_tol=1e-12)
+    q2 = dict(omega=gq["omega"], weight=gq["weight"])
+    return p, q2, s2
+
+
+def test_level_symmetric_s2_quadrature(prob):
+    p, q2, _ = prob
+    om, w = q2["omega"], q2["weight"]
+    assert om.shape == (8, 3)
+    np.testing.assert_allclose(np.abs(om), 0.577350269189625, rtol=0, atol=0)
+    assert abs(w.sum() - 4 * np.pi) < 1e-13
+    # same octant numbering as the product quadrature
+    assert np.array_equal(np.sign(om), np.sign(p.quadrature()["omega"][:8]))
+
+
+def test_grey_opacities_one_group():
+    sigt = np.array([[2.0, 1.0, 5.0]])
+    sigs = np.array([[1.5, 1e-12, 0.0]])
+    tot, scat, eps, stretched = OG.grey_opacities(sigt, sigs)
+    np.testing.assert_allclose(tot, [[2.0, 1.0 - 1e-12, 5.0]])
+    np.testing.assert_allclose(scat, [[1.5, 0.0, 0.0]])
+    np.testing.assert_allclose(eps, [[2.0, 1.0, 1.0]])   # 1/sqrt(1 - 0.75) = 2
+    np.testing.assert_allclose(stretched, [[1.0, 1.0 - 1e-12, 5.0]])
+    # stretched problem has no scattering: sigt/eps - eps sigma_a = 0
+    np.testing.assert_allclose(tot[0, 0] / eps[0, 0] - eps[0, 0] * (tot[0, 0] - scat[0, 0]), 0.0,
+                               atol=1e-15)
+
+
+def test_bicgstab_solves_the_grey_correction_equation(prob):
+    """F from the solver satisfies (I - M) F = M_src(gsrc) to the grey tolerance."""
+    p, q2, s2 = prob
+    G = 2
+    st, ss, _ = U.draw_cross_sections(3, p.nzones, G, (1e-1, 1e2), (0.5, 0.99), (0, 0))
+    solver = OG.GreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    rng = np.random.default_rng(0)
+    gsrc = rng.uniform(0.0, 1.0, size=(p.ncorner, G)) * np.repeat(ss, 8, axis=0)
+    half = np.ones((p.ncorner, G))
+    F, n = solver.solve(gsrc, half, eps_grey=1e-12, max_sweeps=201, eps_point=1e-12)
+    b = solver.grey_sweep(np.zeros_like(gsrc), gsrc)
+    resid = F - solver.grey_sweep(F) - b
+    assert np.linalg.norm(resid) < 1e-8 * np.linalg.norm(b)
+
+
+def test_no_scattering_means_no_correction(prob):
+    p, q2, s2 = prob
+    G = 1
+    st = np.full((p.nzones, G), 2.0)
+    ss = np.zeros((p.nzones, G))
+    qq = np.ones((p.nzones, G))
+    phi, it, res, _, _ = OG.source_iteration_gta(p.mesh(), p.geometry(), p.quadrature(),
+                                                 p.schedule(), q2, s2, st, ss, qq, 1e-12, 10)
+    ref, rit, _, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), p.schedule(),
+                                        st, ss, qq, 1e-12, 10)
+    assert it == rit == 2
+    np.testing.assert_array_equal(phi, ref)
+
+
+def test_accelerated_iteration_reaches_the_plain_fixed_point(prob):
+    p, q2, s2 = prob
+    G = 2
+    st, ss, qq = U.draw_cross_sections(7, p.nzones, G, (1e-2, 1e3), (0.9, 0.999), (0.0, 1.0))
+    args = (p.mesh(), p.geometry(), p.quadrature(), p.schedule())
+    phi, it, res, hist, sweeps = OG.source_iteration_gta(*args, q2, s2, st, ss, qq, 1e-11, 200,
+                                                         nthreads=4)
+    ref, rit, rres, _ = O.source_iteration(*args, st, ss, qq, 1e-13, 50000, nthreads=4)
+    assert res < 1e-11 and rres < 1e-13
+    assert it * 10 < rit
+    np.testing.assert_allclose(phi, ref, rtol=1e-7)

@@ -186,6 +186,30 @@ void product_quadrature(int npolar, int nazimuthal, Quadrature& q) {
   for (double& x : q.weight) x *= fac;
 }
 
+// Level-symmetric S2 as rt/quadxyz.F90 builds it for order 2: one direction
+// per octant with all three cosines equal to the tabulated 0.577350269189625
+// (mods/QuadratureData_mod.F90:711-713), octants numbered as in quadProduct,
+// weights normalised to 4*pi (rt/rtquad.F90:95-105). The grey-acceleration
+// sweeps use it (gtaorder = 2, driver/test_driver.cc:1206).
+void level_symmetric_s2(Quadrature& q) {
+  static const int sgn[8][3] = {{1, 1, 1}, {-1, 1, 1}, {-1, -1, 1}, {1, -1, 1},
+                                {1, 1, -1}, {-1, 1, -1}, {-1, -1, -1}, {1, -1, -1}};
+  const double a = 0.577350269189625;
+  q.npolar = 1;
+  q.nazimuthal = 1;
+  q.nangles = 8;
+  q.omega.assign(24, 0.0);
+  q.weight.assign(8, 0.0);
+  for (int o = 0; o < 8; ++o) {
+    for (int d = 0; d < 3; ++d) q.omega[3 * o + d] = sgn[o][d] * a;
+    q.weight[o] = 0.5 * M_PI;  // pi/2 per octant, normalised below
+  }
+  double sum = 0.0;
+  for (double w : q.weight) sum += w;
+  const double fac = 1.0 / (sum / (4.0 * M_PI));
+  for (double& w : q.weight) w *= fac;
+}
+
 namespace {
 inline uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;

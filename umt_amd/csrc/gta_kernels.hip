@@ -1,0 +1,225 @@
+// Kernels of the grey-transport acceleration (GTA) solve, applied per group:
+// each group is its own one-group ("grey") problem, for which Teton's grey
+// opacities reduce to that group's sigma_t and sigma_s
+// (rt/setGTAOpacity.F90:195-233 with one group and no emission coupling).
+//
+// Vectors are (G, C) FP64 arrays, groups fastest. Inner products are the
+// sigma_s V weighted sums of rt/scat_prod.F90 / rt/scat_prod1.F90, one per
+// group, reduced in a fixed order so every run gives the same bits.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace umt {
+
+namespace {
+constexpr double kWtiso = 0.07957747154594767;  // 1/(4 pi)
+
+inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+}  // namespace
+
+// Scattering-rate residual after a sweep: sigma_s (phi_half - phi_prev)
+// (rt/getCollisionRate.F90:64,87 for one group).
+__global__ __launch_bounds__(256) void gta_grey_source(const double* __restrict__ phiHalf,
+                                                       const double* __restrict__ phiPrev,
+                                                       const double* __restrict__ sigs, int G,
+                                                       long n, double* __restrict__ gsrc) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long c = i / G;
+  const int g = (int)(i - c * G);
+  gsrc[i] = sigs[(c >> 3) * G + g] * (phiHalf[i] - phiPrev[i]);
+}
+
+// Sources of the two grey sweeps (snac/GTASweep.F90:89-102):
+//   kind 1: wtiso (sigma_s P + gsrc)          (gsrc may be null)
+//   kind 2: wtiso sigma_s eps (P1 - Pold)     (stretched TSA step)
+template <typename Real>
+__global__ __launch_bounds__(256) void gta_sweep_source(int kind, const double* __restrict__ P,
+                                                        const double* __restrict__ other,
+                                                        const double* __restrict__ sigs,
+                                                        const double* __restrict__ eps, int G,
+                                                        long n, Real* __restrict__ S) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long c = i / G;
+  const int g = (int)(i - c * G);
+  const long zg = (c >> 3) * G + g;
+  const double p = P ? P[i] : 0.0;  // null P stands for zero
+  double v;
+  if (kind == 1)
+    v = kWtiso * (sigs[zg] * p + (other ? other[i] : 0.0));
+  else
+    v = kWtiso * sigs[zg] * eps[zg] * (other[i] - p);
+  S[i] = (Real)v;
+}
+
+// out = a_g x + b_g y + c_g z with per-group coefficients coef[3g..3g+2];
+// null inputs count as zero. out may alias any input.
+__global__ __launch_bounds__(256) void gta_lincomb(const double* x, const double* y,
+                                                   const double* z,
+                                                   const double* __restrict__ coef, int G,
+                                                   long n, double* out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int g = (int)(i % G);
+  double v = 0.0;
+  if (x) v += coef[3 * g + 0] * x[i];
+  if (y) v += coef[3 * g + 1] * y[i];
+  if (z) v += coef[3 * g + 2] * z[i];
+  out[i] = v;
+}
+
+// Per-group sums of x * y * sigma_s V (y null: x * sigma_s V). Block size is
+// a multiple of G and the grid stride a multiple of the block size, so each
+// thread always sees the same group; partial[block][g] is summed in thread
+// order, then gta_reduce_partials sums the blocks in block order.
+__global__ void gta_wdot_partial(const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ sigs, const double* __restrict__ vol,
+                                 int G, long n, double* __restrict__ partial) {
+  extern __shared__ double red[];
+  double acc = 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long c = i / G;
+    const int g = (int)(i - c * G);
+    const double w = sigs[(c >> 3) * G + g] * vol[c];
+    acc += x[i] * (y ? y[i] : 1.0) * w;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if ((int)threadIdx.x < G) {
+    double s = 0.0;
+    for (int t = threadIdx.x; t < (int)blockDim.x; t += G) s += red[t];
+    partial[(long)blockIdx.x * G + threadIdx.x] = s;
+  }
+}
+
+__global__ void gta_reduce_partials(const double* __restrict__ partial, int nblocks, int G,
+                                    double* __restrict__ out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += partial[(long)b * G + g];
+  out[g] = s;
+}
+
+// Grey convergence measure (rt/GTASolver.F90:318-356 per group): zone
+// correction pz, change against the previous pz, relative to the zone
+// average of phi_half + pz; per-block max of the pointwise relative change
+// and sums of V errZone^2 and V phiNew^2. pzOld is updated.
+__global__ void gta_zone_error(const double* __restrict__ F, const double* __restrict__ phiHalf,
+                               const double* __restrict__ vol, const double* __restrict__ volZ,
+                               int G, long nzg, double* __restrict__ pzOld,
+                               double* __restrict__ partial /* [block][3] */) {
+  __shared__ double smax[256], serr[256], sphi[256];
+  double mx = 0.0, e2 = 0.0, p2 = 0.0;
+  bool bad = false;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nzg; i += stride) {
+    const long z = i / G;
+    const int g = (int)(i - z * G);
+    double pz = 0.0, rad = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const long k = (8 * z + c) * G + g;
+      pz += vol[8 * z + c] * F[k];
+      rad += vol[8 * z + c] * phiHalf[k];
+    }
+    pz /= volZ[z];
+    rad /= volZ[z];
+    const double err = pz - pzOld[i];
+    const double phiNew = rad + pz;
+    e2 += volZ[z] * err * err;
+    p2 += volZ[z] * phiNew * phiNew;
+    // A NaN or Inf anywhere poisons the maximum: the host checks isfinite.
+    if (!isfinite(err) || !isfinite(phiNew)) bad = true;
+    else if (phiNew != 0.0) mx = fmax(mx, fabs(err / phiNew));
+    pzOld[i] = pz;
+  }
+  smax[threadIdx.x] = bad ? __builtin_nan("") : mx;
+  serr[threadIdx.x] = e2;
+  sphi[threadIdx.x] = p2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0, a = 0.0, b = 0.0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      const double v = smax[t];
+      m = (v != v || m != m) ? v : fmax(m, v);
+      a += serr[t];
+      b += sphi[t];
+    }
+    partial[3 * blockIdx.x + 0] = m;
+    partial[3 * blockIdx.x + 1] = a;
+    partial[3 * blockIdx.x + 2] = b;
+  }
+}
+
+__global__ void gta_reduce_error(const double* __restrict__ partial, int nblocks,
+                                 double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double m = 0.0, a = 0.0, b = 0.0;
+  for (int k = 0; k < nblocks; ++k) {
+    const double v = partial[3 * k];
+    m = (v != v || m != m) ? v : fmax(m, v);
+    a += partial[3 * k + 1];
+    b += partial[3 * k + 2];
+  }
+  out[0] = m;
+  out[1] = a;
+  out[2] = b;
+}
+
+// ----------------------------------------------------------------------------
+
+hipError_t launch_gta_grey_source(const double* phiHalf, const double* phiPrev, const double* sigs,
+                                  int G, long n, double* gsrc, hipStream_t s) {
+  hipLaunchKernelGGL(gta_grey_source, dim3(blocks_for(n, 256)), dim3(256), 0, s, phiHalf, phiPrev,
+                     sigs, G, n, gsrc);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_gta_sweep_source(int kind, const double* P, const double* other,
+                                   const double* sigs, const double* eps, int G, long n, Real* S,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL((gta_sweep_source<Real>), dim3(blocks_for(n, 256)), dim3(256), 0, s, kind, P,
+                     other, sigs, eps, G, n, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_gta_lincomb(const double* x, const double* y, const double* z,
+                              const double* coef, int G, long n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(gta_lincomb, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, y, z, coef, G, n,
+                     out);
+  return hipGetLastError();
+}
+
+int gta_dot_block(int G) { return G * (256 / G > 0 ? 256 / G : 1); }
+
+hipError_t launch_gta_wdot(const double* x, const double* y, const double* sigs, const double* vol,
+                           int G, long n, int nblocks, double* partial, double* out,
+                           hipStream_t s) {
+  const int bs = gta_dot_block(G);
+  hipLaunchKernelGGL(gta_wdot_partial, dim3(nblocks), dim3(bs), bs * sizeof(double), s, x, y, sigs,
+                     vol, G, n, partial);
+  hipLaunchKernelGGL(gta_reduce_partials, dim3(blocks_for(G, 64)), dim3(64), 0, s, partial, nblocks,
+                     G, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const double* vol,
+                                 const double* volZ, int G, long nzg, double* pzOld, int nblocks,
+                                 double* partial, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(gta_zone_error, dim3(nblocks), dim3(256), 0, s, F, phiHalf, vol, volZ, G, nzg,
+                     pzOld, partial);
+  hipLaunchKernelGGL(gta_reduce_error, dim3(1), dim3(64), 0, s, partial, nblocks, out);
+  return hipGetLastError();
+}
+
+template hipError_t launch_gta_sweep_source<double>(int, const double*, const double*,
+                                                    const double*, const double*, int, long,
+                                                    double*, hipStream_t);
+template hipError_t launch_gta_sweep_source<float>(int, const double*, const double*, const double*,
+                                                   const double*, int, long, float*, hipStream_t);
+
+}  // namespace umt

@@ -59,9 +59,13 @@ struct Schedule {
 void build_tiled_mesh(int nx, int ny, int nz, bool kdtree, Mesh& m);
 void compute_geometry(const Mesh& m, Geometry& g);
 void product_quadrature(int npolar, int nazimuthal, Quadrature& q);
+// Level-symmetric S2 (8 directions), the grey-acceleration quadrature.
+void level_symmetric_s2(Quadrature& q);
 // Returns 0 on success, nonzero (and a message) on an unsupported mesh cycle.
+// Corner faces with |Omega.A| <= parallelTol * |A|_1 are treated as parallel
+// to Omega (no dependency); parallelTol = 0 is the reference's exact test.
 int build_schedule(const Mesh& m, const Geometry& g, const Quadrature& q,
-                   Schedule& s, std::string& err);
+                   Schedule& s, std::string& err, double parallelTol = 0.0);
 
 // Counter-based uniform draw in [0,1) keyed by (seed, zone, group, field).
 double draw_uniform(uint64_t seed, uint64_t zone, uint64_t group, uint64_t field);

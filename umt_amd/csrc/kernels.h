@@ -43,6 +43,26 @@ hipError_t launch_residual(const double* phi, const double* vol, const double* v
 template <typename T>
 hipError_t launch_fill_zero(T* p, long n, hipStream_t s);
 hipError_t launch_copy_probe(const double* src, double* dst, long n, hipStream_t s);
+// phi = sum_a w_a psi (accumulate: phi += ...).
+template <typename Real>
+hipError_t launch_phi_from_psi_acc(const Real* psi, const double* w, int A, long stride, long n,
+                                   double* phi, bool accumulate, hipStream_t s);
+
+// Grey-transport acceleration (gta_kernels.hip).
+hipError_t launch_gta_grey_source(const double* phiHalf, const double* phiPrev, const double* sigs,
+                                  int G, long n, double* gsrc, hipStream_t s);
+template <typename Real>
+hipError_t launch_gta_sweep_source(int kind, const double* P, const double* other,
+                                   const double* sigs, const double* eps, int G, long n, Real* S,
+                                   hipStream_t s);
+hipError_t launch_gta_lincomb(const double* x, const double* y, const double* z,
+                              const double* coef, int G, long n, double* out, hipStream_t s);
+int gta_dot_block(int G);
+hipError_t launch_gta_wdot(const double* x, const double* y, const double* sigs, const double* vol,
+                           int G, long n, int nblocks, double* partial, double* out, hipStream_t s);
+hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const double* vol,
+                                 const double* volZ, int G, long nzg, double* pzOld, int nblocks,
+                                 double* partial, double* out, hipStream_t s);
 template <typename Dst, typename Src>
 hipError_t launch_convert(const Src* s, Dst* d, long n, hipStream_t s_);
 

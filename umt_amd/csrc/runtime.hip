@@ -84,36 +84,72 @@ struct umt_problem {
   umt::Geometry g;
   umt::Quadrature q;
   umt::Schedule s;
+  umt::Quadrature q2;  // level-symmetric S2 of the grey-acceleration sweeps
+  umt::Schedule s2;
+};
+
+// Grey-acceleration sweeps use directions that lie exactly in some faces
+// of the tile (|Omega.A| is rounding noise there); such corner faces are
+// treated as parallel below this fraction of the face area.
+static const double kGtaParallelTol = 1e-12;
+
+// One quadrature's sweep: directions, device-resident hyperplane batches
+// (level h = plane h of every angle) and psi for all its angles.
+struct SweepSet {
+  int A = 0;
+  double* omega = nullptr;   // (3, A)
+  double* weight = nullptr;  // (A)
+  int2* items = nullptr;     // (zone, angle), level by level
+  std::vector<int> levelOffset;
+  void* psi = nullptr;       // (G, C + nbelem, A), sweep precision
+  std::vector<hipEvent_t> evLevel;
+};
+
+// Grey-transport acceleration state (per-group one-group GTA).
+struct GtaState {
+  bool enabled = false;
+  void* sigt2 = nullptr;  // stretched sigma_t / eps, (G, Z), sweep precision
+  double *eps = nullptr;  // 1/sqrt(1 - c), (G, Z)
+  void* S2 = nullptr;     // grey sweep source, (G, C), sweep precision
+  double *F = nullptr, *r = nullptr, *d = nullptr, *Ad = nullptr, *As = nullptr, *P1 = nullptr;
+  double *gsrc = nullptr, *phiHalf = nullptr, *pzOld = nullptr;
+  double *partial = nullptr, *dots = nullptr, *coef = nullptr, *err = nullptr;
+  int dotBlocks = 0, errBlocks = 0;
+  int maxSweeps = 21;         // grey_max_sweeps (mods/default_iter_controls_mod.F90)
+  double epsPoint = 1.0e-6;   // grey_reltol
+  double epsGreyInit = 1.0e-5;  // rt/rtmainsn.F90:129
+  long long greyIterations = 0, greySweeps = 0;
 };
 
 struct umt_ctx {
-  int Z = 0, C = 0, nb = 0, G = 0, A = 0, precision = 64;
+  int Z = 0, C = 0, nb = 0, G = 0, precision = 64;
   hipStream_t stream = nullptr;
-  // Geometry and schedule (sweep precision for geometry).
+  // Geometry (sweep precision) and connectivity.
   void* Afp = nullptr;
   void* Aez = nullptr;
   void* vol = nullptr;
   int* cFP = nullptr;
-  double* omega = nullptr;
-  double* weight = nullptr;
   double* vol64 = nullptr;
   double* volZ = nullptr;
-  int2* items = nullptr;
-  std::vector<int> levelOffset;  // (levels + 1)
+  SweepSet sn;  // the S_N quadrature
+  SweepSet s2;  // S2 grey sweeps
   // Fields.
   void* S = nullptr;     // (G, C)
   void* sigt = nullptr;  // (G, Z)
   double* sigs = nullptr;
   double* q = nullptr;
-  void* psi = nullptr;  // (G, C+nb, A)
   double* phi = nullptr;
+  double* phiPrev = nullptr;
   double* phibar = nullptr;
   double* phibarOld = nullptr;
   unsigned long long* bits = nullptr;
   bool haveScattering = false;
+  bool haveSigt = false;
   int groupsPerLane = 1;  // K of the sweep kernels
-  double nIn = 0.0;  // mean incoming interior FP faces per (corner, angle)
-  std::vector<hipEvent_t> evLevel;  // 2 per level
+  int acceleration = 0;   // 0 plain source iteration, 1 GTA
+  std::vector<double> sigtHost, sigsHost;  // last uploaded cross sections (GTA opacities)
+  double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
+  GtaState gta;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   size_t rsize() const { return precision == 32 ? sizeof(float) : sizeof(double); }
@@ -123,10 +159,11 @@ struct umt_ctx {
 namespace {
 
 template <typename Real>
-umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, int level) {
+umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, const void* S,
+                                     const void* sigt, int level) {
   umt::SweepLevelArgs<Real> a{};
-  a.items = x->items + x->levelOffset[level];
-  a.nItems = x->levelOffset[level + 1] - x->levelOffset[level];
+  a.items = set.items + set.levelOffset[level];
+  a.nItems = set.levelOffset[level + 1] - set.levelOffset[level];
   a.G = x->G;
   a.groupsPerLane = x->groupsPerLane;
   a.C = x->C;
@@ -134,29 +171,41 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, int level) {
   a.Aez = (const Real*)x->Aez;
   a.vol = (const Real*)x->vol;
   a.cFP = x->cFP;
-  a.S = (const Real*)x->S;
-  a.sigt = (const Real*)x->sigt;
-  a.omega = x->omega;
-  a.psi = (Real*)x->psi;
+  a.S = (const Real*)S;
+  a.sigt = (const Real*)sigt;
+  a.omega = set.omega;
+  a.psi = (Real*)set.psi;
   a.psiAngleStride = x->psiStride();
   return a;
 }
 
+// One sweep of every angle of `set` with source S and cross section sigt,
+// then phiOut = (or +=) sum_a w_a psi.
 template <typename Real>
-int sweep_impl(umt_ctx* x, bool timeLevels) {
-  const int L = (int)x->levelOffset.size() - 1;
+int sweep_set_impl(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
+                   bool accumulate, bool timeLevels) {
+  const int L = (int)set.levelOffset.size() - 1;
   for (int h = 0; h < L; ++h) {
-    if (timeLevels) HIP_TRY(hipEventRecord(x->evLevel[2 * h], x->stream));
-    HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, h), false, x->stream));
-    if (timeLevels) HIP_TRY(hipEventRecord(x->evLevel[2 * h + 1], x->stream));
+    if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h], x->stream));
+    HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h), false, x->stream));
+    if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h + 1], x->stream));
   }
-  HIP_TRY(umt::launch_phi_from_psi<Real>((const Real*)x->psi, x->weight, x->A, x->psiStride(),
-                                         (long)x->C * x->G, x->phi, x->stream));
+  HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)set.psi, set.weight, set.A,
+                                             x->psiStride(), (long)x->C * x->G, phiOut, accumulate,
+                                             x->stream));
   return 0;
 }
 
+int sweep_set(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
+              bool accumulate, bool timeLevels) {
+  return x->precision == 32
+             ? sweep_set_impl<float>(x, set, S, sigt, phiOut, accumulate, timeLevels)
+             : sweep_set_impl<double>(x, set, S, sigt, phiOut, accumulate, timeLevels);
+}
+
+// The S_N sweep of the context: phi = sum_a w_a psi.
 int sweep(umt_ctx* x, bool timeLevels) {
-  return x->precision == 32 ? sweep_impl<float>(x, timeLevels) : sweep_impl<double>(x, timeLevels);
+  return sweep_set(x, x->sn, x->S, x->sigt, x->phi, false, timeLevels);
 }
 
 // Upload a host FP64 array into a device array of the sweep precision.
@@ -196,6 +245,12 @@ int umt_problem_tiled(int nx, int ny, int nz, int npolar, int nazimuthal, int kd
     if (rc) {
       delete p;
       return fail(20 + rc, "schedule: %s", err.c_str());
+    }
+    umt::level_symmetric_s2(p->q2);
+    rc = umt::build_schedule(p->m, p->g, p->q2, p->s2, err, kGtaParallelTol);
+    if (rc) {
+      delete p;
+      return fail(20 + rc, "grey-acceleration schedule: %s", err.c_str());
     }
   } catch (const std::exception& e) {
     delete p;
@@ -269,6 +324,18 @@ int umt_problem_schedule(const umt_problem* p, int* nHyperPlanes, int* zonesInPl
   return 0;
 }
 
+int umt_problem_gta(const umt_problem* p, int* nangles, double* omega, double* weight,
+                    int* nHyperPlanes, int* zonesInPlane, int* nextZ) {
+  if (!p) return fail(1, "null problem");
+  if (nangles) *nangles = p->q2.nangles;
+  copy_out(p->q2.omega, omega);
+  copy_out(p->q2.weight, weight);
+  copy_out(p->s2.nHyperPlanes, nHyperPlanes);
+  copy_out(p->s2.zonesInPlane, zonesInPlane);
+  copy_out(p->s2.nextZ, nextZ);
+  return 0;
+}
+
 int umt_draw_cross_sections(uint64_t seed, int nzones, int groups, double sigt_lo, double sigt_hi,
                             double c_lo, double c_hi, double q_lo, double q_hi, double* sigt,
                             double* sigs, double* q) {
@@ -290,17 +357,95 @@ int umt_draw_cross_sections(uint64_t seed, int nzones, int groups, double sigt_l
 // ---------------------------------------------------------------------------
 // Device-resident context
 
+static void free_set(SweepSet& set) {
+  dfree(set.omega);
+  dfree(set.weight);
+  dfree(set.items);
+  dfree(set.psi);
+  for (auto e : set.evLevel) (void)hipEventDestroy(e);
+  set.evLevel.clear();
+}
+
+static void free_gta(GtaState& g) {
+  dfree(g.sigt2); dfree(g.eps); dfree(g.S2); dfree(g.F); dfree(g.r); dfree(g.d); dfree(g.Ad);
+  dfree(g.As); dfree(g.P1); dfree(g.gsrc); dfree(g.phiHalf); dfree(g.pzOld); dfree(g.partial);
+  dfree(g.dots); dfree(g.coef); dfree(g.err);
+  g.enabled = false;
+}
+
 void umt_ctx_destroy(umt_ctx* x) {
   if (!x) return;
   if (x->stream) (void)hipStreamSynchronize(x->stream);
-  dfree(x->Afp); dfree(x->Aez); dfree(x->vol); dfree(x->cFP); dfree(x->omega); dfree(x->weight);
-  dfree(x->vol64); dfree(x->volZ); dfree(x->items); dfree(x->S); dfree(x->sigt); dfree(x->sigs);
-  dfree(x->q); dfree(x->psi); dfree(x->phi); dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
-  for (auto e : x->evLevel) (void)hipEventDestroy(e);
+  dfree(x->Afp); dfree(x->Aez); dfree(x->vol); dfree(x->cFP); dfree(x->vol64); dfree(x->volZ);
+  dfree(x->S); dfree(x->sigt); dfree(x->sigs); dfree(x->q); dfree(x->phi); dfree(x->phiPrev);
+  dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
+  free_set(x->sn);
+  free_set(x->s2);
+  free_gta(x->gta);
   if (x->ev0) (void)hipEventDestroy(x->ev0);
   if (x->ev1) (void)hipEventDestroy(x->ev1);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
+}
+
+// Hyperplane zone batches of one quadrature: level h holds plane h of every
+// angle, angles in order, zones in each plane's nextZ order.
+static int batch_levels(int Z, int A, const int* nHyperPlanes, const int* zonesInPlane,
+                        const int* nextZ, std::vector<int2>& items, std::vector<int>& levelOffset) {
+  std::vector<int> planeStart(A + 1, 0);
+  int maxPlanes = 0;
+  for (int a = 0; a < A; ++a) {
+    if (nHyperPlanes[a] < 1) return fail(16, "angle %d has no hyperplanes", a + 1);
+    planeStart[a + 1] = planeStart[a] + nHyperPlanes[a];
+    maxPlanes = std::max(maxPlanes, nHyperPlanes[a]);
+  }
+  std::vector<std::vector<int2>> lv(maxPlanes);
+  for (int a = 0; a < A; ++a) {
+    long done = 0;
+    for (int h = 0; h < nHyperPlanes[a]; ++h) {
+      const int n = zonesInPlane[planeStart[a] + h];
+      if (n < 0 || done + n > Z) return fail(16, "angle %d: hyperplane sizes exceed the mesh", a + 1);
+      for (int i = 0; i < n; ++i) {
+        const int z = nextZ[(long)a * Z + done + i];
+        if (z <= 0 || z > Z) return fail(15, "angle %d: nextZ entry %d (cyclic or out of range)", a + 1, z);
+        lv[h].push_back(make_int2(z - 1, a));
+      }
+      done += n;
+    }
+    if (done != Z) return fail(16, "angle %d: hyperplanes hold %ld zones, mesh has %d", a + 1, done, Z);
+  }
+  items.clear();
+  items.reserve((size_t)Z * A);
+  levelOffset.assign(1, 0);
+  for (int h = 0; h < maxPlanes; ++h) {
+    items.insert(items.end(), lv[h].begin(), lv[h].end());
+    levelOffset.push_back((int)items.size());
+  }
+  return 0;
+}
+
+static int upload_set(umt_ctx* x, SweepSet& set, int A, const double* omega, const double* weight,
+                      const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ) {
+  std::vector<int2> items;
+  if (int rc = batch_levels(x->Z, A, nHyperPlanes, zonesInPlane, nextZ, items, set.levelOffset))
+    return rc;
+  set.A = A;
+  const size_t nPsi = (size_t)x->psiStride() * A;
+  int rc;
+  if ((rc = dalloc(&set.omega, 3 * (size_t)A)) || (rc = dalloc(&set.weight, (size_t)A)) ||
+      (rc = dalloc(&set.items, items.size())) || (rc = dalloc((char**)&set.psi, nPsi * x->rsize())))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(set.omega, omega, 3 * (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(set.weight, weight, (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(set.items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, x->stream));
+  if (x->precision == 32)
+    HIP_TRY(umt::launch_fill_zero<float>((float*)set.psi, (long)nPsi, x->stream));
+  else
+    HIP_TRY(umt::launch_fill_zero<double>((double*)set.psi, (long)nPsi, x->stream));
+  set.evLevel.resize(2 * (set.levelOffset.size() - 1));
+  for (auto& e : set.evLevel) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
 }
 
 static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, const int* numCorner,
@@ -314,27 +459,10 @@ static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, co
   if (int rc = check_hex_topology(Z, C, numCorner, cOffSet, nCFaces, cEZ)) return rc;
   for (long i = 0; i < 3L * C; ++i)
     if (cFP[i] < 1 || cFP[i] > C + nb) return fail(14, "cFP entry %ld out of range", i + 1);
-
-  // Hyperplane zone batches: level h holds plane h of every angle.
-  std::vector<int> planeStart(A + 1, 0);
-  int maxPlanes = 0;
-  for (int a = 0; a < A; ++a) {
-    planeStart[a + 1] = planeStart[a] + nHyperPlanes[a];
-    maxPlanes = std::max(maxPlanes, nHyperPlanes[a]);
-  }
-  std::vector<std::vector<int2>> lv(maxPlanes);
-  for (int a = 0; a < A; ++a) {
-    long done = 0;
-    for (int h = 0; h < nHyperPlanes[a]; ++h) {
-      const int n = zonesInPlane[planeStart[a] + h];
-      for (int i = 0; i < n; ++i) {
-        const int z = nextZ[(long)a * Z + done + i];
-        if (z <= 0 || z > Z) return fail(15, "angle %d: nextZ entry %d (cyclic or out of range)", a + 1, z);
-        lv[h].push_back(make_int2(z - 1, a));
-      }
-      done += n;
-    }
-    if (done != Z) return fail(16, "angle %d: hyperplanes hold %ld zones, mesh has %d", a + 1, done, Z);
+  {  // validate the schedule before touching the device
+    std::vector<int2> items;
+    std::vector<int> lo;
+    if (int rc = batch_levels(Z, A, nHyperPlanes, zonesInPlane, nextZ, items, lo)) return rc;
   }
   // n_in of the algorithmic-bytes model, counted exactly.
   {
@@ -349,32 +477,21 @@ static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, co
     }
     x->nIn = (double)cnt / ((double)C * A);
   }
-  std::vector<int2> items;
-  items.reserve((size_t)Z * A);
-  x->levelOffset.assign(1, 0);
-  for (int h = 0; h < maxPlanes; ++h) {
-    items.insert(items.end(), lv[h].begin(), lv[h].end());
-    x->levelOffset.push_back((int)items.size());
-  }
 
-  x->Z = Z; x->C = C; x->nb = nb; x->G = G; x->A = A; x->precision = precision;
+  x->Z = Z; x->C = C; x->nb = nb; x->G = G; x->precision = precision;
   HIP_TRY(hipStreamCreate(&x->stream));
   HIP_TRY(hipEventCreate(&x->ev0));
   HIP_TRY(hipEventCreate(&x->ev1));
-  x->evLevel.resize(2 * (size_t)maxPlanes);
-  for (auto& e : x->evLevel) HIP_TRY(hipEventCreate(&e));
 
   const size_t rs = x->rsize();
-  const size_t nPsi = (size_t)(C + nb) * G * A;
   int rc = 0;
   if ((rc = dalloc((char**)&x->Afp, 9 * (size_t)C * rs)) || (rc = dalloc((char**)&x->Aez, 9 * (size_t)C * rs)) ||
       (rc = dalloc((char**)&x->vol, (size_t)C * rs)) || (rc = dalloc(&x->cFP, 3 * (size_t)C)) ||
-      (rc = dalloc(&x->omega, 3 * (size_t)A)) || (rc = dalloc(&x->weight, (size_t)A)) ||
       (rc = dalloc(&x->vol64, (size_t)C)) || (rc = dalloc(&x->volZ, (size_t)Z)) ||
-      (rc = dalloc(&x->items, items.size())) || (rc = dalloc((char**)&x->S, (size_t)G * C * rs)) ||
+      (rc = dalloc((char**)&x->S, (size_t)G * C * rs)) ||
       (rc = dalloc((char**)&x->sigt, (size_t)G * Z * rs)) || (rc = dalloc(&x->sigs, (size_t)G * Z)) ||
-      (rc = dalloc(&x->q, (size_t)G * Z)) || (rc = dalloc((char**)&x->psi, nPsi * rs)) ||
-      (rc = dalloc(&x->phi, (size_t)G * C)) || (rc = dalloc(&x->phibar, (size_t)G * Z)) ||
+      (rc = dalloc(&x->q, (size_t)G * Z)) || (rc = dalloc(&x->phi, (size_t)G * C)) ||
+      (rc = dalloc(&x->phiPrev, (size_t)G * C)) || (rc = dalloc(&x->phibar, (size_t)G * Z)) ||
       (rc = dalloc(&x->phibarOld, (size_t)G * Z)) || (rc = dalloc(&x->bits, 2)))
     return rc;
 
@@ -382,22 +499,20 @@ static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, co
       (rc = upload_real(x, x->vol, Volume, (size_t)C)))
     return rc;
   HIP_TRY(hipMemcpyAsync(x->cFP, cFP, 3 * (size_t)C * sizeof(int), hipMemcpyHostToDevice, x->stream));
-  HIP_TRY(hipMemcpyAsync(x->omega, omega, 3 * (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
-  HIP_TRY(hipMemcpyAsync(x->weight, weight, (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipMemcpyAsync(x->vol64, Volume, (size_t)C * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipMemcpyAsync(x->volZ, VolumeZone, (size_t)Z * sizeof(double), hipMemcpyHostToDevice, x->stream));
-  HIP_TRY(hipMemcpyAsync(x->items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, x->stream));
   if (precision == 32) {
-    HIP_TRY(umt::launch_fill_zero<float>((float*)x->psi, (long)nPsi, x->stream));
     HIP_TRY(umt::launch_fill_zero<float>((float*)x->S, (long)G * C, x->stream));
     HIP_TRY(umt::launch_fill_zero<float>((float*)x->sigt, (long)G * Z, x->stream));
   } else {
-    HIP_TRY(umt::launch_fill_zero<double>((double*)x->psi, (long)nPsi, x->stream));
     HIP_TRY(umt::launch_fill_zero<double>((double*)x->S, (long)G * C, x->stream));
     HIP_TRY(umt::launch_fill_zero<double>((double*)x->sigt, (long)G * Z, x->stream));
   }
   HIP_TRY(umt::launch_fill_zero<double>(x->phi, (long)G * C, x->stream));
   HIP_TRY(umt::launch_fill_zero<double>(x->phibarOld, (long)G * Z, x->stream));
+  HIP_TRY(umt::launch_fill_zero<double>(x->sigs, (long)G * Z, x->stream));
+  HIP_TRY(umt::launch_fill_zero<double>(x->q, (long)G * Z, x->stream));
+  if ((rc = upload_set(x, x->sn, A, omega, weight, nHyperPlanes, zonesInPlane, nextZ))) return rc;
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
 }
@@ -426,19 +541,29 @@ int umt_ctx_create(int nzones, int ncorner, int nbelem, int groups, int nangles,
 
 int umt_ctx_create_from_problem(const umt_problem* p, int groups, int precision, umt_ctx** out) {
   if (!p) return fail(1, "null problem");
-  return umt_ctx_create(p->m.nzones, p->m.ncorner, p->m.nbelem, groups, p->q.nangles, precision,
-                        p->m.numCorner.data(), p->m.cOffSet.data(), p->m.nCFaces.data(),
-                        p->m.cFP.data(), p->m.cEZ.data(), p->g.A_fp.data(), p->g.A_ez.data(),
-                        p->g.Volume.data(), p->g.VolumeZone.data(), p->q.omega.data(),
-                        p->q.weight.data(), p->s.nHyperPlanes.data(), p->s.zonesInPlane.data(),
-                        p->s.nextZ.data(), out);
+  int rc = umt_ctx_create(p->m.nzones, p->m.ncorner, p->m.nbelem, groups, p->q.nangles, precision,
+                          p->m.numCorner.data(), p->m.cOffSet.data(), p->m.nCFaces.data(),
+                          p->m.cFP.data(), p->m.cEZ.data(), p->g.A_fp.data(), p->g.A_ez.data(),
+                          p->g.Volume.data(), p->g.VolumeZone.data(), p->q.omega.data(),
+                          p->q.weight.data(), p->s.nHyperPlanes.data(), p->s.zonesInPlane.data(),
+                          p->s.nextZ.data(), out);
+  if (rc) return rc;
+  rc = umt_ctx_enable_gta(*out, p->q2.nangles, p->q2.omega.data(), p->q2.weight.data(),
+                          p->s2.nHyperPlanes.data(), p->s2.zonesInPlane.data(), p->s2.nextZ.data());
+  if (rc) {
+    std::string keep = g_err;
+    umt_ctx_destroy(*out);
+    *out = nullptr;
+    g_err = keep;
+  }
+  return rc;
 }
 
 int umt_ctx_dims(const umt_ctx* x, int64_t* d) {
   if (!x || !d) return fail(1, "null argument");
-  d[0] = x->Z; d[1] = x->C; d[2] = x->nb; d[3] = x->G; d[4] = x->A;
-  d[5] = (int64_t)x->levelOffset.size() - 1;
-  d[6] = x->levelOffset.back();
+  d[0] = x->Z; d[1] = x->C; d[2] = x->nb; d[3] = x->G; d[4] = x->sn.A;
+  d[5] = (int64_t)x->sn.levelOffset.size() - 1;
+  d[6] = x->sn.levelOffset.back();
   return 0;
 }
 
@@ -449,6 +574,8 @@ int umt_ctx_set_sigt(umt_ctx* x, const double* sigt) {
   int rc = upload_real(x, x->sigt, sigt, (size_t)x->G * x->Z);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(x->stream));
+  x->haveSigt = true;
+  x->sigtHost.assign(sigt, sigt + (size_t)x->G * x->Z);
   return 0;
 }
 
@@ -472,6 +599,7 @@ int umt_ctx_set_scattering(umt_ctx* x, const double* sigs, const double* q) {
   HIP_TRY(hipMemcpyAsync(x->q, q, n * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
   x->haveScattering = true;
+  x->sigsHost.assign(sigs, sigs + n);
   return 0;
 }
 
@@ -584,18 +712,272 @@ int umt_ctx_set_phi(umt_ctx* x, const double* phi) {
 
 int umt_ctx_get_psi(const umt_ctx* x, double* psi) {
   if (!x || !psi) return fail(1, "null argument");
-  const size_t n = (size_t)x->psiStride() * x->A;
+  const size_t n = (size_t)x->psiStride() * x->sn.A;
   if (x->precision == 64) {
-    HIP_TRY(hipMemcpyAsync(psi, x->psi, n * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipMemcpyAsync(psi, x->sn.psi, n * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIP_TRY(hipStreamSynchronize(x->stream));
   } else {
     std::vector<float> tmp(n);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), x->psi, n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipMemcpyAsync(tmp.data(), x->sn.psi, n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
     HIP_TRY(hipStreamSynchronize(x->stream));
     for (size_t i = 0; i < n; ++i) psi[i] = tmp[i];
   }
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Grey-transport acceleration, one one-group GTA problem per group
+// (rt/GTASolver.F90, rt/GreySweep.F90, snac/GTASweep.F90).
+
+int umt_ctx_enable_gta(umt_ctx* x, int nangles, const double* omega, const double* weight,
+                       const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ) {
+  if (!x || !omega || !weight || !nHyperPlanes || !zonesInPlane || !nextZ) return fail(1, "null argument");
+  if (nangles < 1) return fail(1, "bad angle count");
+  if (x->gta.enabled) {
+    free_set(x->s2);
+    free_gta(x->gta);
+  }
+  if (int rc = upload_set(x, x->s2, nangles, omega, weight, nHyperPlanes, zonesInPlane, nextZ)) return rc;
+  GtaState& g = x->gta;
+  const size_t GC = (size_t)x->G * x->C, GZ = (size_t)x->G * x->Z, rs = x->rsize();
+  g.dotBlocks = 512;
+  g.errBlocks = 512;
+  int rc;
+  if ((rc = dalloc((char**)&g.sigt2, 2 * GZ * rs)) || (rc = dalloc(&g.eps, 2 * GZ)) ||
+      (rc = dalloc((char**)&g.S2, GC * rs)) || (rc = dalloc(&g.F, GC)) || (rc = dalloc(&g.r, GC)) ||
+      (rc = dalloc(&g.d, GC)) || (rc = dalloc(&g.Ad, GC)) || (rc = dalloc(&g.As, GC)) ||
+      (rc = dalloc(&g.P1, GC)) || (rc = dalloc(&g.gsrc, GC)) || (rc = dalloc(&g.phiHalf, GC)) ||
+      (rc = dalloc(&g.pzOld, GZ)) ||
+      (rc = dalloc(&g.partial, (size_t)std::max(g.dotBlocks * x->G, 3 * g.errBlocks))) ||
+      (rc = dalloc(&g.dots, 4 * (size_t)x->G)) || (rc = dalloc(&g.coef, 3 * (size_t)x->G)) ||
+      (rc = dalloc(&g.err, 3)))
+    return rc;
+  g.enabled = true;
+  return 0;
+}
+
+int umt_ctx_set_acceleration(umt_ctx* x, int mode) {
+  if (!x) return fail(1, "null context");
+  if (mode != 0 && mode != 1) return fail(36, "acceleration must be 0 (none) or 1 (GTA)");
+  if (mode == 1 && !x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
+  x->acceleration = mode;
+  return 0;
+}
+
+int umt_ctx_gta_stats(const umt_ctx* x, int64_t* greyIterations, int64_t* greySweeps) {
+  if (!x) return fail(1, "null context");
+  if (greyIterations) *greyIterations = x->gta.greyIterations;
+  if (greySweeps) *greySweeps = x->gta.greySweeps;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Grey opacities for one group (rt/setGTAOpacity.F90:195-233): sigma_s kept
+// only above 1e-10 of sigma_t (else sigma_t -> sigma_a), eps = 1/sqrt(1-c)
+// so that the stretched problem has no scattering, stretched sigma_t/eps.
+// Layout on the device: sigt2 = [grey sigma_t (G,Z) | stretched (G,Z)] in
+// sweep precision, eps = [eps (G,Z) | grey sigma_s (G,Z)] in FP64.
+int gta_prepare(umt_ctx* x) {
+  const size_t GZ = (size_t)x->G * x->Z;
+  if (x->sigtHost.size() != GZ || x->sigsHost.size() != GZ)
+    return fail(38, "GTA needs sigma_t and sigma_s (umt_ctx_set_sigt, umt_ctx_set_scattering)");
+  std::vector<double> st(2 * GZ), ep(2 * GZ);
+  for (size_t i = 0; i < GZ; ++i) {
+    const double sigt = x->sigtHost[i], sigs = x->sigsHost[i], siga = sigt - sigs;
+    double greyScat = sigs, greyTot = sigt;
+    if (sigs / sigt <= 1.0e-10) {
+      greyScat = 0.0;
+      greyTot = siga;
+    }
+    const double ratio = greyScat / greyTot;
+    const double eps = 1.0 / std::sqrt(1.0 - ratio);
+    st[i] = greyTot;
+    st[GZ + i] = greyTot / eps;
+    ep[i] = eps;
+    ep[GZ + i] = greyScat;
+  }
+  GtaState& g = x->gta;
+  if (int rc = upload_real(x, g.sigt2, st.data(), 2 * GZ)) return rc;
+  HIP_TRY(hipMemcpyAsync(g.eps, ep.data(), 2 * GZ * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+template <typename Real>
+int gta_source(umt_ctx* x, int kind, const double* P, const double* other) {
+  GtaState& g = x->gta;
+  const size_t GZ = (size_t)x->G * x->Z;
+  return umt::launch_gta_sweep_source<Real>(kind, P, other, g.eps + GZ, g.eps, x->G,
+                                            (long)x->G * x->C, (Real*)g.S2, x->stream) == hipSuccess
+             ? 0
+             : fail(101, "grey source launch failed");
+}
+
+// out = M(P): grey S2 sweep of wtiso (sigma_s P [+ gsrc]) followed by the
+// stretched-TSA sweep of wtiso sigma_s eps (out - P) (rt/GreySweep.F90:60-96).
+// P == nullptr stands for zero. out must differ from P.
+int grey_sweep(umt_ctx* x, const double* P, bool withSource, double* out) {
+  GtaState& g = x->gta;
+  const size_t GZ = (size_t)x->G * x->Z;
+  const char* st = (const char*)g.sigt2;
+  const void* sigGrey = st;
+  const void* sigStretch = st + GZ * x->rsize();
+  int rc;
+  if (x->precision == 32) rc = gta_source<float>(x, 1, P, withSource ? g.gsrc : nullptr);
+  else rc = gta_source<double>(x, 1, P, withSource ? g.gsrc : nullptr);
+  if (rc) return rc;
+  if ((rc = sweep_set(x, x->s2, g.S2, sigGrey, out, false, false))) return rc;
+  if (x->precision == 32) rc = gta_source<float>(x, 2, P, out);
+  else rc = gta_source<double>(x, 2, P, out);
+  if (rc) return rc;
+  if ((rc = sweep_set(x, x->s2, g.S2, sigStretch, out, true, false))) return rc;
+  g.greySweeps += 2;
+  return 0;
+}
+
+int lincomb(umt_ctx* x, const double* a, const double* b, const double* c, double* out,
+            const std::vector<double>& coef) {
+  HIP_TRY(hipMemcpyAsync(x->gta.coef, coef.data(), coef.size() * sizeof(double),
+                         hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(umt::launch_gta_lincomb(a, b, c, x->gta.coef, x->G, (long)x->G * x->C, out, x->stream));
+  // coef is host memory reused by the next call: finish this one first.
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int wdot(umt_ctx* x, const double* a, const double* b, std::vector<double>& out) {
+  GtaState& g = x->gta;
+  const size_t GZ = (size_t)x->G * x->Z;
+  HIP_TRY(umt::launch_gta_wdot(a, b, g.eps + GZ, x->vol64, x->G, (long)x->G * x->C, g.dotBlocks,
+                               g.partial, g.dots, x->stream));
+  out.resize(x->G);
+  HIP_TRY(hipMemcpyAsync(out.data(), g.dots, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+// BiCGStab on the grey corrections (rt/GTASolver.F90:185-402), the G
+// one-group problems in lockstep with their own coefficients. phiHalf is the
+// multigroup phi after the sweep; on return g.F holds the correction.
+int gta_solve(umt_ctx* x, const double* phiHalf, double epsGrey) {
+  GtaState& g = x->gta;
+  const int G = x->G;
+  const long GC = (long)G * x->C, GZ = (long)G * x->Z;
+  const double tiny = 1.0e-300;  // adqtSmall-style floor on inner products
+  HIP_TRY(umt::launch_fill_zero<double>(g.F, GC, x->stream));
+  HIP_TRY(umt::launch_fill_zero<double>(g.pzOld, GZ, x->stream));
+  int rc;
+  if ((rc = grey_sweep(x, nullptr, true, g.r))) return rc;
+  {
+    std::vector<double> cf(3 * G, 0.0);
+    for (int k = 0; k < G; ++k) cf[3 * k] = 1.0;
+    if ((rc = lincomb(x, g.r, nullptr, nullptr, g.d, cf))) return rc;
+  }
+  std::vector<double> rrOld, dAd, wn, wd, rr;
+  if ((rc = wdot(x, g.r, nullptr, rrOld))) return rc;
+  std::vector<char> active(G, 1);
+  int nGreyIter = 1;
+  for (;;) {
+    bool any = false;
+    for (int k = 0; k < G; ++k) {
+      if (active[k] && std::fabs(rrOld[k]) < tiny) active[k] = 0;
+      any = any || active[k];
+    }
+    if (!any) {
+      if (nGreyIter <= 2) {  // no iteration ran: the first residual is the correction
+        std::vector<double> cf(3 * G, 0.0);
+        for (int k = 0; k < G; ++k) cf[3 * k] = 1.0;
+        if ((rc = lincomb(x, g.r, nullptr, nullptr, g.F, cf))) return rc;
+      }
+      break;
+    }
+    nGreyIter += 2;
+    ++g.greyIterations;
+    // Ad = d - M d
+    if ((rc = grey_sweep(x, g.d, false, g.Ad))) return rc;
+    {
+      std::vector<double> cf(3 * G);
+      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -1.0; cf[3 * k + 2] = 0.0; }
+      if ((rc = lincomb(x, g.d, g.Ad, nullptr, g.Ad, cf))) return rc;
+    }
+    if ((rc = wdot(x, g.Ad, nullptr, dAd))) return rc;
+    std::vector<double> alpha(G, 0.0);
+    for (int k = 0; k < G; ++k) {
+      if (active[k] && std::fabs(dAd[k]) < tiny) active[k] = 0;
+      alpha[k] = active[k] ? rrOld[k] / dAd[k] : 0.0;
+    }
+    // r = r - alpha Ad
+    {
+      std::vector<double> cf(3 * G);
+      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -alpha[k]; cf[3 * k + 2] = 0.0; }
+      if ((rc = lincomb(x, g.r, g.Ad, nullptr, g.r, cf))) return rc;
+    }
+    // As = r - M r
+    if ((rc = grey_sweep(x, g.r, false, g.As))) return rc;
+    {
+      std::vector<double> cf(3 * G);
+      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -1.0; cf[3 * k + 2] = 0.0; }
+      if ((rc = lincomb(x, g.r, g.As, nullptr, g.As, cf))) return rc;
+    }
+    if ((rc = wdot(x, g.As, g.r, wn)) || (rc = wdot(x, g.As, g.As, wd))) return rc;
+    std::vector<double> omega(G, 0.0);
+    std::vector<char> stopAfter(G, 0);
+    for (int k = 0; k < G; ++k) {
+      if (!active[k]) continue;
+      if (std::fabs(wd[k]) < tiny || std::fabs(wn[k]) < tiny) stopAfter[k] = 1;  // F += alpha d only
+      else omega[k] = wn[k] / wd[k];
+    }
+    // F = F + alpha d + omega r
+    {
+      std::vector<double> cf(3 * G);
+      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = alpha[k]; cf[3 * k + 2] = omega[k]; }
+      if ((rc = lincomb(x, g.F, g.d, g.r, g.F, cf))) return rc;
+    }
+    // r = r - omega As
+    {
+      std::vector<double> cf(3 * G);
+      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -omega[k]; cf[3 * k + 2] = 0.0; }
+      if ((rc = lincomb(x, g.r, g.As, nullptr, g.r, cf))) return rc;
+    }
+    if ((rc = wdot(x, g.r, nullptr, rr))) return rc;
+    // d = r + beta (d - omega Ad)
+    {
+      std::vector<double> cf(3 * G);
+      for (int k = 0; k < G; ++k) {
+        const double beta = (active[k] && !stopAfter[k] && omega[k] != 0.0)
+                                ? (rr[k] * alpha[k]) / (rrOld[k] * omega[k]) : 0.0;
+        cf[3 * k] = beta;
+        cf[3 * k + 1] = -beta * omega[k];
+        cf[3 * k + 2] = 1.0;
+      }
+      if ((rc = lincomb(x, g.d, g.Ad, g.r, g.d, cf))) return rc;
+    }
+    for (int k = 0; k < G; ++k)
+      if (stopAfter[k]) active[k] = 0;
+    // Convergence of the grey corrections (rt/GTASolver.F90:318-370).
+    HIP_TRY(umt::launch_gta_zone_error(g.F, phiHalf, x->vol64, x->volZ, G, GZ, g.pzOld, g.errBlocks,
+                                       g.partial, g.err, x->stream));
+    double e[3];
+    HIP_TRY(hipMemcpyAsync(e, g.err, sizeof e, hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+    if (!std::isfinite(e[0]) || !std::isfinite(e[1]) || !std::isfinite(e[2]))
+      return fail(39, "grey solver produced a NaN at grey iteration %d", nGreyIter);
+    const double relL2 = e[2] != 0.0 ? std::sqrt(std::fabs(e[1] / e[2])) : 0.0;
+    const double errGrey = std::max(e[0], relL2);
+    if ((errGrey < g.epsPoint || nGreyIter >= g.maxSweeps) && errGrey < epsGrey) break;
+    if (nGreyIter >= 100 * g.maxSweeps)
+      return fail(40, "grey solver not converging after %d grey iterations (error %.3e)", nGreyIter, errGrey);
+    rrOld = rr;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
 
 int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, double* residual,
                              double* hist) {
@@ -607,12 +989,27 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
   // Zone averages of the starting phi are the "old" values of iteration 1.
   HIP_TRY(umt::launch_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld, 1e-6,
                                x->bits, x->stream));
+  const bool gta = x->acceleration == 1;
+  if (gta)
+    if (int rc = gta_prepare(x)) return rc;
+  double epsGrey = x->gta.epsGreyInit;
   while (it < maxIters) {
+    if (gta)
+      HIP_TRY(hipMemcpyAsync(x->phiPrev, x->phi, nGC * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
     if (x->precision == 32)
       HIP_TRY(umt::launch_source_update<float>(x->phi, x->sigs, x->q, x->G, nGC, (float*)x->S, x->stream));
     else
       HIP_TRY(umt::launch_source_update<double>(x->phi, x->sigs, x->q, x->G, nGC, (double*)x->S, x->stream));
     if (int rc = sweep(x, false)) return rc;
+    if (gta) {
+      // Grey source from the scattering-rate change, solve, add the
+      // correction (rt/LinearSolver.F90:81-111, rt/addGreyCorrections.F90).
+      HIP_TRY(umt::launch_gta_grey_source(x->phi, x->phiPrev, x->sigs, x->G, nGC, x->gta.gsrc, x->stream));
+      if (int rc = gta_solve(x, x->phi, epsGrey)) return rc;
+      std::vector<double> cf(3 * (size_t)x->G);
+      for (int k = 0; k < x->G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = 1.0; cf[3 * k + 2] = 0.0; }
+      if (int rc = lincomb(x, x->phi, x->gta.F, nullptr, x->phi, cf)) return rc;
+    }
     HIP_TRY(umt::launch_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld, 1e-6,
                                  x->bits, x->stream));
     unsigned long long b[2];
@@ -624,6 +1021,7 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
     res = r;
     if (hist) hist[it] = r;
     ++it;
+    epsGrey = res / 20.0;  // rt/rtmainsn.F90:220
     if (res < tol) break;
   }
   if (iters) *iters = it;
@@ -637,7 +1035,7 @@ int umt_ctx_time_sweeps(umt_ctx* x, int warmup, int reps, double* msSweep, doubl
   for (int i = 0; i < warmup; ++i)
     if (int rc = sweep(x, false)) return rc;
   HIP_TRY(hipStreamSynchronize(x->stream));
-  const int L = (int)x->levelOffset.size() - 1;
+  const int L = (int)x->sn.levelOffset.size() - 1;
   double sumLevels = 0.0;
   HIP_TRY(hipEventRecord(x->ev0, x->stream));
   for (int r = 0; r < reps; ++r) {
@@ -645,7 +1043,7 @@ int umt_ctx_time_sweeps(umt_ctx* x, int warmup, int reps, double* msSweep, doubl
     HIP_TRY(hipStreamSynchronize(x->stream));
     for (int h = 0; h < L; ++h) {
       float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, x->evLevel[2 * h], x->evLevel[2 * h + 1]));
+      HIP_TRY(hipEventElapsedTime(&ms, x->sn.evLevel[2 * h], x->sn.evLevel[2 * h + 1]));
       sumLevels += ms;
     }
   }

@@ -8,6 +8,7 @@
 // are rejected with an error: cycle breaking (snac/cyclebreaker.F90,
 // snac/sccsearch.F90) and lagged cycle fluxes are not built here.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 
 #include "host.h"
@@ -20,9 +21,19 @@ inline double dot3(const double* o, const double* a) {
   return o[0] * a[0] + o[1] * a[1] + o[2] * a[2];
 }
 
+// Omega.A, or exactly 0 when the corner face is parallel to Omega within
+// tol * |A|_1 (directions lying in a mesh face leave rounding noise of
+// either sign; with tol = 0 this is the plain dot product).
+inline double face_dot(const double* o, const double* a, double tol) {
+  const double d = dot3(o, a);
+  if (tol > 0.0 && std::fabs(d) <= tol * (std::fabs(a[0]) + std::fabs(a[1]) + std::fabs(a[2])))
+    return 0.0;
+  return d;
+}
+
 // One angle. Returns 0 or an error code with a message.
 int schedule_angle(const Mesh& m, const Geometry& g, const double* omega, int angle,
-                   Schedule& s, std::vector<int>& planes, std::string& err) {
+                   Schedule& s, std::vector<int>& planes, std::string& err, double tol) {
   const int Z = m.nzones;
   std::vector<int> needZ(Z, 0);
   std::vector<char> exitFace(kMaxFaces * (size_t)Z, 0);
@@ -41,7 +52,7 @@ int schedule_angle(const Mesh& m, const Geometry& g, const double* omega, int an
         const int face = m.CToFace[kMaxCF * cc + cf] - 1;
         const int zoneOpp = m.zoneOpp[kMaxFaces * (size_t)z + face];
         if (zoneOpp > z + 1) {
-          const double afpm = dot3(omega, &g.A_fp[3 * (kMaxCF * cc + cf)]);
+          const double afpm = face_dot(omega, &g.A_fp[3 * (kMaxCF * cc + cf)], tol);
           afpmFace[face] += afpm;
           if (afpm < 0.0) nInc[face] += 1;
           else if (afpm > 0.0) nExit[face] += 1;
@@ -95,7 +106,7 @@ int schedule_angle(const Mesh& m, const Geometry& g, const double* omega, int an
       for (int cf = 0; cf < m.nCFaces[cc]; ++cf) {
         const int cez = m.cEZ[kMaxCF * cc + cf];  // 1-based local
         if (cez > c + 1) {
-          const double aez = dot3(omega, &g.A_ez[3 * (kMaxCF * cc + cf)]);
+          const double aez = face_dot(omega, &g.A_ez[3 * (kMaxCF * cc + cf)], tol);
           if (aez < 0.0) {
             need[c] += 1;
             ds[cez - 1][nDSC[cez - 1]++] = c;
@@ -168,7 +179,7 @@ int schedule_angle(const Mesh& m, const Geometry& g, const double* omega, int an
 }  // namespace
 
 int build_schedule(const Mesh& m, const Geometry& g, const Quadrature& q, Schedule& s,
-                   std::string& err) {
+                   std::string& err, double parallelTol) {
   const int A = q.nangles;
   s = Schedule();
   s.nangles = A;
@@ -184,7 +195,7 @@ int build_schedule(const Mesh& m, const Geometry& g, const Quadrature& q, Schedu
 #pragma omp parallel for schedule(dynamic, 1)
   for (int a = 0; a < A; ++a) {
     std::string e;
-    int r = schedule_angle(m, g, &q.omega[3 * (size_t)a], a, s, planes[a], e);
+    int r = schedule_angle(m, g, &q.omega[3 * (size_t)a], a, s, planes[a], e, parallelTol);
     if (r) {
 #pragma omp critical
       if (!rc) { rc = r; firstErr = e; }

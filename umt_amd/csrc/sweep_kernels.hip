@@ -239,17 +239,18 @@ __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
   }
 }
 
-// phi(g,c) = sum_a w_a psi(g,c,a), accumulated in FP64 in angle order.
+// phi(g,c) = sum_a w_a psi(g,c,a), accumulated in FP64 in angle order
+// (or added to phi when accumulate is set).
 template <typename Real>
 __global__ __launch_bounds__(256) void phi_from_psi(const Real* __restrict__ psi,
                                                     const double* __restrict__ weight, int A,
                                                     long angleStride, long n,
-                                                    double* __restrict__ phi) {
+                                                    double* __restrict__ phi, int accumulate) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
   for (int a = 0; a < A; ++a) acc += weight[a] * (double)psi[(long)a * angleStride + i];
-  phi[i] = acc;
+  phi[i] = accumulate ? phi[i] + acc : acc;
 }
 
 // Source for the next sweep: S = (sigma_s phi + q) / (4 pi), within-group
@@ -365,11 +366,17 @@ hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, h
 }
 
 template <typename Real>
+hipError_t launch_phi_from_psi_acc(const Real* psi, const double* w, int A, long stride, long n,
+                                   double* phi, bool accumulate, hipStream_t s) {
+  hipLaunchKernelGGL((phi_from_psi<Real>), dim3(blocks_for(n, 256)), dim3(256), 0, s, psi, w, A,
+                     stride, n, phi, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <typename Real>
 hipError_t launch_phi_from_psi(const Real* psi, const double* w, int A, long stride, long n,
                                double* phi, hipStream_t s) {
-  hipLaunchKernelGGL((phi_from_psi<Real>), dim3(blocks_for(n, 256)), dim3(256), 0, s, psi, w, A,
-                     stride, n, phi);
-  return hipGetLastError();
+  return launch_phi_from_psi_acc<Real>(psi, w, A, stride, n, phi, false, s);
 }
 
 template <typename Real>
@@ -417,6 +424,10 @@ template hipError_t launch_phi_from_psi<double>(const double*, const double*, in
                                                 double*, hipStream_t);
 template hipError_t launch_phi_from_psi<float>(const float*, const double*, int, long, long, double*,
                                                hipStream_t);
+template hipError_t launch_phi_from_psi_acc<double>(const double*, const double*, int, long, long,
+                                                    double*, bool, hipStream_t);
+template hipError_t launch_phi_from_psi_acc<float>(const float*, const double*, int, long, long,
+                                                   double*, bool, hipStream_t);
 template hipError_t launch_source_update<double>(const double*, const double*, const double*, int,
                                                  long, double*, hipStream_t);
 template hipError_t launch_source_update<float>(const double*, const double*, const double*, int,

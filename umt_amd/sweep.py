@@ -113,6 +113,21 @@ class TiledProblem:
             self._sched = s
         return self._sched
 
+    def gta(self):
+        """Grey-acceleration S2 quadrature and schedule (dict like schedule())."""
+        if getattr(self, "_gta", None) is None:
+            A = 8
+            n = C.c_int(0)
+            om, w = np.zeros((A, 3)), np.zeros(A)
+            nh = np.zeros(A, np.int32)
+            zp = np.zeros(A * self.nzones, np.int32)  # upper bound on total planes
+            nz = np.zeros((A, self.nzones), np.int32)
+            check("umt_problem_gta", _lib.load().umt_problem_gta(
+                self._h, C.byref(n), _dp(om), _dp(w), _ip(nh), _ip(zp), _ip(nz)))
+            self._gta = dict(omega=om, weight=w, nHyperPlanes=nh,
+                             zonesInPlane=zp[:int(nh.sum())].copy(), nextZ=nz)
+        return self._gta
+
     def unknowns(self, groups):
         """Unknowns per sweep: corners x angles x groups (driver/test_driver.cc:1895)."""
         return self.ncorner * self.nangles * groups
@@ -208,6 +223,17 @@ class SweepContext:
         out = np.zeros((p.nangles, p.ncorner + p.nbelem, self.G))
         check("umt_ctx_get_psi", _lib.load().umt_ctx_get_psi(self._h, _dp(out)))
         return out
+
+    def set_acceleration(self, mode):
+        """"none" (plain source iteration) or "gta" (grey-transport acceleration)."""
+        m = {"none": 0, "gta": 1}[mode]
+        check("umt_ctx_set_acceleration", _lib.load().umt_ctx_set_acceleration(self._h, m))
+
+    def gta_stats(self):
+        """(BiCGStab iterations, S2 sweeps) done by GTA so far."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        check("umt_ctx_gta_stats", _lib.load().umt_ctx_gta_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def source_iteration(self, tol=1e-8, max_iters=1000):
         """Returns (converged, iterations, residual, residual_history)."""
