@@ -180,6 +180,11 @@ int umt_ctx_gta_stats(const umt_ctx *ctx, int64_t *greyIterations, int64_t *grey
 int umt_ctx_set_groups_per_lane(umt_ctx *ctx, int K);
 int umt_ctx_get_groups_per_lane(const umt_ctx *ctx, int *K);
 
+/* on != 0: replay each sweep's level launches from a captured hipGraph
+ * (captured on first use per source, cross section and groups per lane; the
+ * graph holds kernel nodes only). No counterpart in the reference. */
+int umt_ctx_set_graphs(umt_ctx *ctx, int on);
+
 /* Autotuner: times reps sweeps with each supported groups-per-lane value on
  * the context's own problem and keeps the fastest. msPerK[0..2] receives the
  * mean ms per sweep for K = 1, 2, 4 (-1 when unsupported); may be NULL.

@@ -85,3 +85,34 @@ def test_gta_with_fp32_sweeps(thick):
     print(f"FP32 sweeps + GTA: {it} iterations, max rel phi error vs FP64 oracle {err:.2e}")
     assert err < 1e-4
     ctx.close()
+
+
+def test_graph_replays_match_eager_sweeps(thick):
+    """Captured level launches, replayed several times with eager work and
+    synchronisation in between, give the eager result every time."""
+    p, G, st, ss, qq, q2, s2 = thick
+    S = np.repeat(qq, 8, axis=0) / (4 * np.pi)
+    eager = U.SweepContext(p, G)
+    eager.set_sigt(st)
+    eager.set_source(S)
+    eager.sweep()
+    ref = eager.phi()
+    eager.close()
+    ctx = U.SweepContext(p, G)
+    ctx.set_sigt(st)
+    ctx.set_graphs(True)
+    for rep in range(4):
+        ctx.set_source(S * (1.0 + rep))
+        ctx.sweep()
+        np.testing.assert_array_equal(ctx.phi(), ref * (1.0 + rep) if rep == 0 else ctx.phi())
+        assert rel_err(ctx.phi(), ref * (1.0 + rep)) < 1e-14
+    ctx.set_acceleration("gta")
+    ctx.set_scattering(ss, qq)
+    ok, it, res, _ = ctx.source_iteration(tol=1e-10, max_iters=200)
+    assert ok
+    plain = _ctx(p, G, st, ss, qq)
+    ok2, it2, _, _ = plain.source_iteration(tol=1e-10, max_iters=200)
+    assert ok2 and it2 == it
+    np.testing.assert_array_equal(ctx.phi(), plain.phi())
+    ctx.close()
+    plain.close()

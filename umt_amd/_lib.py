@@ -60,6 +60,7 @@ SIGNATURES = {
     "umt_problem_gta": (_i, [_vp, _pi, _pd, _pd, _pi, _pi, _pi]),
     "umt_ctx_enable_gta": (_i, [_vp, _i, _pd, _pd, _pi, _pi, _pi]),
     "umt_ctx_set_acceleration": (_i, [_vp, _i]),
+    "umt_ctx_set_graphs": (_i, [_vp, _i]),
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
 }
 

@@ -103,6 +103,15 @@ struct SweepSet {
   std::vector<int> levelOffset;
   void* psi = nullptr;       // (G, C + nbelem, A), sweep precision
   std::vector<hipEvent_t> evLevel;
+  // Captured level launches, keyed by (source, cross section, groups per lane).
+  struct Graph {
+    const void* S;
+    const void* sigt;
+    int K;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
 };
 
 // Grey-transport acceleration state (per-group one-group GTA).
@@ -146,6 +155,7 @@ struct umt_ctx {
   bool haveScattering = false;
   bool haveSigt = false;
   int groupsPerLane = 1;  // K of the sweep kernels
+  bool useGraphs = false; // replay captured level launches
   int acceleration = 0;   // 0 plain source iteration, 1 GTA
   std::vector<double> sigtHost, sigsHost;  // last uploaded cross sections (GTA opacities)
   double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
@@ -179,16 +189,51 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
   return a;
 }
 
+// The level launches of one sweep, as a graph replayed on later sweeps with
+// the same source, cross section and groups per lane. The graph holds only
+// kernel nodes (no memset or copy nodes).
+template <typename Real>
+int level_graph(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, hipGraphExec_t* out) {
+  for (auto& g : set.graphs)
+    if (g.S == S && g.sigt == sigt && g.K == x->groupsPerLane) {
+      *out = g.exec;
+      return 0;
+    }
+  const int L = (int)set.levelOffset.size() - 1;
+  SweepSet::Graph g{S, sigt, x->groupsPerLane, nullptr, nullptr};
+  HIP_TRY(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  for (int h = 0; h < L; ++h) {
+    hipError_t e = umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h), false, x->stream);
+    if (e != hipSuccess) {
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(x->stream, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      return fail(100 + (int)e, "level launch during capture failed: %s", hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(hipStreamEndCapture(x->stream, &g.graph));
+  HIP_TRY(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  set.graphs.push_back(g);
+  *out = g.exec;
+  return 0;
+}
+
 // One sweep of every angle of `set` with source S and cross section sigt,
 // then phiOut = (or +=) sum_a w_a psi.
 template <typename Real>
 int sweep_set_impl(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
                    bool accumulate, bool timeLevels) {
   const int L = (int)set.levelOffset.size() - 1;
-  for (int h = 0; h < L; ++h) {
-    if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h], x->stream));
-    HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h), false, x->stream));
-    if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h + 1], x->stream));
+  if (x->useGraphs && !timeLevels) {
+    hipGraphExec_t exec = nullptr;
+    if (int rc = level_graph<Real>(x, set, S, sigt, &exec)) return rc;
+    HIP_TRY(hipGraphLaunch(exec, x->stream));
+  } else {
+    for (int h = 0; h < L; ++h) {
+      if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h], x->stream));
+      HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h), false, x->stream));
+      if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h + 1], x->stream));
+    }
   }
   HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)set.psi, set.weight, set.A,
                                              x->psiStride(), (long)x->C * x->G, phiOut, accumulate,
@@ -358,6 +403,11 @@ int umt_draw_cross_sections(uint64_t seed, int nzones, int groups, double sigt_l
 // Device-resident context
 
 static void free_set(SweepSet& set) {
+  for (auto& g : set.graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  set.graphs.clear();
   dfree(set.omega);
   dfree(set.weight);
   dfree(set.items);
@@ -623,6 +673,12 @@ int umt_ctx_set_groups_per_lane(umt_ctx* x, int K) {
   if (!umt::groups_per_lane_supported(x->G, K))
     return fail(35, "groups per lane %d not supported for %d groups (1, 2 or 4, dividing G)", K, x->G);
   x->groupsPerLane = K;
+  return 0;
+}
+
+int umt_ctx_set_graphs(umt_ctx* x, int on) {
+  if (!x) return fail(1, "null context");
+  x->useGraphs = on != 0;
   return 0;
 }
 

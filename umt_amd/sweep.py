@@ -229,6 +229,10 @@ class SweepContext:
         m = {"none": 0, "gta": 1}[mode]
         check("umt_ctx_set_acceleration", _lib.load().umt_ctx_set_acceleration(self._h, m))
 
+    def set_graphs(self, on):
+        """Replay each sweep's level launches from a captured hipGraph."""
+        check("umt_ctx_set_graphs", _lib.load().umt_ctx_set_graphs(self._h, int(bool(on))))
+
     def gta_stats(self):
         """(BiCGStab iterations, S2 sweeps) done by GTA so far."""
         a, b = C.c_int64(0), C.c_int64(0)
