@@ -170,7 +170,8 @@ int umt_ctx_time_sweeps(umt_ctx *ctx, int warmup, int reps, double *ms_sweep,
  * mini-app build compiles out. */
 int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const double *weight,
                        const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ);
-/* mode 0: plain source iteration (default); 1: GTA-accelerated. */
+/* mode 0: plain source iteration (default); 1: GTA-accelerated; 2: learned
+ * correction (umt_ctx_set_learner). */
 int umt_ctx_set_acceleration(umt_ctx *ctx, int mode);
 /* Cumulative BiCGStab iterations and S2 sweeps done by GTA on this context. */
 int umt_ctx_gta_stats(const umt_ctx *ctx, int64_t *greyIterations, int64_t *greySweeps);
@@ -206,6 +207,76 @@ int umt_hbm_copy_probe(int64_t n, int reps, double *ms);
 
 /* dims[0..6] = nzones, ncorner, nbelem, groups, nangles, levels, items. */
 int umt_ctx_dims(const umt_ctx *ctx, int64_t *dims);
+
+/* ------------------------------------------------------------------------
+ * Learned correction of source iteration. No counterpart in the reference:
+ * it accelerates the iteration rt/rtmainsn.F90:167-235 drives, next to GTA.
+ *
+ * A per-(zone, group) network (10 features in 16 slots -> 64 -> 64 -> 1,
+ * tanh; BF16 MFMA with FP32 accumulation, FP32 Adam masters) predicts r, and
+ * every `every` sweeps the state becomes phi_half + r (phi_half - phi_prev)
+ * per corner. Plain source iteration stays the judge: the residual is always
+ * that of a plain sweep, a fixed point of plain iteration is left unchanged
+ * (the correction scales the sweep's change), and with guard != 0 the
+ * corrections stop for the rest of the solve as soon as a corrected state's
+ * sweep changes phi more than the sweep the correction followed.
+ *
+ * Training data come from the method's own iterates: with collection on,
+ * source iteration records each state it could correct (features, zone
+ * averages, change); umt_ctx_learn_commit turns them into samples whose
+ * target is the correction to the context's current (converged) zone
+ * averages, (phibar* - phibar_half) / phibar_half, both that and the change
+ * scaled by the state's rms change.
+ * ---------------------------------------------------------------------- */
+typedef struct umt_learner umt_learner;
+
+/* Parameter count: W1 (16 x 64, feature-major), b1 (64), W2 (64 x 64, in-major),
+ * b2 (64), w3 (64), b3. */
+int umt_learner_num_params(void);
+/* New network with Uniform(+-1/sqrt(fan_in)) weights drawn from seed. */
+int umt_learner_create(uint64_t seed, umt_learner **out);
+void umt_learner_destroy(umt_learner *net);
+int umt_learner_get_params(const umt_learner *net, float *params);
+/* Sets the FP32 parameters and resets the Adam moments. */
+int umt_learner_set_params(umt_learner *net, const float *params);
+/* r for n samples of 16 FP32 feature slots each. */
+int umt_learner_forward(umt_learner *net, const float *X, int64_t n, float *r);
+/* Gradient of (1/n) sum (r a - b)^2 over n samples, and sum (r a - b)^2. */
+int umt_learner_gradient(umt_learner *net, const float *X, const float *a, const float *b,
+                         int64_t n, float *grad, double *sse);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) on the given batch. */
+int umt_learner_step(umt_learner *net, const float *X, const float *a, const float *b, int64_t n,
+                     double lr);
+int umt_learner_add_samples(umt_learner *net, const float *X, const float *a, const float *b,
+                            int64_t n);
+int umt_learner_num_samples(const umt_learner *net, int64_t *n);
+int umt_learner_get_samples(const umt_learner *net, float *X, float *a, float *b);
+int umt_learner_clear_samples(umt_learner *net);
+/* Adam over the held samples: epochs passes in seeded random order,
+ * mini-batches of `batch`; mean_loss = the last epoch's mean (r a - b)^2. */
+int umt_learner_train(umt_learner *net, int epochs, int batch, double lr, uint64_t seed,
+                      double *mean_loss);
+
+/* Zone face neighbours, zoneOpp(6, nzones) 1-based with <= 0 on the boundary
+ * (mods/Geometry_mod.F90 zoneOpp); umt_ctx_create_from_problem sets them. */
+int umt_ctx_set_zone_neighbors(umt_ctx *ctx, const int *zoneOpp);
+/* Network used by acceleration mode 2 (NULL detaches), corrections every
+ * `every` sweeps, guard as described above. */
+int umt_ctx_set_learner(umt_ctx *ctx, umt_learner *net, int every, int guard);
+/* on != 0: source iteration in modes 0 and 2 records its states. */
+int umt_ctx_learn_collect(umt_ctx *ctx, int on);
+/* Appends the recorded states whose residual is >= min_residual to net's
+ * samples, targets against the current zone averages; drops the records. */
+int umt_ctx_learn_commit(umt_ctx *ctx, umt_learner *net, double min_residual, int *states);
+int umt_ctx_learn_discard(umt_ctx *ctx);
+/* The 16 feature slots per (zone, group) for zone averages ph (after a
+ * sweep), pp (before it) and dp (the previous sweep's change; NULL: none),
+ * all (groups, nzones). X: (16, groups, nzones) FP32. */
+int umt_ctx_learn_features(umt_ctx *ctx, const double *ph, const double *pp, const double *dp,
+                           float *X);
+/* Corrections applied on this context so far, and the iteration at which
+ * the last solve stopped correcting (0: it did not). */
+int umt_ctx_learn_stats(const umt_ctx *ctx, int64_t *corrections, int *fell_back_at);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,7 @@ _d = C.c_double
 _pi = C.POINTER(C.c_int)
 _pi64 = C.POINTER(C.c_int64)
 _pd = C.POINTER(C.c_double)
+_pf = C.POINTER(C.c_float)
 _vp = C.c_void_p
 
 # name -> (restype, argtypes)
@@ -62,6 +63,26 @@ SIGNATURES = {
     "umt_ctx_set_acceleration": (_i, [_vp, _i]),
     "umt_ctx_set_graphs": (_i, [_vp, _i]),
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
+    "umt_learner_num_params": (_i, []),
+    "umt_learner_create": (_i, [_u64, C.POINTER(_vp)]),
+    "umt_learner_destroy": (None, [_vp]),
+    "umt_learner_get_params": (_i, [_vp, _pf]),
+    "umt_learner_set_params": (_i, [_vp, _pf]),
+    "umt_learner_forward": (_i, [_vp, _pf, _i64, _pf]),
+    "umt_learner_gradient": (_i, [_vp, _pf, _pf, _pf, _i64, _pf, _pd]),
+    "umt_learner_step": (_i, [_vp, _pf, _pf, _pf, _i64, _d]),
+    "umt_learner_add_samples": (_i, [_vp, _pf, _pf, _pf, _i64]),
+    "umt_learner_num_samples": (_i, [_vp, _pi64]),
+    "umt_learner_get_samples": (_i, [_vp, _pf, _pf, _pf]),
+    "umt_learner_clear_samples": (_i, [_vp]),
+    "umt_learner_train": (_i, [_vp, _i, _i, _d, _u64, _pd]),
+    "umt_ctx_set_zone_neighbors": (_i, [_vp, _pi]),
+    "umt_ctx_set_learner": (_i, [_vp, _vp, _i, _i]),
+    "umt_ctx_learn_collect": (_i, [_vp, _i]),
+    "umt_ctx_learn_commit": (_i, [_vp, _vp, _d, _pi]),
+    "umt_ctx_learn_discard": (_i, [_vp]),
+    "umt_ctx_learn_features": (_i, [_vp, _pd, _pd, _pd, _pf]),
+    "umt_ctx_learn_stats": (_i, [_vp, _pi64, _pi]),
 }
 
 _lib = None

@@ -39,7 +39,7 @@ hipError_t launch_source_update(const double* phi, const double* sigs, const dou
                                 long n, Real* S, hipStream_t s);
 hipError_t launch_residual(const double* phi, const double* vol, const double* volZ, int G, long nzg,
                            double* phibar, double* phibarOld, double cutoff,
-                           unsigned long long* bits, hipStream_t s);
+                           unsigned long long* bits, hipStream_t s, bool magnitude = false);
 template <typename T>
 hipError_t launch_fill_zero(T* p, long n, hipStream_t s);
 hipError_t launch_copy_probe(const double* src, double* dst, long n, hipStream_t s);
@@ -65,5 +65,50 @@ hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const d
                                  double* partial, double* out, hipStream_t s);
 template <typename Dst, typename Src>
 hipError_t launch_convert(const Src* s, Dst* d, long n, hipStream_t s_);
+// Zone averages phibar(g,z) = sum_c V_c phi(g,c) / V_z (maxBits: scratch running max).
+hipError_t launch_zone_average(const double* phi, const double* vol, const double* volZ, int G,
+                               long nzg, double* phibar, unsigned long long* maxBits,
+                               hipStream_t s);
+
+// Learned correction network (learn_kernels.hip): 16 feature slots -> 64 ->
+// 64 -> 1 with tanh. FP32 parameter layout: W1 [feature][hidden], b1,
+// W2 [in][out], b2, w3, b3.
+constexpr int kMlpFeatures = 16;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpUsedFeatures = 10;
+constexpr int kMlpW1 = 0;
+constexpr int kMlpB1 = kMlpW1 + kMlpFeatures * kMlpHidden;
+constexpr int kMlpW2 = kMlpB1 + kMlpHidden;
+constexpr int kMlpB2 = kMlpW2 + kMlpHidden * kMlpHidden;
+constexpr int kMlpW3 = kMlpB2 + kMlpHidden;
+constexpr int kMlpB3 = kMlpW3 + kMlpHidden;
+constexpr int kMlpParams = kMlpB3 + 1;
+constexpr int kMlpPacked = 20 * 64 * 8;  // BF16 operand slots (mlp_pack)
+constexpr double kLearnTiny = 1e-300;
+constexpr int kLearnTargetBlocks = 256;
+
+hipError_t launch_mlp_pack(const float* params, unsigned short* packed, hipStream_t s);
+hipError_t launch_mlp_forward(const float* X, long n, const unsigned short* packed,
+                              const float* params, float* r, hipStream_t s);
+// Waves (a multiple of 4) mlp_train uses for count samples; partial needs
+// that many rows of kMlpParams + 1 floats.
+int mlp_train_waves(long count, int stepsPerWave);
+hipError_t launch_mlp_train(const float* X, const float* a, const float* b, const int* idx,
+                            long start, long count, int stepsPerWave,
+                            const unsigned short* packed, const float* params, float* partial,
+                            hipStream_t s);
+hipError_t launch_mlp_reduce_adam(const float* partial, int nWaves, float* grad, float* params,
+                                  float* m, float* v, float lr, float beta1, float beta2, float eps,
+                                  float bc1, float bc2, hipStream_t s);
+hipError_t launch_learn_features(const double* ph, const double* pp, const double* dp,
+                                 const double* sigt, const double* sigs, const double* q,
+                                 const int* zoneNbr, int G, long n, double* rel, double* ratio,
+                                 unsigned long long* maxRel, float* X, hipStream_t s);
+hipError_t launch_learn_apply(double* phi, const double* phiPrev, const float* r, int G, long n,
+                              hipStream_t s);
+hipError_t launch_learn_delta(const double* ph, const double* pp, long n, double* d, hipStream_t s);
+hipError_t launch_learn_targets(const double* ph, const double* d, const double* pstar, long n,
+                                double* a, double* b, double* partial, float* aOut, float* bOut,
+                                hipStream_t s);
 
 }  // namespace umt

@@ -130,6 +130,54 @@ struct GtaState {
   long long greyIterations = 0, greySweeps = 0;
 };
 
+// The learned correction network: FP32 parameters with Adam moments, their
+// packed BF16 MFMA operands, and the training samples (features X, scaled
+// sweep change a, scaled target correction b).
+struct umt_learner {
+  hipStream_t stream = nullptr;
+  float* params = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  float* grad = nullptr;  // kMlpParams gradient entries + the squared-error sum
+  unsigned short* packed = nullptr;
+  long long adamSteps = 0;
+  float* X = nullptr;
+  float* a = nullptr;
+  float* b = nullptr;
+  long n = 0, cap = 0;
+  int* perm = nullptr;
+  long permCap = 0;
+  float* partial = nullptr;
+  long partialCap = 0;  // rows of kMlpParams + 1
+  float* lossLog = nullptr;
+  long lossCap = 0;
+};
+
+// Learned acceleration state of a context.
+struct LearnState {
+  umt_learner* net = nullptr;
+  int every = 2;       // sweeps between corrections
+  bool guard = true;   // stop correcting once a corrected step raises the residual
+  bool collect = false;
+  int* zoneNbr = nullptr;   // (6, Z) 0-based face neighbours, -1 on the boundary
+  double* sigt64 = nullptr; // (G, Z) FP64 sigma_t for the features
+  double *pp = nullptr, *d = nullptr, *dp = nullptr, *rel = nullptr, *ratio = nullptr;
+  double *ta = nullptr, *tb = nullptr, *tpart = nullptr;
+  unsigned long long* maxRel = nullptr;
+  unsigned long long* scratchBits = nullptr;
+  float* X = nullptr;
+  float* r = nullptr;
+  struct Pending {
+    float* X;
+    double* ph;
+    double* d;
+    double res;
+  };
+  std::vector<Pending> pending;
+  long long corrections = 0;
+  int fellBack = 0;
+};
+
 struct umt_ctx {
   int Z = 0, C = 0, nb = 0, G = 0, precision = 64;
   hipStream_t stream = nullptr;
@@ -156,10 +204,11 @@ struct umt_ctx {
   bool haveSigt = false;
   int groupsPerLane = 1;  // K of the sweep kernels
   bool useGraphs = false; // replay captured level launches
-  int acceleration = 0;   // 0 plain source iteration, 1 GTA
+  int acceleration = 0;   // 0 plain source iteration, 1 GTA, 2 learned correction
   std::vector<double> sigtHost, sigsHost;  // last uploaded cross sections (GTA opacities)
   double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
   GtaState gta;
+  LearnState learn;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   size_t rsize() const { return precision == 32 ? sizeof(float) : sizeof(double); }
@@ -423,9 +472,26 @@ static void free_gta(GtaState& g) {
   g.enabled = false;
 }
 
+static void free_pending(LearnState& l) {
+  for (auto& p : l.pending) {
+    dfree(p.X);
+    dfree(p.ph);
+    dfree(p.d);
+  }
+  l.pending.clear();
+}
+
+static void free_learn(LearnState& l) {
+  free_pending(l);
+  dfree(l.zoneNbr); dfree(l.sigt64); dfree(l.pp); dfree(l.d); dfree(l.dp); dfree(l.rel);
+  dfree(l.ratio); dfree(l.ta); dfree(l.tb); dfree(l.tpart); dfree(l.maxRel); dfree(l.scratchBits);
+  dfree(l.X); dfree(l.r);
+}
+
 void umt_ctx_destroy(umt_ctx* x) {
   if (!x) return;
   if (x->stream) (void)hipStreamSynchronize(x->stream);
+  free_learn(x->learn);
   dfree(x->Afp); dfree(x->Aez); dfree(x->vol); dfree(x->cFP); dfree(x->vol64); dfree(x->volZ);
   dfree(x->S); dfree(x->sigt); dfree(x->sigs); dfree(x->q); dfree(x->phi); dfree(x->phiPrev);
   dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
@@ -600,6 +666,7 @@ int umt_ctx_create_from_problem(const umt_problem* p, int groups, int precision,
   if (rc) return rc;
   rc = umt_ctx_enable_gta(*out, p->q2.nangles, p->q2.omega.data(), p->q2.weight.data(),
                           p->s2.nHyperPlanes.data(), p->s2.zonesInPlane.data(), p->s2.nextZ.data());
+  if (!rc) rc = umt_ctx_set_zone_neighbors(*out, p->m.zoneOpp.data());
   if (rc) {
     std::string keep = g_err;
     umt_ctx_destroy(*out);
@@ -814,8 +881,10 @@ int umt_ctx_enable_gta(umt_ctx* x, int nangles, const double* omega, const doubl
 
 int umt_ctx_set_acceleration(umt_ctx* x, int mode) {
   if (!x) return fail(1, "null context");
-  if (mode != 0 && mode != 1) return fail(36, "acceleration must be 0 (none) or 1 (GTA)");
+  if (mode < 0 || mode > 2) return fail(36, "acceleration must be 0 (none), 1 (GTA) or 2 (learned)");
   if (mode == 1 && !x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
+  if (mode == 2 && !x->learn.net) return fail(38, "learned acceleration needs umt_ctx_set_learner first");
+  if (mode == 2 && !x->learn.zoneNbr) return fail(39, "learned acceleration needs umt_ctx_set_zone_neighbors first");
   x->acceleration = mode;
   return 0;
 }
@@ -1031,6 +1100,42 @@ int gta_solve(umt_ctx* x, const double* phiHalf, double epsGrey) {
   return 0;
 }
 
+int learn_prepare(umt_ctx* x) {
+  LearnState& L = x->learn;
+  if (!L.zoneNbr) return fail(39, "learned acceleration needs umt_ctx_set_zone_neighbors first");
+  if (!x->haveSigt) return fail(40, "learned acceleration needs umt_ctx_set_sigt first");
+  const size_t GZ = (size_t)x->G * x->Z;
+  int rc = 0;
+  if (!L.pp) {
+    if ((rc = dalloc(&L.sigt64, GZ)) || (rc = dalloc(&L.pp, GZ)) || (rc = dalloc(&L.dp, GZ)) ||
+        (rc = dalloc(&L.rel, GZ)) || (rc = dalloc(&L.ratio, GZ)) || (rc = dalloc(&L.ta, GZ)) ||
+        (rc = dalloc(&L.tb, GZ)) || (rc = dalloc(&L.tpart, (size_t)umt::kLearnTargetBlocks)) ||
+        (rc = dalloc(&L.maxRel, (size_t)x->G)) || (rc = dalloc(&L.scratchBits, 2)) ||
+        (rc = dalloc(&L.X, GZ * umt::kMlpFeatures)) || (rc = dalloc(&L.r, GZ)))
+      return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(L.sigt64, x->sigtHost.data(), GZ * sizeof(double), hipMemcpyHostToDevice,
+                         x->stream));
+  return 0;
+}
+
+// Keeps one plain state (features, zone averages, change) for training.
+int learn_record(umt_ctx* x, double res) {
+  LearnState& L = x->learn;
+  const size_t GZ = (size_t)x->G * x->Z;
+  LearnState::Pending p{nullptr, nullptr, nullptr, res};
+  int rc;
+  if ((rc = dalloc(&p.X, GZ * umt::kMlpFeatures)) || (rc = dalloc(&p.ph, GZ)) || (rc = dalloc(&p.d, GZ))) {
+    dfree(p.X); dfree(p.ph); dfree(p.d);
+    return rc;
+  }
+  L.pending.push_back(p);
+  HIP_TRY(hipMemcpyAsync(p.X, L.X, GZ * umt::kMlpFeatures * sizeof(float), hipMemcpyDeviceToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(p.ph, x->phibar, GZ * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+  HIP_TRY(umt::launch_learn_delta(x->phibar, L.pp, (long)GZ, p.d, x->stream));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1046,12 +1151,24 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
   HIP_TRY(umt::launch_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld, 1e-6,
                                x->bits, x->stream));
   const bool gta = x->acceleration == 1;
+  const bool learned = x->acceleration == 2;
+  // Learned-correction bookkeeping (corrections, or recording plain states).
+  const bool track = learned || (x->learn.collect && !gta);
   if (gta)
     if (int rc = gta_prepare(x)) return rc;
+  if (track)
+    if (int rc = learn_prepare(x)) return rc;
+  LearnState& L = x->learn;
+  bool correcting = learned, haveDp = false;
+  int since = 0;
+  double lastCorrectedRes = -1.0;
+  L.fellBack = 0;
   double epsGrey = x->gta.epsGreyInit;
   while (it < maxIters) {
-    if (gta)
+    if (gta || track)
       HIP_TRY(hipMemcpyAsync(x->phiPrev, x->phi, nGC * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+    if (track)
+      HIP_TRY(hipMemcpyAsync(L.pp, x->phibarOld, nGZ * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
     if (x->precision == 32)
       HIP_TRY(umt::launch_source_update<float>(x->phi, x->sigs, x->q, x->G, nGC, (float*)x->S, x->stream));
     else
@@ -1066,8 +1183,13 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
       for (int k = 0; k < x->G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = 1.0; cf[3 * k + 2] = 0.0; }
       if (int rc = lincomb(x, x->phi, x->gta.F, nullptr, x->phi, cf)) return rc;
     }
+    // Corrected iterates may go negative; their residual tests |phibar|.
     HIP_TRY(umt::launch_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld, 1e-6,
-                                 x->bits, x->stream));
+                                 x->bits, x->stream, learned));
+    if (track)  // d = phibar - pp, the network's features (X) and the previous change dp
+      HIP_TRY(umt::launch_learn_features(x->phibar, L.pp, haveDp ? L.dp : nullptr, L.sigt64, x->sigs,
+                                         x->q, L.zoneNbr, x->G, nGZ, L.rel, L.ratio, L.maxRel, L.X,
+                                         x->stream));
     unsigned long long b[2];
     HIP_TRY(hipMemcpyAsync(b, x->bits, sizeof b, hipMemcpyDeviceToHost, x->stream));
     HIP_TRY(hipStreamSynchronize(x->stream));
@@ -1079,6 +1201,35 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
     ++it;
     epsGrey = res / 20.0;  // rt/rtmainsn.F90:220
     if (res < tol) break;
+    if (track) {
+      // Plain source iteration judges every correction: once a corrected
+      // state's sweep changes phi more than the sweep the correction
+      // followed, corrections stop and the iteration continues plain.
+      if (lastCorrectedRes >= 0.0) {
+        if (L.guard && res > lastCorrectedRes && correcting) {
+          correcting = false;
+          L.fellBack = it;
+        }
+        lastCorrectedRes = -1.0;
+      }
+      ++since;
+      if (since >= L.every && haveDp) {
+        if (L.collect)
+          if (int rc = learn_record(x, res)) return rc;
+        if (correcting) {
+          HIP_TRY(umt::launch_mlp_forward(L.X, nGZ, L.net->packed, L.net->params, L.r, x->stream));
+          HIP_TRY(umt::launch_learn_apply(x->phi, x->phiPrev, L.r, x->G, nGC, x->stream));
+          // The corrected state's zone averages are the "old" values of the next residual.
+          HIP_TRY(umt::launch_zone_average(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibarOld,
+                                           L.scratchBits, x->stream));
+          since = 0;
+          lastCorrectedRes = res;
+          ++L.corrections;
+        }
+      }
+      HIP_TRY(umt::launch_learn_delta(x->phibar, L.pp, nGZ, L.dp, x->stream));
+      haveDp = true;
+    }
   }
   if (iters) *iters = it;
   if (residual) *residual = res;
@@ -1110,6 +1261,370 @@ int umt_ctx_time_sweeps(umt_ctx* x, int warmup, int reps, double* msSweep, doubl
   if (msSweep) *msSweep = total / reps;
   if (msLevels) *msLevels = sumLevels / reps;
   if (launches) *launches = L;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Learned correction network
+
+}  // extern "C"
+
+namespace {
+
+constexpr float kAdamBeta1 = 0.9f, kAdamBeta2 = 0.999f, kAdamEps = 1e-8f;
+constexpr int kTrainStepsPerWave = 4;
+
+// splitmix64 stream of uniform doubles in [0, 1).
+double next_unit(uint64_t& state) {
+  state += 0x9E3779B97F4A7C15ull;
+  uint64_t z = state;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+int learner_upload_params(umt_learner* L, const float* host) {
+  HIP_TRY(hipMemcpyAsync(L->params, host, umt::kMlpParams * sizeof(float), hipMemcpyHostToDevice, L->stream));
+  HIP_TRY(umt::launch_fill_zero<float>(L->m, umt::kMlpParams, L->stream));
+  HIP_TRY(umt::launch_fill_zero<float>(L->v, umt::kMlpParams, L->stream));
+  HIP_TRY(umt::launch_mlp_pack(L->params, L->packed, L->stream));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  L->adamSteps = 0;
+  return 0;
+}
+
+// Room for `need` training samples, keeping the ones held.
+int learner_reserve(umt_learner* L, long need) {
+  if (need <= L->cap) return 0;
+  long cap = std::max(need, 2 * L->cap);
+  float *X = nullptr, *a = nullptr, *b = nullptr;
+  int rc;
+  if ((rc = dalloc(&X, (size_t)cap * umt::kMlpFeatures)) || (rc = dalloc(&a, (size_t)cap)) ||
+      (rc = dalloc(&b, (size_t)cap))) {
+    dfree(X); dfree(a); dfree(b);
+    return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  if (L->n) {
+    HIP_TRY(hipMemcpy(X, L->X, (size_t)L->n * umt::kMlpFeatures * sizeof(float), hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(a, L->a, (size_t)L->n * sizeof(float), hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(b, L->b, (size_t)L->n * sizeof(float), hipMemcpyDeviceToDevice));
+  }
+  dfree(L->X); dfree(L->a); dfree(L->b);
+  L->X = X; L->a = a; L->b = b; L->cap = cap;
+  return 0;
+}
+
+int learner_partial(umt_learner* L, long rows) {
+  if (rows <= L->partialCap) return 0;
+  HIP_TRY(hipStreamSynchronize(L->stream));  // earlier steps may still read it
+  dfree(L->partial);
+  if (int rc = dalloc(&L->partial, (size_t)rows * (umt::kMlpParams + 1))) return rc;
+  L->partialCap = rows;
+  return 0;
+}
+
+// Gradient (and, with lr > 0, one Adam step) over samples [start, start +
+// count) of (X, a, b), in idx order when idx is given.
+int learner_step(umt_learner* L, const float* X, const float* a, const float* b, const int* idx,
+                 long start, long count, float lr) {
+  const int waves = umt::mlp_train_waves(count, kTrainStepsPerWave);
+  if (int rc = learner_partial(L, waves)) return rc;
+  HIP_TRY(umt::launch_mlp_train(X, a, b, idx, start, count, kTrainStepsPerWave, L->packed, L->params,
+                                L->partial, L->stream));
+  float bc1 = 1.f, bc2 = 1.f;
+  if (lr > 0.f) {
+    ++L->adamSteps;
+    bc1 = (float)(1.0 - std::pow((double)kAdamBeta1, (double)L->adamSteps));
+    bc2 = (float)(1.0 - std::pow((double)kAdamBeta2, (double)L->adamSteps));
+  }
+  HIP_TRY(umt::launch_mlp_reduce_adam(L->partial, waves, L->grad, L->params, L->m, L->v, lr,
+                                      kAdamBeta1, kAdamBeta2, kAdamEps, bc1, bc2, L->stream));
+  if (lr > 0.f) HIP_TRY(umt::launch_mlp_pack(L->params, L->packed, L->stream));
+  return 0;
+}
+
+// Host arrays of n samples uploaded to temporary device buffers.
+struct SampleUpload {
+  float *X = nullptr, *a = nullptr, *b = nullptr;
+  ~SampleUpload() { dfree(X); dfree(a); dfree(b); }
+  int load(hipStream_t s, const float* hX, const float* ha, const float* hb, long n) {
+    int rc;
+    if ((rc = dalloc(&X, (size_t)n * umt::kMlpFeatures))) return rc;
+    HIP_TRY(hipMemcpyAsync(X, hX, (size_t)n * umt::kMlpFeatures * sizeof(float), hipMemcpyHostToDevice, s));
+    if (ha) {
+      if ((rc = dalloc(&a, (size_t)n)) || (rc = dalloc(&b, (size_t)n))) return rc;
+      HIP_TRY(hipMemcpyAsync(a, ha, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(b, hb, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    return 0;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int umt_learner_num_params(void) { return umt::kMlpParams; }
+
+void umt_learner_destroy(umt_learner* L) {
+  if (!L) return;
+  if (L->stream) (void)hipStreamSynchronize(L->stream);
+  dfree(L->params); dfree(L->m); dfree(L->v); dfree(L->grad); dfree(L->packed);
+  dfree(L->X); dfree(L->a); dfree(L->b); dfree(L->perm); dfree(L->partial); dfree(L->lossLog);
+  if (L->stream) (void)hipStreamDestroy(L->stream);
+  delete L;
+}
+
+int umt_learner_create(uint64_t seed, umt_learner** out) {
+  if (!out) return fail(1, "null output pointer");
+  *out = nullptr;
+  umt_learner* L = new umt_learner();
+  auto bail = [&](int rc) {
+    std::string keep = g_err;
+    umt_learner_destroy(L);
+    g_err = keep;
+    return rc;
+  };
+  int rc;
+  if (hipStreamCreate(&L->stream) != hipSuccess) return bail(fail(2, "hipStreamCreate failed"));
+  if ((rc = dalloc(&L->params, umt::kMlpParams)) || (rc = dalloc(&L->m, umt::kMlpParams)) ||
+      (rc = dalloc(&L->v, umt::kMlpParams)) || (rc = dalloc(&L->grad, umt::kMlpParams + 1)) ||
+      (rc = dalloc(&L->packed, umt::kMlpPacked)))
+    return bail(rc);
+  // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights and biases; the
+  // feature slots beyond the used ones get zero weights.
+  std::vector<float> p(umt::kMlpParams, 0.f);
+  uint64_t st = seed;
+  auto draw = [&](double bound) { return (float)((2.0 * next_unit(st) - 1.0) * bound); };
+  const double b1 = 1.0 / std::sqrt((double)umt::kMlpUsedFeatures), b2 = 1.0 / std::sqrt((double)umt::kMlpHidden);
+  for (int f = 0; f < umt::kMlpFeatures; ++f)
+    for (int h = 0; h < umt::kMlpHidden; ++h)
+      p[umt::kMlpW1 + f * umt::kMlpHidden + h] = f < umt::kMlpUsedFeatures ? draw(b1) : 0.f;
+  for (int h = 0; h < umt::kMlpHidden; ++h) p[umt::kMlpB1 + h] = draw(b1);
+  for (int i = 0; i < umt::kMlpHidden * umt::kMlpHidden; ++i) p[umt::kMlpW2 + i] = draw(b2);
+  for (int h = 0; h < umt::kMlpHidden; ++h) p[umt::kMlpB2 + h] = draw(b2);
+  for (int h = 0; h < umt::kMlpHidden; ++h) p[umt::kMlpW3 + h] = draw(b2);
+  p[umt::kMlpB3] = draw(b2);
+  if ((rc = learner_upload_params(L, p.data()))) return bail(rc);
+  *out = L;
+  return 0;
+}
+
+int umt_learner_get_params(const umt_learner* L, float* params) {
+  if (!L || !params) return fail(1, "null argument");
+  HIP_TRY(hipMemcpyAsync(params, L->params, umt::kMlpParams * sizeof(float), hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int umt_learner_set_params(umt_learner* L, const float* params) {
+  if (!L || !params) return fail(1, "null argument");
+  for (int i = 0; i < umt::kMlpParams; ++i)
+    if (!std::isfinite(params[i])) return fail(41, "parameter %d is not finite", i);
+  return learner_upload_params(L, params);
+}
+
+int umt_learner_forward(umt_learner* L, const float* X, int64_t n, float* r) {
+  if (!L || (n > 0 && (!X || !r)) || n < 0) return fail(1, "bad argument");
+  if (n == 0) return 0;
+  SampleUpload u;
+  if (int rc = u.load(L->stream, X, nullptr, nullptr, n)) return rc;
+  float* dr = nullptr;
+  if (int rc = dalloc(&dr, (size_t)n)) return rc;
+  hipError_t e = umt::launch_mlp_forward(u.X, n, L->packed, L->params, dr, L->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(r, dr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, L->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  dfree(dr);
+  HIP_TRY(e);
+  return 0;
+}
+
+int umt_learner_gradient(umt_learner* L, const float* X, const float* a, const float* b, int64_t n,
+                         float* grad, double* sse) {
+  if (!L || !X || !a || !b || n < 1) return fail(1, "bad argument");
+  SampleUpload u;
+  if (int rc = u.load(L->stream, X, a, b, n)) return rc;
+  if (int rc = learner_step(L, u.X, u.a, u.b, nullptr, 0, n, 0.f)) return rc;
+  std::vector<float> g(umt::kMlpParams + 1);
+  HIP_TRY(hipMemcpyAsync(g.data(), L->grad, g.size() * sizeof(float), hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  if (grad) std::memcpy(grad, g.data(), umt::kMlpParams * sizeof(float));
+  if (sse) *sse = g[umt::kMlpParams];
+  return 0;
+}
+
+int umt_learner_step(umt_learner* L, const float* X, const float* a, const float* b, int64_t n,
+                     double lr) {
+  if (!L || !X || !a || !b || n < 1 || !(lr > 0.0)) return fail(1, "bad argument");
+  SampleUpload u;
+  if (int rc = u.load(L->stream, X, a, b, n)) return rc;
+  if (int rc = learner_step(L, u.X, u.a, u.b, nullptr, 0, n, (float)lr)) return rc;
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int umt_learner_add_samples(umt_learner* L, const float* X, const float* a, const float* b, int64_t n) {
+  if (!L || n < 0 || (n > 0 && (!X || !a || !b))) return fail(1, "bad argument");
+  if (n == 0) return 0;
+  if (int rc = learner_reserve(L, L->n + n)) return rc;
+  HIP_TRY(hipMemcpyAsync(L->X + (size_t)L->n * umt::kMlpFeatures, X, (size_t)n * umt::kMlpFeatures * sizeof(float),
+                         hipMemcpyHostToDevice, L->stream));
+  HIP_TRY(hipMemcpyAsync(L->a + L->n, a, (size_t)n * sizeof(float), hipMemcpyHostToDevice, L->stream));
+  HIP_TRY(hipMemcpyAsync(L->b + L->n, b, (size_t)n * sizeof(float), hipMemcpyHostToDevice, L->stream));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  L->n += n;
+  return 0;
+}
+
+int umt_learner_num_samples(const umt_learner* L, int64_t* n) {
+  if (!L || !n) return fail(1, "null argument");
+  *n = L->n;
+  return 0;
+}
+
+int umt_learner_get_samples(const umt_learner* L, float* X, float* a, float* b) {
+  if (!L || !X || !a || !b) return fail(1, "null argument");
+  if (!L->n) return 0;
+  HIP_TRY(hipMemcpyAsync(X, L->X, (size_t)L->n * umt::kMlpFeatures * sizeof(float), hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(hipMemcpyAsync(a, L->a, (size_t)L->n * sizeof(float), hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(hipMemcpyAsync(b, L->b, (size_t)L->n * sizeof(float), hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int umt_learner_clear_samples(umt_learner* L) {
+  if (!L) return fail(1, "null learner");
+  L->n = 0;
+  return 0;
+}
+
+int umt_learner_train(umt_learner* L, int epochs, int batch, double lr, uint64_t seed, double* meanLoss) {
+  if (!L || epochs < 1 || batch < 1 || !(lr > 0.0)) return fail(1, "bad argument");
+  if (L->n < 1) return fail(42, "the learner holds no training samples");
+  const long n = L->n;
+  const long B = std::min<long>(batch, n);
+  const long steps = (n + B - 1) / B;
+  if (n > L->permCap) {
+    dfree(L->perm);
+    if (int rc = dalloc(&L->perm, (size_t)n)) return rc;
+    L->permCap = n;
+  }
+  if (steps > L->lossCap) {
+    dfree(L->lossLog);
+    if (int rc = dalloc(&L->lossLog, (size_t)steps)) return rc;
+    L->lossCap = steps;
+  }
+  std::vector<int> perm(n);
+  uint64_t st = seed;
+  for (int e = 0; e < epochs; ++e) {
+    for (long i = 0; i < n; ++i) perm[i] = (int)i;
+    for (long i = n - 1; i > 0; --i) {  // Fisher-Yates
+      const long j = (long)(next_unit(st) * (double)(i + 1));
+      std::swap(perm[i], perm[std::min(j, i)]);
+    }
+    HIP_TRY(hipMemcpyAsync(L->perm, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, L->stream));
+    for (long k = 0; k < steps; ++k) {
+      const long start = k * B, count = std::min(B, n - start);
+      if (int rc = learner_step(L, L->X, L->a, L->b, L->perm, start, count, (float)lr)) return rc;
+      HIP_TRY(hipMemcpyAsync(L->lossLog + k, L->grad + umt::kMlpParams, sizeof(float),
+                             hipMemcpyDeviceToDevice, L->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(L->stream));  // perm is rewritten next epoch
+  }
+  std::vector<float> loss(steps);
+  HIP_TRY(hipMemcpyAsync(loss.data(), L->lossLog, steps * sizeof(float), hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  double s = 0.0;
+  for (float v : loss) s += v;
+  if (meanLoss) *meanLoss = s / (double)n;
+  if (!std::isfinite(s)) return fail(43, "training loss is not finite");
+  return 0;
+}
+
+int umt_ctx_set_zone_neighbors(umt_ctx* x, const int* zoneOpp) {
+  if (!x || !zoneOpp) return fail(1, "null argument");
+  std::vector<int> nb(6 * (size_t)x->Z);
+  for (size_t i = 0; i < nb.size(); ++i) {
+    const int z = zoneOpp[i];
+    if (z > x->Z) return fail(44, "zoneOpp entry %zu (%d) exceeds the zone count", i + 1, z);
+    nb[i] = z > 0 ? z - 1 : -1;
+  }
+  if (!x->learn.zoneNbr)
+    if (int rc = dalloc(&x->learn.zoneNbr, nb.size())) return rc;
+  HIP_TRY(hipMemcpyAsync(x->learn.zoneNbr, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_set_learner(umt_ctx* x, umt_learner* L, int every, int guard) {
+  if (!x) return fail(1, "null context");
+  if (every < 1) return fail(1, "every must be at least 1");
+  x->learn.net = L;
+  x->learn.every = every;
+  x->learn.guard = guard != 0;
+  if (!L && x->acceleration == 2) x->acceleration = 0;
+  return 0;
+}
+
+int umt_ctx_learn_collect(umt_ctx* x, int on) {
+  if (!x) return fail(1, "null context");
+  x->learn.collect = on != 0;
+  return 0;
+}
+
+int umt_ctx_learn_commit(umt_ctx* x, umt_learner* L, double minResidual, int* nStates) {
+  if (!x || !L) return fail(1, "null argument");
+  LearnState& S = x->learn;
+  const long GZ = (long)x->G * x->Z;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  int kept = 0;
+  for (auto& p : S.pending)
+    if (p.res >= minResidual) ++kept;
+  if (int rc = learner_reserve(L, L->n + (long)kept * GZ)) return rc;
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  // Targets against the current zone averages (the converged state).
+  for (auto& p : S.pending) {
+    if (p.res < minResidual) continue;
+    HIP_TRY(umt::launch_learn_targets(p.ph, p.d, x->phibar, GZ, S.ta, S.tb, S.tpart, L->a + L->n,
+                                      L->b + L->n, x->stream));
+    HIP_TRY(hipMemcpyAsync(L->X + (size_t)L->n * umt::kMlpFeatures, p.X, (size_t)GZ * umt::kMlpFeatures * sizeof(float),
+                           hipMemcpyDeviceToDevice, x->stream));
+    L->n += GZ;
+  }
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  free_pending(S);
+  if (nStates) *nStates = kept;
+  return 0;
+}
+
+int umt_ctx_learn_discard(umt_ctx* x) {
+  if (!x) return fail(1, "null context");
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  free_pending(x->learn);
+  return 0;
+}
+
+int umt_ctx_learn_features(umt_ctx* x, const double* ph, const double* pp, const double* dp, float* X) {
+  if (!x || !ph || !pp || !X) return fail(1, "null argument");
+  if (int rc = learn_prepare(x)) return rc;
+  LearnState& L = x->learn;
+  const size_t GZ = (size_t)x->G * x->Z;
+  HIP_TRY(hipMemcpyAsync(L.ta, ph, GZ * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(L.pp, pp, GZ * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  if (dp) HIP_TRY(hipMemcpyAsync(L.dp, dp, GZ * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(umt::launch_learn_features(L.ta, L.pp, dp ? L.dp : nullptr, L.sigt64, x->sigs, x->q,
+                                     L.zoneNbr, x->G, (long)GZ, L.rel, L.ratio, L.maxRel, L.X, x->stream));
+  HIP_TRY(hipMemcpyAsync(X, L.X, GZ * umt::kMlpFeatures * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_learn_stats(const umt_ctx* x, int64_t* corrections, int* fellBackAt) {
+  if (!x) return fail(1, "null context");
+  if (corrections) *corrections = x->learn.corrections;
+  if (fellBackAt) *fellBackAt = x->learn.fellBack;
   return 0;
 }
 

@@ -269,12 +269,14 @@ __global__ __launch_bounds__(256) void source_update(const double* __restrict__ 
   S[i] = (Real)(wtiso * (sigs[zg] * phi[i] + q[zg]));
 }
 
-// Zone averages phibar(g,z) = sum_c V_c phi(g,c) / V_z and the running max.
+// Zone averages phibar(g,z) = sum_c V_c phi(g,c) / V_z and the running max
+// (of the values, or of their magnitudes when magnitude != 0).
 __global__ __launch_bounds__(256) void zone_average(const double* __restrict__ phi,
                                                     const double* __restrict__ vol,
                                                     const double* __restrict__ volZ, int G,
                                                     long n, double* __restrict__ phibar,
-                                                    unsigned long long* __restrict__ maxBits) {
+                                                    unsigned long long* __restrict__ maxBits,
+                                                    int magnitude) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (i < n) {
@@ -286,6 +288,7 @@ __global__ __launch_bounds__(256) void zone_average(const double* __restrict__ p
     phibar[i] = v;
   }
   // Wave-level max of non-negative values, then one atomic per wave.
+  if (magnitude) v = fabs(v);
   double m = v > 0.0 ? v : 0.0;
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
   if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxBits, (unsigned long long)__double_as_longlong(m));
@@ -293,18 +296,21 @@ __global__ __launch_bounds__(256) void zone_average(const double* __restrict__ p
 
 // Max relative change of zone averages over entries above cutoff * max
 // (rt/ConvergenceTest.F90:150-168 semantics per zone and group), and copy of
-// the new averages into the old.
+// the new averages into the old. With magnitude != 0 the test is on |phibar|
+// (the same for non-negative fluxes; corrected iterates may go negative).
 __global__ __launch_bounds__(256) void residual_max(const double* __restrict__ phibar,
                                                     double* __restrict__ phibarOld, long n,
                                                     const unsigned long long* __restrict__ maxBits,
                                                     double cutoff,
-                                                    unsigned long long* __restrict__ resBits) {
+                                                    unsigned long long* __restrict__ resBits,
+                                                    int magnitude) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
   if (i < n) {
     const double thr = cutoff * __longlong_as_double((long long)*maxBits);
     const double v = phibar[i];
-    if (v > thr) r = fabs((v - phibarOld[i]) / v);
+    const double a = magnitude ? fabs(v) : v;
+    if (a > thr) r = fabs((v - phibarOld[i]) / v);
     phibarOld[i] = v;
   }
   for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_down(r, off, 64));
@@ -389,12 +395,20 @@ hipError_t launch_source_update(const double* phi, const double* sigs, const dou
 
 hipError_t launch_residual(const double* phi, const double* vol, const double* volZ, int G, long nzg,
                            double* phibar, double* phibarOld, double cutoff,
-                           unsigned long long* bits, hipStream_t s) {
+                           unsigned long long* bits, hipStream_t s, bool magnitude) {
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, bits, 2, 0ull);
   hipLaunchKernelGGL(zone_average, dim3(blocks_for(nzg, 256)), dim3(256), 0, s, phi, vol, volZ, G,
-                     nzg, phibar, bits);
+                     nzg, phibar, bits, (int)magnitude);
   hipLaunchKernelGGL(residual_max, dim3(blocks_for(nzg, 256)), dim3(256), 0, s, phibar, phibarOld,
-                     nzg, bits, cutoff, bits + 1);
+                     nzg, bits, cutoff, bits + 1, (int)magnitude);
+  return hipGetLastError();
+}
+
+hipError_t launch_zone_average(const double* phi, const double* vol, const double* volZ, int G,
+                               long nzg, double* phibar, unsigned long long* maxBits,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(zone_average, dim3(blocks_for(nzg, 256)), dim3(256), 0, s, phi, vol, volZ, G,
+                     nzg, phibar, maxBits, 0);
   return hipGetLastError();
 }
 

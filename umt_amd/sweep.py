@@ -225,9 +225,48 @@ class SweepContext:
         return out
 
     def set_acceleration(self, mode):
-        """"none" (plain source iteration) or "gta" (grey-transport acceleration)."""
-        m = {"none": 0, "gta": 1}[mode]
+        """"none" (plain source iteration), "gta" (grey-transport acceleration)
+        or "learned" (the network given to set_learner)."""
+        m = {"none": 0, "gta": 1, "learned": 2}[mode]
         check("umt_ctx_set_acceleration", _lib.load().umt_ctx_set_acceleration(self._h, m))
+
+    def set_learner(self, learner, every=2, guard=True):
+        """Network for mode "learned" (None detaches); corrections every
+        `every` sweeps; guard stops them once one raises the residual."""
+        self._learner = learner  # keep it alive while attached
+        h = learner.handle if learner is not None else None
+        check("umt_ctx_set_learner", _lib.load().umt_ctx_set_learner(self._h, h, every, int(bool(guard))))
+
+    def learn_collect(self, on=True):
+        """Record the states source iteration could correct (training data)."""
+        check("umt_ctx_learn_collect", _lib.load().umt_ctx_learn_collect(self._h, int(bool(on))))
+
+    def learn_commit(self, learner, min_residual=1e-9):
+        """Turn the recorded states into samples of learner, with targets
+        against the current (converged) zone averages. Returns the state count."""
+        n = C.c_int(0)
+        check("umt_ctx_learn_commit", _lib.load().umt_ctx_learn_commit(
+            self._h, learner.handle, min_residual, C.byref(n)))
+        return n.value
+
+    def learn_discard(self):
+        check("umt_ctx_learn_discard", _lib.load().umt_ctx_learn_discard(self._h))
+
+    def learn_features(self, ph, pp, dp=None):
+        """Features (Z, G, 16) for zone averages ph, pp, dp given as (Z, G)."""
+        a, b = _f64(ph), _f64(pp)
+        c = _f64(dp) if dp is not None else None
+        X = np.zeros((self.problem.nzones, self.G, 16), dtype=np.float32)
+        check("umt_ctx_learn_features", _lib.load().umt_ctx_learn_features(
+            self._h, _dp(a), _dp(b), _dp(c) if c is not None else None,
+            X.ctypes.data_as(C.POINTER(C.c_float))))
+        return X
+
+    def learn_stats(self):
+        """(corrections applied so far, iteration the last solve stopped correcting at or 0)."""
+        a, b = C.c_int64(0), C.c_int(0)
+        check("umt_ctx_learn_stats", _lib.load().umt_ctx_learn_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_graphs(self, on):
         """Replay each sweep's level launches from a captured hipGraph."""
