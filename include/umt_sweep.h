@@ -186,6 +186,18 @@ int umt_ctx_get_groups_per_lane(const umt_ctx *ctx, int *K);
  * graph holds kernel nodes only). No counterpart in the reference. */
 int umt_ctx_set_graphs(umt_ctx *ctx, int on);
 
+/* Sweep the angles in blocks of n (default: all in one block). Each block
+ * runs its own hyperplane levels and reuses one psi buffer of n angles, so
+ * psi memory is (C + nbelem) * G * n values; phi sums the blocks in angle
+ * order. With more than one block umt_ctx_get_psi is unavailable. No
+ * counterpart in the reference (its angle sets, mods/QuadratureList_mod.F90,
+ * play the same role for threads). */
+int umt_ctx_set_angles_per_block(umt_ctx *ctx, int n);
+int umt_ctx_get_angles_per_block(const umt_ctx *ctx, int *n, int *blocks);
+
+/* dst's phi = src's phi, device to device (same groups and corners). */
+int umt_ctx_copy_phi(umt_ctx *dst, const umt_ctx *src);
+
 /* Autotuner: times reps sweeps with each supported groups-per-lane value on
  * the context's own problem and keeps the fastest. msPerK[0..2] receives the
  * mean ms per sweep for K = 1, 2, 4 (-1 when unsupported); may be NULL.

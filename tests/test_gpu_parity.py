@@ -179,3 +179,32 @@ def test_fp32_sweep_error_against_fp64_oracle(config1_problem):
     print(f"FP32 sweep: max rel phi error vs FP64 oracle = {err:.3e}")
     assert err < 2e-5
     ctx.close()
+
+
+@pytest.mark.parametrize("n", [1, 3, 7, 24])
+def test_angle_blocks_give_the_same_bits(n):
+    """Sweeping the angles in blocks of n (one psi buffer of n angles) gives
+    exactly the one-block phi, and the oracle's to 1e-12, with and without
+    graph replay."""
+    p = U.TiledProblem(2, 2, 2, 3, 1)  # 24 angles
+    G = 4
+    st, _, _, S = _inputs(p, G, sigt=(1e-2, 1e3), c=(0.0, 0.95), q=(0.0, 1.0))
+    print(f"case angles_per_block={n}", flush=True)
+    ctx = U.SweepContext(p, G, precision=64)
+    ctx.set_sigt(st)
+    ctx.set_source(S)
+    ctx.sweep()
+    one = ctx.phi()
+    ctx.set_angles_per_block(n)
+    assert ctx.angles_per_block() == (n, -(-24 // n))
+    for graphs in (False, True, True):
+        ctx.set_graphs(graphs)
+        ctx.set_phi(np.zeros_like(one))
+        ctx.sweep()
+        np.testing.assert_array_equal(ctx.phi(), one)
+    ref, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st)
+    assert rel_err(one, ref) < 1e-12
+    if n < 24:
+        with pytest.raises(U.UmtError):
+            ctx.psi()
+    ctx.close()

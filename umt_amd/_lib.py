@@ -62,6 +62,9 @@ SIGNATURES = {
     "umt_ctx_enable_gta": (_i, [_vp, _i, _pd, _pd, _pi, _pi, _pi]),
     "umt_ctx_set_acceleration": (_i, [_vp, _i]),
     "umt_ctx_set_graphs": (_i, [_vp, _i]),
+    "umt_ctx_set_angles_per_block": (_i, [_vp, _i]),
+    "umt_ctx_get_angles_per_block": (_i, [_vp, _pi, _pi]),
+    "umt_ctx_copy_phi": (_i, [_vp, _vp]),
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
     "umt_learner_num_params": (_i, []),
     "umt_learner_create": (_i, [_u64, C.POINTER(_vp)]),

@@ -20,6 +20,7 @@ struct SweepLevelArgs {
   const double* omega;      // (3, A)
   Real* psi;                // (G, C + nbelem, A) all angles, or one angle
   long psiAngleStride;      // (C + nbelem) * G
+  int angle0;               // first angle held in psi (angle blocks)
   // Single-angle (drop-in) extras.
   double* phi;              // (G, C), phi += quadwt * psi
   double quadwt;

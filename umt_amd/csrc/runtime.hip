@@ -95,23 +95,35 @@ static const double kGtaParallelTol = 1e-12;
 
 // One quadrature's sweep: directions, device-resident hyperplane batches
 // (level h = plane h of every angle) and psi for all its angles.
+// Angles are swept in blocks of anglesPerBlock (all of them by default):
+// block b holds angles [b * anglesPerBlock, ...), its own levels (plane h of
+// each of its angles) and reuses one psi buffer of anglesPerBlock angles.
 struct SweepSet {
   int A = 0;
+  int anglesPerBlock = 0;
   double* omega = nullptr;   // (3, A)
   double* weight = nullptr;  // (A)
-  int2* items = nullptr;     // (zone, angle), level by level
-  std::vector<int> levelOffset;
-  void* psi = nullptr;       // (G, C + nbelem, A), sweep precision
+  int2* items = nullptr;     // (zone, angle), block by block, level by level
+  std::vector<int> levelOffset;  // item offsets of the levels, all blocks
+  std::vector<int> blockLevel;   // first level of each block (+ end)
+  std::vector<int2> allItems;    // the unblocked levels (plane h of every angle)
+  std::vector<int> allLevelOffset;
+  void* psi = nullptr;       // (G, C + nbelem, anglesPerBlock), sweep precision
   std::vector<hipEvent_t> evLevel;
-  // Captured level launches, keyed by (source, cross section, groups per lane).
+  // Captured sweeps, keyed by source, cross section, groups per lane and
+  // the phi they write.
   struct Graph {
     const void* S;
     const void* sigt;
     int K;
+    const double* phi;
+    bool accumulate;
     hipGraph_t graph;
     hipGraphExec_t exec;
   };
   std::vector<Graph> graphs;
+  int blocks() const { return (int)blockLevel.size() - 1; }
+  int levels() const { return (int)levelOffset.size() - 1; }
 };
 
 // Grey-transport acceleration state (per-group one-group GTA).
@@ -219,7 +231,7 @@ namespace {
 
 template <typename Real>
 umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, const void* S,
-                                     const void* sigt, int level) {
+                                     const void* sigt, int level, int angle0) {
   umt::SweepLevelArgs<Real> a{};
   a.items = set.items + set.levelOffset[level];
   a.nItems = set.levelOffset[level + 1] - set.levelOffset[level];
@@ -235,30 +247,49 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
   a.omega = set.omega;
   a.psi = (Real*)set.psi;
   a.psiAngleStride = x->psiStride();
+  a.angle0 = angle0;
   return a;
 }
 
-// The level launches of one sweep, as a graph replayed on later sweeps with
-// the same source, cross section and groups per lane. The graph holds only
+// The launches of one sweep: per angle block its levels, then its share of
+// phi (FP64, angle order).
+template <typename Real>
+int launch_sweep(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
+                 bool accumulate, bool timeLevels) {
+  for (int b = 0; b < set.blocks(); ++b) {
+    const int a0 = b * set.anglesPerBlock;
+    const int nA = std::min(set.anglesPerBlock, set.A - a0);
+    for (int h = set.blockLevel[b]; h < set.blockLevel[b + 1]; ++h) {
+      if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h], x->stream));
+      HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h, a0), false, x->stream));
+      if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h + 1], x->stream));
+    }
+    HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)set.psi, set.weight + a0, nA,
+                                               x->psiStride(), (long)x->C * x->G, phiOut,
+                                               accumulate || b > 0, x->stream));
+  }
+  return 0;
+}
+
+// One sweep's launches as a graph, replayed on later sweeps with the same
+// source, cross section, groups per lane and output. The graph holds only
 // kernel nodes (no memset or copy nodes).
 template <typename Real>
-int level_graph(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, hipGraphExec_t* out) {
+int sweep_graph(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
+                bool accumulate, hipGraphExec_t* out) {
   for (auto& g : set.graphs)
-    if (g.S == S && g.sigt == sigt && g.K == x->groupsPerLane) {
+    if (g.S == S && g.sigt == sigt && g.K == x->groupsPerLane && g.phi == phiOut &&
+        g.accumulate == accumulate) {
       *out = g.exec;
       return 0;
     }
-  const int L = (int)set.levelOffset.size() - 1;
-  SweepSet::Graph g{S, sigt, x->groupsPerLane, nullptr, nullptr};
+  SweepSet::Graph g{S, sigt, x->groupsPerLane, phiOut, accumulate, nullptr, nullptr};
   HIP_TRY(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
-  for (int h = 0; h < L; ++h) {
-    hipError_t e = umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h), false, x->stream);
-    if (e != hipSuccess) {
-      hipGraph_t dummy = nullptr;
-      (void)hipStreamEndCapture(x->stream, &dummy);
-      if (dummy) (void)hipGraphDestroy(dummy);
-      return fail(100 + (int)e, "level launch during capture failed: %s", hipGetErrorString(e));
-    }
+  if (int rc = launch_sweep<Real>(x, set, S, sigt, phiOut, accumulate, false)) {
+    hipGraph_t dummy = nullptr;
+    (void)hipStreamEndCapture(x->stream, &dummy);
+    if (dummy) (void)hipGraphDestroy(dummy);
+    return rc;
   }
   HIP_TRY(hipStreamEndCapture(x->stream, &g.graph));
   HIP_TRY(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
@@ -272,22 +303,13 @@ int level_graph(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, hipG
 template <typename Real>
 int sweep_set_impl(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
                    bool accumulate, bool timeLevels) {
-  const int L = (int)set.levelOffset.size() - 1;
   if (x->useGraphs && !timeLevels) {
     hipGraphExec_t exec = nullptr;
-    if (int rc = level_graph<Real>(x, set, S, sigt, &exec)) return rc;
+    if (int rc = sweep_graph<Real>(x, set, S, sigt, phiOut, accumulate, &exec)) return rc;
     HIP_TRY(hipGraphLaunch(exec, x->stream));
-  } else {
-    for (int h = 0; h < L; ++h) {
-      if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h], x->stream));
-      HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h), false, x->stream));
-      if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h + 1], x->stream));
-    }
+    return 0;
   }
-  HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)set.psi, set.weight, set.A,
-                                             x->psiStride(), (long)x->C * x->G, phiOut, accumulate,
-                                             x->stream));
-  return 0;
+  return launch_sweep<Real>(x, set, S, sigt, phiOut, accumulate, timeLevels);
 }
 
 int sweep_set(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
@@ -540,28 +562,66 @@ static int batch_levels(int Z, int A, const int* nHyperPlanes, const int* zonesI
   return 0;
 }
 
-static int upload_set(umt_ctx* x, SweepSet& set, int A, const double* omega, const double* weight,
-                      const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ) {
+// Splits the all-angle levels into angle blocks of n angles and (re)builds
+// the device items, the psi buffer and the level events.
+static int block_set(umt_ctx* x, SweepSet& set, int n) {
+  n = std::max(1, std::min(n, set.A));
+  const int nBlocks = (set.A + n - 1) / n;
   std::vector<int2> items;
-  if (int rc = batch_levels(x->Z, A, nHyperPlanes, zonesInPlane, nextZ, items, set.levelOffset))
-    return rc;
-  set.A = A;
-  const size_t nPsi = (size_t)x->psiStride() * A;
+  items.reserve(set.allItems.size());
+  std::vector<int> lo(1, 0), bl(1, 0);
+  const int L = (int)set.allLevelOffset.size() - 1;
+  for (int b = 0; b < nBlocks; ++b) {
+    const int a0 = b * n, a1 = std::min(set.A, a0 + n);
+    for (int h = 0; h < L; ++h) {
+      const size_t before = items.size();
+      for (int i = set.allLevelOffset[h]; i < set.allLevelOffset[h + 1]; ++i) {
+        const int2 it = set.allItems[i];
+        if (it.y >= a0 && it.y < a1) items.push_back(it);
+      }
+      if (items.size() > before) lo.push_back((int)items.size());
+    }
+    bl.push_back((int)lo.size() - 1);
+  }
+  if (x->stream) HIP_TRY(hipStreamSynchronize(x->stream));
+  for (auto& g : set.graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  set.graphs.clear();
+  for (auto e : set.evLevel) (void)hipEventDestroy(e);
+  set.evLevel.clear();
+  dfree(set.items);
+  dfree(set.psi);
+  const size_t nPsi = (size_t)x->psiStride() * n;
   int rc;
-  if ((rc = dalloc(&set.omega, 3 * (size_t)A)) || (rc = dalloc(&set.weight, (size_t)A)) ||
-      (rc = dalloc(&set.items, items.size())) || (rc = dalloc((char**)&set.psi, nPsi * x->rsize())))
+  if ((rc = dalloc(&set.items, items.size())) || (rc = dalloc((char**)&set.psi, nPsi * x->rsize())))
     return rc;
-  HIP_TRY(hipMemcpyAsync(set.omega, omega, 3 * (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
-  HIP_TRY(hipMemcpyAsync(set.weight, weight, (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipMemcpyAsync(set.items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, x->stream));
   if (x->precision == 32)
     HIP_TRY(umt::launch_fill_zero<float>((float*)set.psi, (long)nPsi, x->stream));
   else
     HIP_TRY(umt::launch_fill_zero<double>((double*)set.psi, (long)nPsi, x->stream));
-  set.evLevel.resize(2 * (set.levelOffset.size() - 1));
+  set.anglesPerBlock = n;
+  set.levelOffset = lo;
+  set.blockLevel = bl;
+  set.evLevel.resize(2 * (size_t)set.levels());
   for (auto& e : set.evLevel) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
+}
+
+static int upload_set(umt_ctx* x, SweepSet& set, int A, const double* omega, const double* weight,
+                      const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ) {
+  if (int rc = batch_levels(x->Z, A, nHyperPlanes, zonesInPlane, nextZ, set.allItems,
+                            set.allLevelOffset))
+    return rc;
+  set.A = A;
+  int rc;
+  if ((rc = dalloc(&set.omega, 3 * (size_t)A)) || (rc = dalloc(&set.weight, (size_t)A))) return rc;
+  HIP_TRY(hipMemcpyAsync(set.omega, omega, 3 * (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipMemcpyAsync(set.weight, weight, (size_t)A * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  return block_set(x, set, A);
 }
 
 static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, const int* numCorner,
@@ -679,7 +739,7 @@ int umt_ctx_create_from_problem(const umt_problem* p, int groups, int precision,
 int umt_ctx_dims(const umt_ctx* x, int64_t* d) {
   if (!x || !d) return fail(1, "null argument");
   d[0] = x->Z; d[1] = x->C; d[2] = x->nb; d[3] = x->G; d[4] = x->sn.A;
-  d[5] = (int64_t)x->sn.levelOffset.size() - 1;
+  d[5] = x->sn.levels();
   d[6] = x->sn.levelOffset.back();
   return 0;
 }
@@ -746,6 +806,29 @@ int umt_ctx_set_groups_per_lane(umt_ctx* x, int K) {
 int umt_ctx_set_graphs(umt_ctx* x, int on) {
   if (!x) return fail(1, "null context");
   x->useGraphs = on != 0;
+  return 0;
+}
+
+int umt_ctx_set_angles_per_block(umt_ctx* x, int n) {
+  if (!x) return fail(1, "null context");
+  if (n < 1) return fail(1, "angles per block must be at least 1");
+  return block_set(x, x->sn, n);
+}
+
+int umt_ctx_get_angles_per_block(const umt_ctx* x, int* n, int* blocks) {
+  if (!x) return fail(1, "null context");
+  if (n) *n = x->sn.anglesPerBlock;
+  if (blocks) *blocks = x->sn.blocks();
+  return 0;
+}
+
+int umt_ctx_copy_phi(umt_ctx* dst, const umt_ctx* src) {
+  if (!dst || !src) return fail(1, "null context");
+  if (dst->G != src->G || dst->C != src->C) return fail(46, "contexts differ in groups or corners");
+  HIP_TRY(hipStreamSynchronize(src->stream));
+  HIP_TRY(hipMemcpyAsync(dst->phi, src->phi, (size_t)src->G * src->C * sizeof(double),
+                         hipMemcpyDeviceToDevice, dst->stream));
+  HIP_TRY(hipStreamSynchronize(dst->stream));
   return 0;
 }
 
@@ -835,6 +918,8 @@ int umt_ctx_set_phi(umt_ctx* x, const double* phi) {
 
 int umt_ctx_get_psi(const umt_ctx* x, double* psi) {
   if (!x || !psi) return fail(1, "null argument");
+  if (x->sn.blocks() != 1)
+    return fail(45, "psi of every angle is held only with one angle block (umt_ctx_set_angles_per_block)");
   const size_t n = (size_t)x->psiStride() * x->sn.A;
   if (x->precision == 64) {
     HIP_TRY(hipMemcpyAsync(psi, x->sn.psi, n * sizeof(double), hipMemcpyDeviceToHost, x->stream));

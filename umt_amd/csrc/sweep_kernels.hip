@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
   const Real ox = (Real)p.omega[3 * a + 0];
   const Real oy = (Real)p.omega[3 * a + 1];
   const Real oz = (Real)p.omega[3 * a + 2];
-  const long angleBase = kSingleAngle ? 0L : (long)a * p.psiAngleStride;
+  const long angleBase = kSingleAngle ? 0L : (long)(a - p.angle0) * p.psiAngleStride;
   Real* __restrict__ psiA = p.psi + angleBase;
 
   const Real v = p.vol[cc];
@@ -248,9 +248,11 @@ __global__ __launch_bounds__(256) void phi_from_psi(const Real* __restrict__ psi
                                                     double* __restrict__ phi, int accumulate) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double acc = 0.0;
+  // Continuing from phi when accumulating keeps the sum in angle order, so
+  // sweeping the angles in blocks gives the same bits as one block.
+  double acc = accumulate ? phi[i] : 0.0;
   for (int a = 0; a < A; ++a) acc += weight[a] * (double)psi[(long)a * angleStride + i];
-  phi[i] = accumulate ? phi[i] + acc : acc;
+  phi[i] = acc;
 }
 
 // Source for the next sweep: S = (sigma_s phi + q) / (4 pi), within-group

@@ -272,6 +272,21 @@ class SweepContext:
         """Replay each sweep's level launches from a captured hipGraph."""
         check("umt_ctx_set_graphs", _lib.load().umt_ctx_set_graphs(self._h, int(bool(on))))
 
+    def set_angles_per_block(self, n):
+        """Sweep angles in blocks of n (psi memory for n angles)."""
+        check("umt_ctx_set_angles_per_block", _lib.load().umt_ctx_set_angles_per_block(self._h, n))
+
+    def angles_per_block(self):
+        """(angles per block, blocks)."""
+        n, b = C.c_int(0), C.c_int(0)
+        check("umt_ctx_get_angles_per_block",
+              _lib.load().umt_ctx_get_angles_per_block(self._h, C.byref(n), C.byref(b)))
+        return n.value, b.value
+
+    def copy_phi_from(self, other):
+        """phi = other's phi (device to device)."""
+        check("umt_ctx_copy_phi", _lib.load().umt_ctx_copy_phi(self._h, other._h))
+
     def gta_stats(self):
         """(BiCGStab iterations, S2 sweeps) done by GTA so far."""
         a, b = C.c_int64(0), C.c_int64(0)
