@@ -175,6 +175,15 @@ int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const dou
 int umt_ctx_set_acceleration(umt_ctx *ctx, int mode);
 /* Cumulative BiCGStab iterations and S2 sweeps done by GTA on this context. */
 int umt_ctx_gta_stats(const umt_ctx *ctx, int64_t *greyIterations, int64_t *greySweeps);
+/* The grey operator on host arrays (groups, corners): out = M(P), the S2
+ * grey sweep of wtiso (sigma_s P + gsrc) followed by the stretched-TSA sweep
+ * (rt/GreySweep.F90:60-96); P or gsrc NULL stand for zero. */
+int umt_ctx_grey_sweep(umt_ctx *ctx, const double *P, const double *gsrc, double *out);
+/* One grey solve (rt/GTASolver.F90:185-402) for the grey source gsrc after a
+ * sweep that gave phiHalf: F = the grey correction; greyIterations = the
+ * BiCGStab iterations it took. */
+int umt_ctx_grey_solve(umt_ctx *ctx, const double *gsrc, const double *phiHalf, double epsGrey,
+                       double *F, int64_t *greyIterations);
 
 /* Groups each lane of the sweep kernels carries (K = 1, 2 or 4, dividing
  * the group count; default 1). No counterpart in the reference. */

@@ -43,6 +43,54 @@ def test_gta_s2_schedule_matches_oracle(thick):
         np.testing.assert_array_equal(gq[k], s2[k])
 
 
+def test_s2_sweep_matches_oracle(thick):
+    """Without scattering the grey operator is one S2 sweep of wtiso gsrc."""
+    p, G, st, ss, qq, q2, s2 = thick
+    ctx = _ctx(p, G, st, np.zeros_like(ss), qq)
+    rng = np.random.default_rng(1)
+    gsrc = rng.uniform(0.0, 1.0, (p.ncorner, G))
+    got = ctx.grey_sweep(None, gsrc)
+    ref, _ = O.sweep_all(p.mesh(), p.geometry(), q2, s2, gsrc / (4 * np.pi), st)
+    err = rel_err(got, ref)
+    print(f"S2 sweep rel err {err:.3e}", flush=True)
+    assert err < 1e-12
+    ctx.close()
+
+
+def test_grey_operator_matches_oracle(thick):
+    p, G, st, ss, qq, q2, s2 = thick
+    ctx = _ctx(p, G, st, ss, qq)
+    solver = OG.GreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    rng = np.random.default_rng(2)
+    P = rng.uniform(0.5, 2.0, (p.ncorner, G))
+    gsrc = rng.uniform(0.0, 1.0, (p.ncorner, G))
+    for args in ((P, None), (None, gsrc), (P, gsrc)):
+        got = ctx.grey_sweep(*args)
+        ref = solver.grey_sweep(*args) if args[0] is not None else solver.grey_sweep(
+            np.zeros_like(P), args[1])
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        print(f"grey operator ({args[0] is not None}, {args[1] is not None}) err {err:.3e}", flush=True)
+        assert err < 1e-12
+    ctx.close()
+
+
+def test_grey_solve_matches_oracle(thick):
+    p, G, st, ss, qq, q2, s2 = thick
+    ctx = _ctx(p, G, st, ss, qq)
+    solver = OG.GreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    rng = np.random.default_rng(3)
+    half = rng.uniform(0.5, 2.0, (p.ncorner, G))
+    gsrc = np.repeat(ss, 8, axis=0) * rng.normal(0.0, 0.1, (p.ncorner, G))
+    for eps in (1e-5, 1e-8):
+        F, n = ctx.grey_solve(gsrc, half, eps)
+        Fr, nr = solver.solve(gsrc, half, eps)
+        err = np.max(np.abs(F - Fr)) / np.max(np.abs(Fr))
+        print(f"grey solve eps={eps}: GPU {n} iterations, oracle {(nr - 1) // 2}; err {err:.3e}", flush=True)
+        assert n == (nr - 1) // 2
+        assert err < 1e-9
+    ctx.close()
+
+
 def test_gta_iterates_match_oracle(thick):
     """Three accelerated iterations, compared iterate by iterate."""
     p, G, st, ss, qq, q2, s2 = thick

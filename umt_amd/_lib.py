@@ -66,6 +66,8 @@ SIGNATURES = {
     "umt_ctx_get_angles_per_block": (_i, [_vp, _pi, _pi]),
     "umt_ctx_copy_phi": (_i, [_vp, _vp]),
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
+    "umt_ctx_grey_sweep": (_i, [_vp, _pd, _pd, _pd]),
+    "umt_ctx_grey_solve": (_i, [_vp, _pd, _pd, _d, _pd, _pi64]),
     "umt_learner_num_params": (_i, []),
     "umt_learner_create": (_i, [_u64, C.POINTER(_vp)]),
     "umt_learner_destroy": (None, [_vp]),

@@ -1185,6 +1185,15 @@ int gta_solve(umt_ctx* x, const double* phiHalf, double epsGrey) {
   return 0;
 }
 
+// Host-array wrappers of the grey operator and solve (their parity is
+// checked piece by piece against the oracle).
+int grey_upload(umt_ctx* x, double* dst, const double* src) {
+  const size_t GC = (size_t)x->G * x->C;
+  if (src) HIP_TRY(hipMemcpyAsync(dst, src, GC * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  else HIP_TRY(umt::launch_fill_zero<double>(dst, (long)GC, x->stream));
+  return 0;
+}
+
 int learn_prepare(umt_ctx* x) {
   LearnState& L = x->learn;
   if (!L.zoneNbr) return fail(39, "learned acceleration needs umt_ctx_set_zone_neighbors first");
@@ -1224,6 +1233,37 @@ int learn_record(umt_ctx* x, double res) {
 }  // namespace
 
 extern "C" {
+
+int umt_ctx_grey_sweep(umt_ctx* x, const double* P, const double* gsrc, double* out) {
+  if (!x || !out) return fail(1, "null argument");
+  if (!x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
+  if (int rc = gta_prepare(x)) return rc;
+  GtaState& g = x->gta;
+  const size_t GC = (size_t)x->G * x->C;
+  int rc;
+  if ((rc = grey_upload(x, g.P1, P)) || (rc = grey_upload(x, g.gsrc, gsrc))) return rc;
+  if ((rc = grey_sweep(x, P ? g.P1 : nullptr, gsrc != nullptr, g.As))) return rc;
+  HIP_TRY(hipMemcpyAsync(out, g.As, GC * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_grey_solve(umt_ctx* x, const double* gsrc, const double* phiHalf, double epsGrey,
+                       double* F, int64_t* greyIterations) {
+  if (!x || !gsrc || !phiHalf || !F) return fail(1, "null argument");
+  if (!x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
+  if (int rc = gta_prepare(x)) return rc;
+  GtaState& g = x->gta;
+  const size_t GC = (size_t)x->G * x->C;
+  int rc;
+  if ((rc = grey_upload(x, g.gsrc, gsrc)) || (rc = grey_upload(x, g.phiHalf, phiHalf))) return rc;
+  const long long before = g.greyIterations;
+  if ((rc = gta_solve(x, g.phiHalf, epsGrey))) return rc;
+  HIP_TRY(hipMemcpyAsync(F, g.F, GC * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  if (greyIterations) *greyIterations = g.greyIterations - before;
+  return 0;
+}
 
 int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, double* residual,
                              double* hist) {

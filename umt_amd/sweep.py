@@ -272,6 +272,23 @@ class SweepContext:
         """Replay each sweep's level launches from a captured hipGraph."""
         check("umt_ctx_set_graphs", _lib.load().umt_ctx_set_graphs(self._h, int(bool(on))))
 
+    def grey_sweep(self, P=None, gsrc=None):
+        """The grey operator M(P) (+ the response to gsrc), arrays (C, G)."""
+        a = _f64(P) if P is not None else None
+        b = _f64(gsrc) if gsrc is not None else None
+        out = np.zeros((self.problem.ncorner, self.G))
+        check("umt_ctx_grey_sweep", _lib.load().umt_ctx_grey_sweep(self._h, _dp(a), _dp(b), _dp(out)))
+        return out
+
+    def grey_solve(self, gsrc, phi_half, eps_grey):
+        """(F, BiCGStab iterations) of one grey solve."""
+        a, b = _f64(gsrc), _f64(phi_half)
+        F = np.zeros((self.problem.ncorner, self.G))
+        n = C.c_int64(0)
+        check("umt_ctx_grey_solve", _lib.load().umt_ctx_grey_solve(
+            self._h, _dp(a), _dp(b), eps_grey, _dp(F), C.byref(n)))
+        return F, n.value
+
     def set_angles_per_block(self, n):
         """Sweep angles in blocks of n (psi memory for n angles)."""
         check("umt_ctx_set_angles_per_block", _lib.load().umt_ctx_set_angles_per_block(self._h, n))
