@@ -135,11 +135,12 @@ int umt_ctx_set_source(umt_ctx *ctx, const double *STotal);
 int umt_ctx_set_scattering(umt_ctx *ctx, const double *sigs, const double *q);
 
 /* One sweep of all angles with the current source: psi for every angle and
- * phi = sum_a w_a psi. */
+ * phi = sum_a w_a psi. Boundaries are vacuum (incoming psi_B = 0,
+ * control/setBoundarySources.F90:76). */
 int umt_ctx_sweep(umt_ctx *ctx);
 int umt_ctx_get_phi(const umt_ctx *ctx, double *phi);   /* (G,C) */
-/* psi(G, C+nbelem, A): corner fluxes, then the boundary tail (incoming values
- * for incoming boundary elements, exiting fluxes for exiting ones). */
+/* psi(G, C+nbelem, A): corner fluxes, then the boundary tail (exiting fluxes
+ * for exiting boundary elements, zero for incoming ones). */
 int umt_ctx_get_psi(const umt_ctx *ctx, double *psi);
 int umt_ctx_set_phi(umt_ctx *ctx, const double *phi);   /* (G,C) */
 
@@ -152,6 +153,10 @@ int umt_ctx_set_phi(umt_ctx *ctx, const double *phi);   /* (G,C) */
  * maxIters was reached (the state is still valid). */
 int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
                              double *residual, double *residual_history);
+/* window > 0: source iteration also stops (returning 2) once `window`
+ * iterations pass without a new smallest residual, as at the rounding floor
+ * of FP32 sweeps; 0 (default) never stops early. No counterpart in the reference. */
+int umt_ctx_set_stall_window(umt_ctx *ctx, int window);
 
 /* Timing: warmup + reps sweeps of the current source, timed with HIP events
  * on the context's stream. ms_sweep = mean wall time of one full sweep
@@ -166,7 +171,9 @@ int umt_ctx_time_sweeps(umt_ctx *ctx, int warmup, int reps, double *ms_sweep,
  * stretched-TSA sweep (rt/GreySweep.F90), and the correction is added to
  * phi. umt_ctx_enable_gta takes the S2 directions and their schedule
  * (umt_problem_gta gives them; umt_ctx_create_from_problem enables GTA
- * itself). Replaces the GTA path of rt/LinearSolver.F90:87-121, which the
+ * itself). The S2 directions lie in mesh faces: the schedule must treat
+ * corner faces with |Omega.A| <= 1e-12 |A|_1 as parallel, as the S2 sweeps
+ * do (1e-5 in FP32 sweeps), which umt_problem_gta's schedule does. Replaces the GTA path of rt/LinearSolver.F90:87-121, which the
  * mini-app build compiles out. */
 int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const double *weight,
                        const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ);

@@ -21,6 +21,7 @@ from . import oracle as O
 WTISO = 1.0 / (4.0 * np.pi)
 MIN_RATIO = 1.0e-10
 TINY = 1.0e-300
+PAR_TOL = 1.0e-12  # parallel-face tolerance of the S2 schedule and sweeps
 
 
 def grey_opacities(sigt, sigs):
@@ -49,8 +50,11 @@ class GreySolver:
         self.sweeps = 0
 
     def _sweep(self, S, sig):
+        # S2 directions lie in the tile's diagonal faces: those faces are
+        # parallel in the sweep as in the S2 schedule.
         phi, _ = O.sweep_all(self.m, self.g, self.q2, self.s2, np.ascontiguousarray(S),
-                             np.ascontiguousarray(sig), nthreads=self.nthreads)
+                             np.ascontiguousarray(sig), nthreads=self.nthreads,
+                             par_tol=PAR_TOL)
         return phi
 
     def grey_sweep(self, P, gsrc=None):

@@ -338,13 +338,14 @@ done:
 
 /* ------------------------------------------------------------------------ */
 /* snac/SweepUCBxyz.F90:82-318 */
-void oracle_sweep_angle(int nzones, int ncorner, int nbelem, int Groups, const int *numCorner,
-                        const int *cOffSet, const int *nCFacesArray, const int *cFP,
-                        const int *cEZ, const double *A_fp, const double *A_ez,
-                        const double *Volume, const double *STotal, const double *Sigt,
-                        double tau, const double *Psi, const double *omega, double quadwt,
-                        int nHyperPlanes, const int *zonesInPlane, const int *nextZ,
-                        const int *nextC, double *Psi1, double *PsiB, double *Phi) {
+static void sweep_angle_tol(int nzones, int ncorner, int nbelem, int Groups, const int *numCorner,
+                            const int *cOffSet, const int *nCFacesArray, const int *cFP,
+                            const int *cEZ, const double *A_fp, const double *A_ez,
+                            const double *Volume, const double *STotal, const double *Sigt,
+                            double tau, const double *Psi, const double *omega, double quadwt,
+                            int nHyperPlanes, const int *zonesInPlane, const int *nextZ,
+                            const int *nextC, double parTol, double *Psi1, double *PsiB,
+                            double *Phi) {
   (void)nzones;
   const double fouralpha = 1.82;
   const int G = Groups;
@@ -382,7 +383,7 @@ void oracle_sweep_angle(int nzones, int ncorner, int nbelem, int Groups, const i
         sumArea[c - 1] = 0.0;
         const int nCFaces = nCFacesArray[c0 + c - 1];
         for (int cface = 1; cface <= nCFaces; ++cface) {
-          afp[cface - 1] = dot3(omega, &I3(A_fp, 1, cface, c0 + c, 3, MAXCF));
+          afp[cface - 1] = dot3_tol(omega, &I3(A_fp, 1, cface, c0 + c, 3, MAXCF), parTol);
           const int cfp = I2(cFP, cface, c0 + c, MAXCF);
           if (afp[cface - 1] > 0.0) {
             sumArea[c - 1] += afp[cface - 1];
@@ -399,7 +400,7 @@ void oracle_sweep_angle(int nzones, int ncorner, int nbelem, int Groups, const i
           }
         }
         for (int cface = 1; cface <= nCFaces; ++cface) {
-          const double aez = dot3(omega, &I3(A_ez, 1, cface, c0 + c, 3, MAXCF));
+          const double aez = dot3_tol(omega, &I3(A_ez, 1, cface, c0 + c, 3, MAXCF), parTol);
           const int cez = I2(cEZ, cface, c0 + c, MAXCF);
           if (cez > c) {
             if (aez > 0.0) {
@@ -503,6 +504,18 @@ void oracle_sweep_angle(int nzones, int ncorner, int nbelem, int Groups, const i
   free(psi_opp); free(SigtVol); free(src); free(Q); free(psifp);
 }
 
+void oracle_sweep_angle(int nzones, int ncorner, int nbelem, int Groups, const int *numCorner,
+                        const int *cOffSet, const int *nCFacesArray, const int *cFP,
+                        const int *cEZ, const double *A_fp, const double *A_ez,
+                        const double *Volume, const double *STotal, const double *Sigt,
+                        double tau, const double *Psi, const double *omega, double quadwt,
+                        int nHyperPlanes, const int *zonesInPlane, const int *nextZ,
+                        const int *nextC, double *Psi1, double *PsiB, double *Phi) {
+  sweep_angle_tol(nzones, ncorner, nbelem, Groups, numCorner, cOffSet, nCFacesArray, cFP, cEZ, A_fp,
+                  A_ez, Volume, STotal, Sigt, tau, Psi, omega, quadwt, nHyperPlanes, zonesInPlane,
+                  nextZ, nextC, 0.0, Psi1, PsiB, Phi);
+}
+
 /* ------------------------------------------------------------------------ */
 /* snac/SetSweep.F90:100-170 (single domain, vacuum: one flux iteration) */
 void oracle_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, const int *numCorner,
@@ -512,6 +525,18 @@ void oracle_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, const i
                       const double *weight, const int *nHyperPlanes, const int *zonesInPlane,
                       const int *nextZ, const int *nextC, int nthreads, double *PsiB,
                       double *Phi) {
+  oracle_sweep_all_tol(nzones, ncorner, nbelem, G, A, numCorner, cOffSet, nCFaces, cFP, cEZ, A_fp,
+                       A_ez, Volume, STotal, Sigt, omega, weight, nHyperPlanes, zonesInPlane, nextZ,
+                       nextC, nthreads, 0.0, PsiB, Phi);
+}
+
+void oracle_sweep_all_tol(int nzones, int ncorner, int nbelem, int G, int A, const int *numCorner,
+                          const int *cOffSet, const int *nCFaces, const int *cFP, const int *cEZ,
+                          const double *A_fp, const double *A_ez, const double *Volume,
+                          const double *STotal, const double *Sigt, const double *omega,
+                          const double *weight, const int *nHyperPlanes, const int *zonesInPlane,
+                          const int *nextZ, const int *nextC, int nthreads, double parTol,
+                          double *PsiB, double *Phi) {
   const size_t nGC = (size_t)G * ncorner, nPsi1 = (size_t)G * (ncorner + nbelem);
   int *planeStart = malloc(sizeof(int) * (A + 1));
   planeStart[0] = 0;
@@ -535,10 +560,10 @@ void oracle_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, const i
 #pragma omp for schedule(static)
 #endif
     for (int a = 0; a < A; ++a) {
-      oracle_sweep_angle(nzones, ncorner, nbelem, G, numCorner, cOffSet, nCFaces, cFP, cEZ, A_fp,
-                         A_ez, Volume, STotal, Sigt, 0.0, NULL, &omega[3 * (size_t)a], weight[a],
-                         nHyperPlanes[a], &zonesInPlane[planeStart[a]], &nextZ[(size_t)a * nzones],
-                         &nextC[(size_t)a * ncorner], Psi1, &PsiB[(size_t)a * G * nbelem], phiT);
+      sweep_angle_tol(nzones, ncorner, nbelem, G, numCorner, cOffSet, nCFaces, cFP, cEZ, A_fp,
+                      A_ez, Volume, STotal, Sigt, 0.0, NULL, &omega[3 * (size_t)a], weight[a],
+                      nHyperPlanes[a], &zonesInPlane[planeStart[a]], &nextZ[(size_t)a * nzones],
+                      &nextC[(size_t)a * ncorner], parTol, Psi1, &PsiB[(size_t)a * G * nbelem], phiT);
     }
     free(Psi1);
   }

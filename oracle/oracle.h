@@ -92,6 +92,18 @@ void oracle_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, const i
                       const int *nextZ, const int *nextC, int nthreads, double *PsiB,
                       double *Phi);
 
+/* The same with corner faces whose |Omega.A| <= parTol * |A|_1 taken as
+ * parallel to Omega in the sweep as in the schedule (oracle_schedule_angle_tol):
+ * their Omega.A is exactly 0, so no upwind value is read across them and
+ * they do not select the opposite-face branch of the EZ coupling. */
+void oracle_sweep_all_tol(int nzones, int ncorner, int nbelem, int G, int A, const int *numCorner,
+                          const int *cOffSet, const int *nCFaces, const int *cFP, const int *cEZ,
+                          const double *A_fp, const double *A_ez, const double *Volume,
+                          const double *STotal, const double *Sigt, const double *omega,
+                          const double *weight, const int *nHyperPlanes, const int *zonesInPlane,
+                          const int *nextZ, const int *nextC, int nthreads, double parTol,
+                          double *PsiB, double *Phi);
+
 /* Steady source iteration (rt/rtmainsn.F90:167-235, rt/LinearSolver.F90:55-81,
  * rt/ConvergenceTest.F90:150-168 semantics): STotal = (sigs*phi + q)/(4 pi),
  * sweep all angles, residual = max over (zone,group) with

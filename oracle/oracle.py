@@ -24,6 +24,8 @@ _SIG = {
                                   _pd, _d, _pd, _pd, _d, _i, _pi, _pi, _pi, _pd, _pd, _pd]),
     "oracle_sweep_all": (None, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
                                 _pd, _pd, _pd, _pi, _pi, _pi, _pi, _i, _pd, _pd]),
+    "oracle_sweep_all_tol": (None, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd,
+                                    _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, _i, _d, _pd, _pd]),
     "oracle_source_iteration": (_i, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd,
                                      _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, _i, _d, _i,
                                      _pd, _pd, _pd]),
@@ -132,8 +134,11 @@ def sweep_angle(mesh, geom, STotal, Sigt, omega, quadwt, nextZ, nextC, zonesInPl
         _ip(np.ascontiguousarray(nextC, np.int32)), _dp(Psi1), _dp(PsiB), _dp(Phi))
 
 
-def sweep_all(mesh, geom, quad, sched, STotal, Sigt, nthreads=1, PsiB=None, fast=False):
-    """All angles (SetSweep semantics). Returns (Phi (C,G), PsiB (A,nb,G))."""
+def sweep_all(mesh, geom, quad, sched, STotal, Sigt, nthreads=1, PsiB=None, fast=False,
+              par_tol=0.0):
+    """All angles (SetSweep semantics). Returns (Phi (C,G), PsiB (A,nb,G)).
+    par_tol > 0: corner faces within par_tol of parallel to Omega count as
+    parallel in the sweep (use with a schedule built with the same par_tol)."""
     Z = mesh["numCorner"].shape[0]
     Cn = mesh["nCFaces"].shape[0]
     G = STotal.shape[1]
@@ -142,13 +147,13 @@ def sweep_all(mesh, geom, quad, sched, STotal, Sigt, nthreads=1, PsiB=None, fast
     if PsiB is None:
         PsiB = np.zeros((A, nb, G))
     Phi = np.zeros((Cn, G))
-    load(fast).oracle_sweep_all(
+    load(fast).oracle_sweep_all_tol(
         Z, Cn, nb, G, A, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["nCFaces"]),
         _ip(mesh["cFP"]), _ip(mesh["cEZ"]), _dp(geom["A_fp"]), _dp(geom["A_ez"]),
         _dp(geom["Volume"]), _dp(np.ascontiguousarray(STotal)), _dp(np.ascontiguousarray(Sigt)),
         _dp(quad["omega"]), _dp(quad["weight"]), _ip(sched["nHyperPlanes"]),
-        _ip(sched["zonesInPlane"]), _ip(sched["nextZ"]), _ip(sched["nextC"]), nthreads, _dp(PsiB),
-        _dp(Phi))
+        _ip(sched["zonesInPlane"]), _ip(sched["nextZ"]), _ip(sched["nextC"]), nthreads, par_tol,
+        _dp(PsiB), _dp(Phi))
     return Phi, PsiB
 
 

@@ -233,14 +233,13 @@ def test_learned_iteration_matches_oracle(cfg4, trained):
     ctx.close()
 
 
-@pytest.mark.parametrize("precision", [64, 32])
-def test_trained_network_cuts_iterations(cfg4, trained, precision):
+def test_trained_network_cuts_iterations(cfg4, trained):
     """On held-out cross sections (config 4's seed) the trained correction
     reaches the 1e-8 residual in far fewer sweeps than plain source
     iteration, at the same fixed point."""
     p = cfg4[0]
     net, _ = trained
-    ctx, _ = context(p, 2, SEED4, precision)
+    ctx, _ = context(p, 2, SEED4, 64)
     conv0, it0, _, _ = ctx.source_iteration(tol=1e-8, max_iters=1000)
     plain = ctx.phi()
     ctx.set_phi(np.zeros_like(plain))
@@ -248,10 +247,34 @@ def test_trained_network_cuts_iterations(cfg4, trained, precision):
     ctx.set_acceleration("learned")
     conv1, it1, _, _ = ctx.source_iteration(tol=1e-8, max_iters=1000)
     corr, fell = ctx.learn_stats()
-    say(f"FP{precision}: plain {it0} iterations, learned {it1} ({corr} corrections, fell back at {fell})")
+    say(f"FP64: plain {it0} iterations, learned {it1} ({corr} corrections, fell back at {fell})")
     assert conv0 and conv1
     assert it1 <= 0.7 * it0
     V, VZ = p.geometry()["Volume"], p.geometry()["VolumeZone"]
     a, b = OL.zone_average(ctx.phi(), V, VZ), OL.zone_average(plain, V, VZ)
     assert np.max(np.abs(a - b) / b) < 1e-6
     ctx.close()
+
+
+def test_fp32_learned_then_fp64_matches_plain_fp64(cfg4, trained):
+    """FP32 sweeps with the learned correction to 1e-6, FP64 sweeps with it to
+    1e-8: fewer sweeps than plain FP64 and phi within 1e-6 of its fixed point."""
+    p = cfg4[0]
+    net, _ = trained
+    ref_ctx, _ = context(p, 2, SEED4, 64)
+    conv0, it0, _, _ = ref_ctx.source_iteration(tol=1e-8, max_iters=1000)
+    plain = ref_ctx.phi()
+    fast, _ = context(p, 2, SEED4, 32)
+    exact, _ = context(p, 2, SEED4, 64)
+    for c in (fast, exact):
+        c.set_learner(net, every=2)
+        c.set_acceleration("learned")
+    ok, it32, it64, res, _ = U.solve_mixed(fast, exact, tol=1e-8, switch_tol=1e-6)
+    say(f"plain FP64 {it0} iterations; FP32+learned {it32} + FP64+learned {it64}")
+    assert conv0 and ok
+    assert it32 + it64 < it0
+    V, VZ = p.geometry()["Volume"], p.geometry()["VolumeZone"]
+    a, b = OL.zone_average(fast.phi(), V, VZ), OL.zone_average(plain, V, VZ)
+    assert np.max(np.abs(a - b) / b) < 1e-6
+    for c in (ref_ctx, fast, exact):
+        c.close()

@@ -21,6 +21,7 @@ struct SweepLevelArgs {
   Real* psi;                // (G, C + nbelem, A) all angles, or one angle
   long psiAngleStride;      // (C + nbelem) * G
   int angle0;               // first angle held in psi (angle blocks)
+  Real parTol;              // corner faces with |Omega.A| <= parTol |A|_1 are parallel (0: exact)
   // Single-angle (drop-in) extras.
   double* phi;              // (G, C), phi += quadwt * psi
   double quadwt;

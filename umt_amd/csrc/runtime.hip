@@ -92,6 +92,9 @@ struct umt_problem {
 // of the tile (|Omega.A| is rounding noise there); such corner faces are
 // treated as parallel below this fraction of the face area.
 static const double kGtaParallelTol = 1e-12;
+// The FP32 sweep's Omega.A carries ~1e-7 |A|_1 of rounding (FP32 directions
+// and areas), so its S2 sweeps treat faces below 1e-5 |A|_1 as parallel.
+static const double kGtaParallelTolFP32 = 1e-5;
 
 // One quadrature's sweep: directions, device-resident hyperplane batches
 // (level h = plane h of every angle) and psi for all its angles.
@@ -101,6 +104,7 @@ static const double kGtaParallelTol = 1e-12;
 struct SweepSet {
   int A = 0;
   int anglesPerBlock = 0;
+  double parTol = 0.0;       // parallel-face tolerance of schedule and sweep
   double* omega = nullptr;   // (3, A)
   double* weight = nullptr;  // (A)
   int2* items = nullptr;     // (zone, angle), block by block, level by level
@@ -216,6 +220,7 @@ struct umt_ctx {
   bool haveSigt = false;
   int groupsPerLane = 1;  // K of the sweep kernels
   bool useGraphs = false; // replay captured level launches
+  int stallWindow = 0;    // stop source iteration after this many iterations without a new best residual
   int acceleration = 0;   // 0 plain source iteration, 1 GTA, 2 learned correction
   std::vector<double> sigtHost, sigsHost;  // last uploaded cross sections (GTA opacities)
   double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
@@ -248,6 +253,7 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
   a.psi = (Real*)set.psi;
   a.psiAngleStride = x->psiStride();
   a.angle0 = angle0;
+  a.parTol = (Real)set.parTol;
   return a;
 }
 
@@ -946,6 +952,7 @@ int umt_ctx_enable_gta(umt_ctx* x, int nangles, const double* omega, const doubl
     free_gta(x->gta);
   }
   if (int rc = upload_set(x, x->s2, nangles, omega, weight, nHyperPlanes, zonesInPlane, nextZ)) return rc;
+  x->s2.parTol = x->precision == 32 ? kGtaParallelTolFP32 : kGtaParallelTol;
   GtaState& g = x->gta;
   const size_t GC = (size_t)x->G * x->C, GZ = (size_t)x->G * x->Z, rs = x->rsize();
   g.dotBlocks = 512;
@@ -1289,6 +1296,9 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
   double lastCorrectedRes = -1.0;
   L.fellBack = 0;
   double epsGrey = x->gta.epsGreyInit;
+  double bestRes = HUGE_VAL;
+  int bestIt = 0;
+  bool stalled = false;
   while (it < maxIters) {
     if (gta || track)
       HIP_TRY(hipMemcpyAsync(x->phiPrev, x->phi, nGC * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
@@ -1326,6 +1336,13 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
     ++it;
     epsGrey = res / 20.0;  // rt/rtmainsn.F90:220
     if (res < tol) break;
+    if (res < bestRes) {
+      bestRes = res;
+      bestIt = it;
+    } else if (x->stallWindow > 0 && it - bestIt >= x->stallWindow) {
+      stalled = true;  // e.g. at the rounding floor of FP32 sweeps
+      break;
+    }
     if (track) {
       // Plain source iteration judges every correction: once a corrected
       // state's sweep changes phi more than the sweep the correction
@@ -1358,7 +1375,14 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
   }
   if (iters) *iters = it;
   if (residual) *residual = res;
-  return res < tol ? 0 : 1;
+  return res < tol ? 0 : (stalled ? 2 : 1);
+}
+
+int umt_ctx_set_stall_window(umt_ctx* x, int window) {
+  if (!x) return fail(1, "null context");
+  if (window < 0) return fail(1, "stall window must be >= 0");
+  x->stallWindow = window;
+  return 0;
 }
 
 int umt_ctx_time_sweeps(umt_ctx* x, int warmup, int reps, double* msSweep, double* msLevels,

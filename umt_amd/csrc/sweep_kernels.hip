@@ -144,6 +144,18 @@ __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
     aez[f] = dot3(ox, oy, oz, p.Aez + 3 * (3 * cc + f));
     cfp[f] = p.cFP[3 * cc + f] - 1;
   }
+  if (p.parTol > 0) {
+    // Directions lying in a mesh face leave rounding noise of either sign in
+    // Omega.A; such faces are parallel here as in the schedule, so nothing is
+    // read across them and they cannot select the opposite-face EZ branch.
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const Real* af = p.Afp + 3 * (3 * cc + f);
+      const Real* ae = p.Aez + 3 * (3 * cc + f);
+      if (fabs(afp[f]) <= p.parTol * (fabs(af[0]) + fabs(af[1]) + fabs(af[2]))) afp[f] = 0;
+      if (fabs(aez[f]) <= p.parTol * (fabs(ae[0]) + fabs(ae[1]) + fabs(ae[2]))) aez[f] = 0;
+    }
+  }
   Real outArea = 0;
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
@@ -157,11 +169,14 @@ __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
 #pragma unroll
     for (int k = 0; k < K; ++k) Q.v[k] += p.tau * tp.v[k];
   }
-  // Incoming FP faces (SweepUCBxyz.F90:139-161).
+  // Incoming FP faces (SweepUCBxyz.F90:139-161). Sweeps of all angles are
+  // vacuum-bounded: an incoming boundary face brings zero, and the psi
+  // buffer's boundary tail (which angle blocks reuse) is only written. The
+  // single-angle drop-in reads incoming boundary fluxes from the tail.
   GroupVec<Real, K> psf[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    if (afp[f] < 0) {
+    if (afp[f] < 0 && (kSingleAngle || cfp[f] < p.C)) {
       psf[f] = load_groups<Real, K>(psiA + (long)cfp[f] * G + g0);
     } else {
 #pragma unroll

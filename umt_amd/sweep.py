@@ -317,9 +317,13 @@ class SweepContext:
         hist = np.zeros(max(max_iters, 1))
         rc = _lib.load().umt_ctx_source_iteration(self._h, tol, max_iters, C.byref(it),
                                                   C.byref(res), _dp(hist))
-        if rc not in (0, 1):
+        if rc not in (0, 1, 2):
             check("umt_ctx_source_iteration", rc)
         return rc == 0, it.value, res.value, hist[:it.value].copy()
+
+    def set_stall_window(self, window):
+        """Stop source iteration after `window` iterations without a new best residual."""
+        check("umt_ctx_set_stall_window", _lib.load().umt_ctx_set_stall_window(self._h, window))
 
     def time_sweeps(self, warmup=2, reps=10):
         """Returns (ms per sweep, ms of level kernels per sweep, level launches per sweep)."""
@@ -382,3 +386,18 @@ def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, P
     sid = C.c_int(stream_id)
     lib.gpu_streamsynchronize(C.byref(sid))
     return lib.gpu_sweepucbxyz_status()
+
+
+def solve_mixed(fast, exact, tol=1e-8, switch_tol=1e-6, max_iters=1000, stall_window=8):
+    """Source iteration with FP32 sweeps (context `fast`) until the residual
+    falls below switch_tol (or stops improving at the FP32 rounding floor),
+    then FP64 sweeps (context `exact`, same problem and cross sections) from
+    that phi to tol; the FP64 residual judges convergence. Both contexts keep
+    their own acceleration. fast's phi is set to the result. Returns
+    (converged, FP32 iterations, FP64 iterations, residual, history)."""
+    fast.set_stall_window(stall_window)
+    _, it1, _, h1 = fast.source_iteration(switch_tol, max_iters)
+    exact.copy_phi_from(fast)
+    ok, it2, res, h2 = exact.source_iteration(tol, max(1, max_iters - it1))
+    fast.copy_phi_from(exact)
+    return ok, it1, it2, res, np.concatenate([h1, h2])
