@@ -10,8 +10,10 @@ corners x angles x groups = 3,145,728,000 (driver/test_driver.cc:1895).
 
 Prints one JSON line. ``value`` is FP64 sweep unknowns/s with every input
 resident in HBM. Also reported: the FP32 sweep (FP64 phi), the wall time of
-source iteration to residual < 1e-8 for both, the sweep-level kernel against
-the HBM roofline, and the oracle's CPU sweep timed on this host's cores.
+source iteration to residual < 1e-8 (plain FP64 against grey-transport
+acceleration and the learned correction, FP64 and FP32 then FP64), the
+sweep-level kernel against the HBM roofline, and the oracle's CPU sweep
+timed on this host's cores.
 """
 import argparse
 import json
@@ -117,20 +119,121 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
                 autotune_ms_per_sweep={str(k): v for k, v in tune_ms.items()})
 
 
-def converge(p, G, precision, st, ss, qq, tol, max_iters):
+def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True):
+    """Source iteration to `tol` from phi = 0: FP64 sweeps (mixed False) or
+    FP32 sweeps to 1e-6 then FP64 sweeps (mixed True), with acceleration
+    accel ("none", "gta" or "learned"), graph replay on. The timed solve is
+    the second of two identical ones (the first captures the graphs)."""
     import umt_amd as U
 
-    ctx = U.SweepContext(p, G, precision=precision)
-    ctx.set_sigt(st)
-    ctx.set_scattering(ss, qq)
-    best_k, _ = ctx.autotune(1)
+    st, ss, qq = xs
+
+    def context(precision):
+        ctx = U.SweepContext(p, G, precision=precision)
+        ctx.set_sigt(st)
+        ctx.set_scattering(ss, qq)
+        ctx.autotune(1)
+        ctx.set_graphs(True)
+        if accel == "learned":
+            ctx.set_learner(net, every=2)
+        ctx.set_acceleration(accel)
+        return ctx
+
+    exact = context(64)
+    fast = context(32) if mixed else None
+    zero = np.zeros((p.ncorner, G))
+    out = None
+    for attempt in range(2 if repeat else 1):
+        for c in (exact, fast):
+            if c is not None:
+                c.set_phi(zero)
+        t0 = time.perf_counter()
+        if mixed:
+            ok, it32, it64, res, _ = U.solve_mixed(fast, exact, tol=tol, switch_tol=1e-6,
+                                                   max_iters=max_iters)
+        else:
+            ok, it64, res, _ = exact.source_iteration(tol=tol, max_iters=max_iters)
+            it32 = 0
+        dt = time.perf_counter() - t0
+        out = dict(converged=ok, iterations_fp32=it32, iterations_fp64=it64, residual=res,
+                   seconds=dt)
+    if accel == "gta":
+        gi, gs = exact.gta_stats()
+        out["grey_iterations_fp64_total"] = gi
+    phi = exact.phi()
+    for c in (exact, fast):
+        if c is not None:
+            c.close()
+    return out, phi
+
+
+def rel_error(phi, ref):
+    return float(np.max(np.abs(phi - ref) / np.maximum(np.abs(ref), 1e-300)))
+
+
+def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log):
+    """Plain FP64 source iteration (the reference method on the GPU) against
+    GTA and the learned correction, FP64 and FP32->FP64."""
+    import umt_amd as U
+
+    res = {}
+    base, phi64 = solve(p, G, xs, "none", False, tol, max_iters)
+    res["fp64_plain"] = base
+    log(f"fp64 plain: {base}")
+    cases = [("fp64_gta", "gta", False), ("fp32_then_fp64_gta", "gta", True)]
+    net = None
+    if learn_tiles:
+        t0 = time.perf_counter()
+        small = U.TiledProblem(learn_tiles, learn_tiles, learn_tiles, p.npolar, p.nazimuthal)
+        net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
+                                        c=(0.0, 0.95), q=(0.0, 1.0), rounds=3, batch=8192,
+                                        steps=1500, collect_stride=4, max_iters=max_iters)
+        res["learned_training"] = {
+            "seconds": time.perf_counter() - t0,
+            "problem": f"{learn_tiles}^3 tiles, {p.nangles} angles, {G} groups, seeds 1-3 "
+                       f"(cross sections drawn like the workload's)",
+            "losses": info["losses"], "samples": info["samples"]}
+        log(f"training: {res['learned_training']}")
+        cases += [("fp64_learned", "learned", False), ("fp32_then_fp64_learned", "learned", True)]
+    for name, accel, mixed in cases:
+        r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, net=net)
+        r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
+        r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
+        res[name] = r
+        log(f"{name}: {r}")
+    if net is not None:
+        net.close()
+    return res
+
+
+def learned_config4(tol, log):
+    """The learned correction on its own workload: 4x4x4 tiles, 2x2 product
+    quadrature (32 angles), 2 groups, sigma_t logU[0.1, 10], c U[0.95, 0.999],
+    q = 1, seed 20261206; trained on seeds 1-4 of the same distribution."""
+    import umt_amd as U
+
+    cfg = dict(sigt=(0.1, 10.0), c=(0.95, 0.999), q=(1.0, 1.0))
+    p = U.TiledProblem(4, 4, 4, 2, 2)
+    G = 2
     t0 = time.perf_counter()
-    ok, it, res, _ = ctx.source_iteration(tol=tol, max_iters=max_iters)
-    dt = time.perf_counter() - t0
-    phi = ctx.phi()
-    ctx.close()
-    return dict(converged=ok, iterations=it, residual=res, seconds=dt,
-                groups_per_lane=best_k), phi
+    net, info = U.train_on_iterates(p, G, seeds=[1, 2, 3, 4], rounds=4, epochs=40, **cfg)
+    train_s = time.perf_counter() - t0
+    xs = U.draw_cross_sections(20261206, p.nzones, G, **cfg)
+    out = {"workload": "4x4x4 tiles (1536 zones), 32 angles, 2 groups, sigma_t logU[0.1,10], "
+                       "c U[0.95,0.999], q=1, seed 20261206",
+           "training_seconds": train_s, "training_losses": info["losses"]}
+    base, phi64 = solve(p, G, xs, "none", False, tol, 2000)
+    out["fp64_plain"] = base
+    for name, accel, mixed in (("fp64_learned", "learned", False),
+                               ("fp32_then_fp64_learned", "learned", True),
+                               ("fp64_gta", "gta", False)):
+        r, phi = solve(p, G, xs, accel, mixed, tol, 2000, net=net)
+        r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
+        r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
+        out[name] = r
+    log(f"learned config 4: {out}")
+    net.close()
+    return out
 
 
 def main():
@@ -147,6 +250,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iters", type=int, default=3000)
+    ap.add_argument("--learn-tiles", type=int, default=4,
+                    help="train the learned correction on n^3-tile problems (0: skip it)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the HBM copy probe (for rocprofv3 --pmc calibration)")
     args = ap.parse_args()
@@ -158,6 +263,9 @@ def main():
 
     import umt_amd as U
     from umt_amd import _lib
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     lib = _lib.load()
     if args.calibrate:
@@ -177,8 +285,10 @@ def main():
     S = np.repeat(qq, 8, axis=0) / (4.0 * np.pi)
     unknowns = p.unknowns(G)
 
+    log(f"problem built in {setup_s:.1f} s: {p.nzones} zones, {p.nangles} angles, {G} groups")
     r64 = measure(p, G, 64, st, S, args.warmup, args.steps, lib)
     value = unknowns * args.steps / r64["dt"]
+    log(f"FP64 sweep: {value:.3e} unknowns/s, {r64['dt'] * 1e3 / args.steps:.2f} ms per sweep")
     B64 = bytes_per_unknown(64, r64["n_in"], G)
     ach64 = B64 * unknowns / (r64["ms_levels"] * 1e-3) / 1e9
     tr = load_traffic("ucb_sweep_level_fp64_20x20x20_traffic.json")
@@ -222,6 +332,7 @@ def main():
 
     if not args.no_fp32:
         r32 = measure(p, G, 32, st, S, args.warmup, args.steps, lib)
+        log(f"FP32 sweep: {unknowns * args.steps / r32['dt']:.3e} unknowns/s")
         B32 = bytes_per_unknown(32, r32["n_in"], G)
         ach32 = B32 * unknowns / (r32["ms_levels"] * 1e-3) / 1e9
         out["fp32_sweep"] = {
@@ -235,14 +346,11 @@ def main():
         }
 
     if not args.no_converge:
-        c64, phi64 = converge(p, G, 64, st, ss, qq, args.tol, args.max_iters)
-        out["time_to_converge_fp64"] = c64
-        if not args.no_fp32:
-            c32, phi32 = converge(p, G, 32, st, ss, qq, args.tol, args.max_iters)
-            rel = np.abs(phi32 - phi64) / np.maximum(np.abs(phi64), 1e-300)
-            c32["max_rel_phi_error_vs_fp64"] = float(rel.max())
-            c32["speedup_vs_fp64"] = c64["seconds"] / c32["seconds"]
-            out["time_to_converge_fp32"] = c32
+        out["time_to_converge"] = time_to_converge(
+            p, G, (st, ss, qq), args.tol, args.max_iters, args.learn_tiles, log)
+
+        if args.learn_tiles:
+            out["learned_config4"] = learned_config4(args.tol, log)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, G, st, S, args.cpu_seconds)

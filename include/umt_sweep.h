@@ -291,8 +291,9 @@ int umt_ctx_set_zone_neighbors(umt_ctx *ctx, const int *zoneOpp);
 /* Network used by acceleration mode 2 (NULL detaches), corrections every
  * `every` sweeps, guard as described above. */
 int umt_ctx_set_learner(umt_ctx *ctx, umt_learner *net, int every, int guard);
-/* on != 0: source iteration in modes 0 and 2 records its states. */
-int umt_ctx_learn_collect(umt_ctx *ctx, int on);
+/* on != 0: source iteration in modes 0 and 2 records every stride-th state
+ * it could correct. */
+int umt_ctx_learn_collect(umt_ctx *ctx, int on, int stride);
 /* Appends the recorded states whose residual is >= min_residual to net's
  * samples, targets against the current zone averages; drops the records. */
 int umt_ctx_learn_commit(umt_ctx *ctx, umt_learner *net, double min_residual, int *states);

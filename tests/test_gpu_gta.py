@@ -160,6 +160,7 @@ def test_graph_replays_match_eager_sweeps(thick):
         assert rel_err(ctx.phi(), ref * (1.0 + rep)) < 1e-14
     ctx.set_acceleration("gta")
     ctx.set_scattering(ss, qq)
+    ctx.set_phi(np.zeros((p.ncorner, G)))  # the same start as the eager context
     ok, it, res, _ = ctx.source_iteration(tol=1e-10, max_iters=200)
     assert ok
     plain = _ctx(p, G, st, ss, qq)

@@ -84,7 +84,7 @@ SIGNATURES = {
     "umt_learner_train": (_i, [_vp, _i, _i, _d, _u64, _pd]),
     "umt_ctx_set_zone_neighbors": (_i, [_vp, _pi]),
     "umt_ctx_set_learner": (_i, [_vp, _vp, _i, _i]),
-    "umt_ctx_learn_collect": (_i, [_vp, _i]),
+    "umt_ctx_learn_collect": (_i, [_vp, _i, _i]),
     "umt_ctx_learn_commit": (_i, [_vp, _vp, _d, _pi]),
     "umt_ctx_learn_discard": (_i, [_vp]),
     "umt_ctx_learn_features": (_i, [_vp, _pd, _pd, _pd, _pf]),

@@ -175,6 +175,7 @@ struct LearnState {
   int every = 2;       // sweeps between corrections
   bool guard = true;   // stop correcting once a corrected step raises the residual
   bool collect = false;
+  int collectStride = 1;   // record every collectStride-th eligible state
   int* zoneNbr = nullptr;   // (6, Z) 0-based face neighbours, -1 on the boundary
   double* sigt64 = nullptr; // (G, Z) FP64 sigma_t for the features
   double *pp = nullptr, *d = nullptr, *dp = nullptr, *rel = nullptr, *ratio = nullptr;
@@ -1292,7 +1293,7 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
     if (int rc = learn_prepare(x)) return rc;
   LearnState& L = x->learn;
   bool correcting = learned, haveDp = false;
-  int since = 0;
+  int since = 0, eligible = 0;
   double lastCorrectedRes = -1.0;
   L.fellBack = 0;
   double epsGrey = x->gta.epsGreyInit;
@@ -1356,7 +1357,7 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
       }
       ++since;
       if (since >= L.every && haveDp) {
-        if (L.collect)
+        if (L.collect && eligible++ % L.collectStride == 0)
           if (int rc = learn_record(x, res)) return rc;
         if (correcting) {
           HIP_TRY(umt::launch_mlp_forward(L.X, nGZ, L.net->packed, L.net->params, L.r, x->stream));
@@ -1717,9 +1718,11 @@ int umt_ctx_set_learner(umt_ctx* x, umt_learner* L, int every, int guard) {
   return 0;
 }
 
-int umt_ctx_learn_collect(umt_ctx* x, int on) {
+int umt_ctx_learn_collect(umt_ctx* x, int on, int stride) {
   if (!x) return fail(1, "null context");
+  if (stride < 1) return fail(1, "collection stride must be at least 1");
   x->learn.collect = on != 0;
+  x->learn.collectStride = stride;
   return 0;
 }
 

@@ -123,13 +123,17 @@ class Learner:
 
 def train_on_iterates(problem, groups, seeds, sigt=(0.1, 10.0), c=(0.95, 0.999), q=(1.0, 1.0),
                       rounds=4, epochs=40, batch=4096, lr=3e-3, every=2, precision=64,
-                      learner=None, net_seed=0, tol=1e-10, max_iters=2000, log=None):
+                      learner=None, net_seed=0, tol=1e-10, max_iters=2000, log=None,
+                      collect_stride=1, steps=None, graphs=True):
     """Train a Learner on source-iteration iterates of `problem` with cross
     sections drawn from `seeds`. Round 0 records plain iterations; each
     later round records the iterations corrected by the network trained so
-    far. Returns (learner, info) with per-round losses and timings."""
+    far. collect_stride keeps every n-th state; steps (if given) fixes the
+    Adam steps per round instead of the epochs. Returns (learner, info) with
+    per-round losses and timings."""
     net = learner if learner is not None else Learner(net_seed)
     ctx = SweepContext(problem, groups, precision=precision)
+    ctx.set_graphs(graphs)
     info = {"losses": [], "samples": [], "seconds": 0.0}
     t0 = time.perf_counter()
     try:
@@ -145,7 +149,7 @@ def train_on_iterates(problem, groups, seeds, sigt=(0.1, 10.0), c=(0.95, 0.999),
                 else:
                     ctx.set_learner(net, every=every, guard=True)
                     ctx.set_acceleration("learned")
-                ctx.learn_collect(True)
+                ctx.learn_collect(True, collect_stride)
                 conv, it, res, _ = ctx.source_iteration(tol=tol, max_iters=max_iters)
                 ctx.learn_collect(False)
                 if conv:
@@ -154,7 +158,10 @@ def train_on_iterates(problem, groups, seeds, sigt=(0.1, 10.0), c=(0.95, 0.999),
                     ctx.learn_discard()
                 if log:
                     log(f"round {rnd} seed {sd}: {it} iterations, converged={conv}")
-            loss = net.train(epochs=epochs, batch=batch, lr=lr, seed=1000 + rnd)
+            ep = epochs
+            if steps is not None:
+                ep = max(1, -(-steps * batch // max(net.num_samples(), 1)))
+            loss = net.train(epochs=ep, batch=batch, lr=lr, seed=1000 + rnd)
             info["losses"].append(loss)
             info["samples"].append(net.num_samples())
             if log:

@@ -237,9 +237,10 @@ class SweepContext:
         h = learner.handle if learner is not None else None
         check("umt_ctx_set_learner", _lib.load().umt_ctx_set_learner(self._h, h, every, int(bool(guard))))
 
-    def learn_collect(self, on=True):
-        """Record the states source iteration could correct (training data)."""
-        check("umt_ctx_learn_collect", _lib.load().umt_ctx_learn_collect(self._h, int(bool(on))))
+    def learn_collect(self, on=True, stride=1):
+        """Record every stride-th state source iteration could correct (training data)."""
+        check("umt_ctx_learn_collect",
+              _lib.load().umt_ctx_learn_collect(self._h, int(bool(on)), stride))
 
     def learn_commit(self, learner, min_residual=1e-9):
         """Turn the recorded states into samples of learner, with targets
