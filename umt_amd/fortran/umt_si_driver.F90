@@ -6,6 +6,8 @@
 !    mode "si":     hands the arrays to a device-resident context and runs
 !                   steady source iteration on the GPU (rt/rtmainsn.F90
 !                   semantics for tau = 0, within-group scattering);
+!    mode "gta":    the same with grey-transport acceleration, the context
+!                   built from the problem handle (S2 schedule included);
 !    mode "dropin": sweeps one angle through gpu_sweepucbxyz with the
 !                   argument list of src/teton/gpu/SweepUCBxyzToGPU.F90:198-240.
 !  Writes the resulting Phi(G,C) as raw FP64 to the output file and prints a
@@ -40,6 +42,7 @@ program umt_si_driver
   real(c_double), allocatable :: Psi1(:,:), PsiB(:,:), Psi(:,:), hist(:)
   real(c_double)              :: residual, msPerK(3), tau, quadwt
   real(c_double)              :: dummy(1)
+  integer(c_int64_t)          :: greyIts, greySweeps
   integer(c_int)              :: idummy(1)
   integer(c_int)              :: ndim, maxcf, maxCorner, zero, one, minusOne, streamId, nHP
 
@@ -93,6 +96,25 @@ program umt_si_driver
     call umt_ctx_destroy(ctx)
     write(*, '(a,i6,a,es12.5,a,i2,a,es22.15)') "si iterations=", iters, " residual=", residual, &
          " groupsPerLane=", bestK, " sumPhi=", sum(Phi)
+
+  elseif (trim(mode) == "gta") then
+
+    ! Context from the problem (it brings the S2 grey-acceleration schedule),
+    ! GTA-accelerated source iteration with graph replay.
+    rc = umt_ctx_create_from_problem(prob, G, precision, ctx)
+    call check(rc, "umt_ctx_create_from_problem")
+    call check(umt_ctx_set_sigt(ctx, Sigt), "umt_ctx_set_sigt")
+    call check(umt_ctx_set_scattering(ctx, sigs, q), "umt_ctx_set_scattering")
+    call check(umt_ctx_set_acceleration(ctx, 1), "umt_ctx_set_acceleration")
+    call check(umt_ctx_set_graphs(ctx, 1), "umt_ctx_set_graphs")
+    allocate(hist(max(maxIters, 1)))
+    rc = umt_ctx_source_iteration(ctx, tol, maxIters, iters, residual, hist)
+    if (rc /= 0 .and. rc /= 1) call check(rc, "umt_ctx_source_iteration")
+    call check(umt_ctx_gta_stats(ctx, greyIts, greySweeps), "umt_ctx_gta_stats")
+    call check(umt_ctx_get_phi(ctx, Phi), "umt_ctx_get_phi")
+    call umt_ctx_destroy(ctx)
+    write(*, '(a,i6,a,es12.5,a,i8,a,es22.15)') "gta iterations=", iters, " residual=", residual, &
+         " greyIterations=", greyIts, " sumPhi=", sum(Phi)
 
   elseif (trim(mode) == "dropin") then
 

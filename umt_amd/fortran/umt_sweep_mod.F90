@@ -25,7 +25,13 @@ module umt_sweep_mod
             umt_draw_cross_sections, umt_ctx_create, umt_ctx_destroy,       &
             umt_ctx_set_sigt, umt_ctx_set_source, umt_ctx_set_scattering,   &
             umt_ctx_sweep, umt_ctx_get_phi, umt_ctx_source_iteration,       &
-            umt_ctx_autotune, umt_error_message
+            umt_ctx_autotune, umt_error_message, umt_ctx_create_from_problem, &
+            umt_ctx_set_phi, umt_ctx_copy_phi, umt_ctx_set_acceleration,      &
+            umt_ctx_gta_stats, umt_ctx_set_graphs, umt_ctx_set_angles_per_block, &
+            umt_ctx_set_stall_window, umt_ctx_set_zone_neighbors,             &
+            umt_learner_create, umt_learner_destroy, umt_learner_train,       &
+            umt_ctx_set_learner, umt_ctx_learn_collect, umt_ctx_learn_commit, &
+            umt_ctx_learn_stats
 
   interface
 
@@ -200,6 +206,119 @@ module umt_sweep_mod
       integer(c_int) :: bestK
       real(c_double) :: msPerK(3)
     end function umt_ctx_autotune
+
+    integer(c_int) function umt_ctx_create_from_problem(prob, groups, precision, ctx) &
+        bind(c, name="umt_ctx_create_from_problem")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: prob
+      integer(c_int), value :: groups, precision
+      type(c_ptr) :: ctx
+    end function umt_ctx_create_from_problem
+
+    integer(c_int) function umt_ctx_set_phi(ctx, phi) bind(c, name="umt_ctx_set_phi")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: phi(*)
+    end function umt_ctx_set_phi
+
+    integer(c_int) function umt_ctx_copy_phi(dst, src) bind(c, name="umt_ctx_copy_phi")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dst, src
+    end function umt_ctx_copy_phi
+
+!   mode 0: plain source iteration, 1: GTA, 2: learned correction
+    integer(c_int) function umt_ctx_set_acceleration(ctx, mode) &
+        bind(c, name="umt_ctx_set_acceleration")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function umt_ctx_set_acceleration
+
+    integer(c_int) function umt_ctx_gta_stats(ctx, greyIterations, greySweeps) &
+        bind(c, name="umt_ctx_gta_stats")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t) :: greyIterations, greySweeps
+    end function umt_ctx_gta_stats
+
+    integer(c_int) function umt_ctx_set_graphs(ctx, on) bind(c, name="umt_ctx_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function umt_ctx_set_graphs
+
+    integer(c_int) function umt_ctx_set_angles_per_block(ctx, n) &
+        bind(c, name="umt_ctx_set_angles_per_block")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+    end function umt_ctx_set_angles_per_block
+
+    integer(c_int) function umt_ctx_set_stall_window(ctx, window) &
+        bind(c, name="umt_ctx_set_stall_window")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: window
+    end function umt_ctx_set_stall_window
+
+!   zoneOpp(6, nzones) of mods/Geometry_mod.F90 (1-based, <= 0 on the boundary)
+    integer(c_int) function umt_ctx_set_zone_neighbors(ctx, zoneOpp) &
+        bind(c, name="umt_ctx_set_zone_neighbors")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: zoneOpp(*)
+    end function umt_ctx_set_zone_neighbors
+
+    integer(c_int) function umt_learner_create(seed, net) bind(c, name="umt_learner_create")
+      import :: c_int, c_ptr, c_int64_t
+      integer(c_int64_t), value :: seed
+      type(c_ptr) :: net
+    end function umt_learner_create
+
+    subroutine umt_learner_destroy(net) bind(c, name="umt_learner_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: net
+    end subroutine umt_learner_destroy
+
+    integer(c_int) function umt_learner_train(net, epochs, batch, lr, seed, meanLoss) &
+        bind(c, name="umt_learner_train")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: net
+      integer(c_int), value :: epochs, batch
+      real(c_double), value :: lr
+      integer(c_int64_t), value :: seed
+      real(c_double) :: meanLoss
+    end function umt_learner_train
+
+    integer(c_int) function umt_ctx_set_learner(ctx, net, every, guard) &
+        bind(c, name="umt_ctx_set_learner")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, net
+      integer(c_int), value :: every, guard
+    end function umt_ctx_set_learner
+
+    integer(c_int) function umt_ctx_learn_collect(ctx, on, stride) &
+        bind(c, name="umt_ctx_learn_collect")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on, stride
+    end function umt_ctx_learn_collect
+
+    integer(c_int) function umt_ctx_learn_commit(ctx, net, minResidual, states) &
+        bind(c, name="umt_ctx_learn_commit")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, net
+      real(c_double), value :: minResidual
+      integer(c_int) :: states
+    end function umt_ctx_learn_commit
+
+    integer(c_int) function umt_ctx_learn_stats(ctx, corrections, fellBackAt) &
+        bind(c, name="umt_ctx_learn_stats")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t) :: corrections
+      integer(c_int) :: fellBackAt
+    end function umt_ctx_learn_stats
 
     integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
       import :: c_ptr, c_size_t
