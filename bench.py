@@ -134,7 +134,7 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True):
         ctx.set_scattering(ss, qq)
         ctx.autotune(1)
         ctx.set_graphs(True)
-        if accel == "learned":
+        if accel in ("learned", "gta+learned"):
             ctx.set_learner(net, every=2)
         ctx.set_acceleration(accel)
         return ctx
@@ -157,7 +157,7 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True):
         dt = time.perf_counter() - t0
         out = dict(converged=ok, iterations_fp32=it32, iterations_fp64=it64, residual=res,
                    seconds=dt)
-    if accel == "gta":
+    if accel in ("gta", "gta+learned"):
         gi, gs = exact.gta_stats()
         out["grey_iterations_fp64_total"] = gi
     phi = exact.phi()
@@ -180,28 +180,35 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log):
     base, phi64 = solve(p, G, xs, "none", False, tol, max_iters)
     res["fp64_plain"] = base
     log(f"fp64 plain: {base}")
-    cases = [("fp64_gta", "gta", False), ("fp32_then_fp64_gta", "gta", True)]
-    net = None
+    cases = [("fp64_gta", "gta", False, None), ("fp32_then_fp64_gta", "gta", True, None)]
+    nets = []
     if learn_tiles:
-        t0 = time.perf_counter()
         small = U.TiledProblem(learn_tiles, learn_tiles, learn_tiles, p.npolar, p.nazimuthal)
-        net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
-                                        c=(0.0, 0.95), q=(0.0, 1.0), rounds=3, batch=8192,
-                                        steps=1500, collect_stride=4, max_iters=max_iters)
-        res["learned_training"] = {
-            "seconds": time.perf_counter() - t0,
-            "problem": f"{learn_tiles}^3 tiles, {p.nangles} angles, {G} groups, seeds 1-3 "
-                       f"(cross sections drawn like the workload's)",
-            "losses": info["losses"], "samples": info["samples"]}
-        log(f"training: {res['learned_training']}")
-        cases += [("fp64_learned", "learned", False), ("fp32_then_fp64_learned", "learned", True)]
-    for name, accel, mixed in cases:
+        for key, on in (("plain", "none"), ("gta", "gta")):
+            t0 = time.perf_counter()
+            net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
+                                            c=(0.0, 0.95), q=(0.0, 1.0), rounds=3, batch=8192,
+                                            steps=1500, collect_stride=4 if on == "none" else 1,
+                                            max_iters=max_iters, base=on)
+            res[f"learned_training_{key}"] = {
+                "seconds": time.perf_counter() - t0,
+                "problem": f"{learn_tiles}^3 tiles, {p.nangles} angles, {G} groups, seeds 1-3 "
+                           f"(cross sections drawn like the workload's), iterates of "
+                           f"{'plain' if on == 'none' else 'GTA-accelerated'} source iteration",
+                "losses": info["losses"], "samples": info["samples"]}
+            log(f"training on {key} iterates: {res[f'learned_training_{key}']}")
+            nets.append(net)
+        cases += [("fp64_learned", "learned", False, nets[0]),
+                  ("fp32_then_fp64_learned", "learned", True, nets[0]),
+                  ("fp64_gta_learned", "gta+learned", False, nets[1]),
+                  ("fp32_then_fp64_gta_learned", "gta+learned", True, nets[1])]
+    for name, accel, mixed, net in cases:
         r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, net=net)
         r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
         r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
         res[name] = r
         log(f"{name}: {r}")
-    if net is not None:
+    for net in nets:
         net.close()
     return res
 

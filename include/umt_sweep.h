@@ -178,7 +178,8 @@ int umt_ctx_time_sweeps(umt_ctx *ctx, int warmup, int reps, double *ms_sweep,
 int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const double *weight,
                        const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ);
 /* mode 0: plain source iteration (default); 1: GTA-accelerated; 2: learned
- * correction (umt_ctx_set_learner). */
+ * correction (umt_ctx_set_learner); 3: both, the learned correction applied to
+ * the GTA-corrected iterates (GTA is then the fallback and the judge). */
 int umt_ctx_set_acceleration(umt_ctx *ctx, int mode);
 /* Cumulative BiCGStab iterations and S2 sweeps done by GTA on this context. */
 int umt_ctx_gta_stats(const umt_ctx *ctx, int64_t *greyIterations, int64_t *greySweeps);
@@ -291,8 +292,8 @@ int umt_ctx_set_zone_neighbors(umt_ctx *ctx, const int *zoneOpp);
 /* Network used by acceleration mode 2 (NULL detaches), corrections every
  * `every` sweeps, guard as described above. */
 int umt_ctx_set_learner(umt_ctx *ctx, umt_learner *net, int every, int guard);
-/* on != 0: source iteration in modes 0 and 2 records every stride-th state
- * it could correct. */
+/* on != 0: source iteration records every stride-th state it could correct
+ * (in modes 1 and 3 the GTA-corrected states). */
 int umt_ctx_learn_collect(umt_ctx *ctx, int on, int stride);
 /* Appends the recorded states whose residual is >= min_residual to net's
  * samples, targets against the current zone averages; drops the records. */

@@ -278,3 +278,31 @@ def test_fp32_learned_then_fp64_matches_plain_fp64(cfg4, trained):
     assert np.max(np.abs(a - b) / b) < 1e-6
     for c in (ref_ctx, fast, exact):
         c.close()
+
+
+def test_learned_correction_on_top_of_gta(cfg4):
+    """Mode 3: the network trained on GTA-accelerated iterates corrects them
+    further; GTA stays the fallback and the judge, the fixed point is plain
+    iteration's, and it takes no more outer iterations than GTA alone."""
+    p = cfg4[0]
+    net, info = U.train_on_iterates(p, 2, seeds=[1, 2, 3], rounds=3, epochs=40, base="gta",
+                                    **CONFIG4)
+    ctx, _ = context(p, 2, SEED4, 64)
+    ok0, it0, _, _ = ctx.source_iteration(tol=1e-10, max_iters=2000)
+    plain = ctx.phi()
+    ctx.set_phi(np.zeros_like(plain))
+    ctx.set_acceleration("gta")
+    ok1, it1, _, _ = ctx.source_iteration(tol=1e-8, max_iters=200)
+    ctx.set_phi(np.zeros_like(plain))
+    ctx.set_learner(net, every=2)
+    ctx.set_acceleration("gta+learned")
+    ok2, it2, _, _ = ctx.source_iteration(tol=1e-8, max_iters=200)
+    corr, fell = ctx.learn_stats()
+    say(f"plain {it0} (to 1e-10), GTA {it1}, GTA+learned {it2} ({corr} corrections, fell back at {fell})")
+    assert ok0 and ok1 and ok2
+    assert it2 <= it1
+    V, VZ = p.geometry()["Volume"], p.geometry()["VolumeZone"]
+    a, b = OL.zone_average(ctx.phi(), V, VZ), OL.zone_average(plain, V, VZ)
+    assert np.max(np.abs(a - b) / b) < 1e-6
+    ctx.close()
+    net.close()

@@ -222,7 +222,7 @@ struct umt_ctx {
   int groupsPerLane = 1;  // K of the sweep kernels
   bool useGraphs = false; // replay captured level launches
   int stallWindow = 0;    // stop source iteration after this many iterations without a new best residual
-  int acceleration = 0;   // 0 plain source iteration, 1 GTA, 2 learned correction
+  int acceleration = 0;   // 0 plain source iteration, 1 GTA, 2 learned correction, 3 both
   std::vector<double> sigtHost, sigsHost;  // last uploaded cross sections (GTA opacities)
   double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
   GtaState gta;
@@ -974,10 +974,11 @@ int umt_ctx_enable_gta(umt_ctx* x, int nangles, const double* omega, const doubl
 
 int umt_ctx_set_acceleration(umt_ctx* x, int mode) {
   if (!x) return fail(1, "null context");
-  if (mode < 0 || mode > 2) return fail(36, "acceleration must be 0 (none), 1 (GTA) or 2 (learned)");
-  if (mode == 1 && !x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
-  if (mode == 2 && !x->learn.net) return fail(38, "learned acceleration needs umt_ctx_set_learner first");
-  if (mode == 2 && !x->learn.zoneNbr) return fail(39, "learned acceleration needs umt_ctx_set_zone_neighbors first");
+  if (mode < 0 || mode > 3)
+    return fail(36, "acceleration must be 0 (none), 1 (GTA), 2 (learned) or 3 (GTA + learned)");
+  if ((mode == 1 || mode == 3) && !x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
+  if (mode >= 2 && !x->learn.net) return fail(38, "learned acceleration needs umt_ctx_set_learner first");
+  if (mode >= 2 && !x->learn.zoneNbr) return fail(39, "learned acceleration needs umt_ctx_set_zone_neighbors first");
   x->acceleration = mode;
   return 0;
 }
@@ -1283,10 +1284,10 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
   // Zone averages of the starting phi are the "old" values of iteration 1.
   HIP_TRY(umt::launch_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld, 1e-6,
                                x->bits, x->stream));
-  const bool gta = x->acceleration == 1;
-  const bool learned = x->acceleration == 2;
-  // Learned-correction bookkeeping (corrections, or recording plain states).
-  const bool track = learned || (x->learn.collect && !gta);
+  const bool gta = x->acceleration == 1 || x->acceleration == 3;
+  const bool learned = x->acceleration == 2 || x->acceleration == 3;
+  // Learned-correction bookkeeping (corrections, or recording states).
+  const bool track = learned || x->learn.collect;
   if (gta)
     if (int rc = gta_prepare(x)) return rc;
   if (track)
@@ -1714,7 +1715,7 @@ int umt_ctx_set_learner(umt_ctx* x, umt_learner* L, int every, int guard) {
   x->learn.net = L;
   x->learn.every = every;
   x->learn.guard = guard != 0;
-  if (!L && x->acceleration == 2) x->acceleration = 0;
+  if (!L && x->acceleration >= 2) x->acceleration = x->acceleration == 3 ? 1 : 0;
   return 0;
 }
 

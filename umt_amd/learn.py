@@ -124,13 +124,14 @@ class Learner:
 def train_on_iterates(problem, groups, seeds, sigt=(0.1, 10.0), c=(0.95, 0.999), q=(1.0, 1.0),
                       rounds=4, epochs=40, batch=4096, lr=3e-3, every=2, precision=64,
                       learner=None, net_seed=0, tol=1e-10, max_iters=2000, log=None,
-                      collect_stride=1, steps=None, graphs=True):
+                      collect_stride=1, steps=None, graphs=True, base="none"):
     """Train a Learner on source-iteration iterates of `problem` with cross
     sections drawn from `seeds`. Round 0 records plain iterations; each
     later round records the iterations corrected by the network trained so
     far. collect_stride keeps every n-th state; steps (if given) fixes the
-    Adam steps per round instead of the epochs. Returns (learner, info) with
-    per-round losses and timings."""
+    Adam steps per round instead of the epochs. base "gta" trains the
+    correction of GTA-accelerated iterates (used as "gta+learned").
+    Returns (learner, info) with per-round losses and timings."""
     net = learner if learner is not None else Learner(net_seed)
     ctx = SweepContext(problem, groups, precision=precision)
     ctx.set_graphs(graphs)
@@ -145,10 +146,10 @@ def train_on_iterates(problem, groups, seeds, sigt=(0.1, 10.0), c=(0.95, 0.999),
                 ctx.set_phi(np.zeros((problem.ncorner, groups)))
                 if rnd == 0:
                     ctx.set_learner(net, every=every, guard=False)
-                    ctx.set_acceleration("none")
+                    ctx.set_acceleration(base)
                 else:
                     ctx.set_learner(net, every=every, guard=True)
-                    ctx.set_acceleration("learned")
+                    ctx.set_acceleration("learned" if base == "none" else base + "+learned")
                 ctx.learn_collect(True, collect_stride)
                 conv, it, res, _ = ctx.source_iteration(tol=tol, max_iters=max_iters)
                 ctx.learn_collect(False)

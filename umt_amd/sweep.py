@@ -225,9 +225,9 @@ class SweepContext:
         return out
 
     def set_acceleration(self, mode):
-        """"none" (plain source iteration), "gta" (grey-transport acceleration)
-        or "learned" (the network given to set_learner)."""
-        m = {"none": 0, "gta": 1, "learned": 2}[mode]
+        """"none" (plain source iteration), "gta" (grey-transport acceleration),
+        "learned" (the network given to set_learner) or "gta+learned"."""
+        m = {"none": 0, "gta": 1, "learned": 2, "gta+learned": 3}[mode]
         check("umt_ctx_set_acceleration", _lib.load().umt_ctx_set_acceleration(self._h, m))
 
     def set_learner(self, learner, every=2, guard=True):
