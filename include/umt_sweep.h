@@ -238,6 +238,56 @@ int umt_hbm_copy_probe(int64_t n, int reps, double *ms);
 int umt_ctx_dims(const umt_ctx *ctx, int64_t *dims);
 
 /* ------------------------------------------------------------------------
+ * Batches of uneven domains. No counterpart in the reference, which sweeps
+ * one spatial domain per MPI rank; here many independent problems (each its
+ * own mesh, quadrature, group count and cross sections) share the GPU.
+ *
+ * Level h of a batched sweep holds level h of every domain (plane h of each
+ * of its angles in the current angle block), so one kernel launch covers all
+ * domains at that depth. A whole source iteration (source update, level
+ * launches and phi per angle block, residual per domain) replays as one
+ * hipGraph. Groups are stored padded to a multiple of 4 (padding groups
+ * carry no flux). Each domain's sweep gives the bits a single-domain context
+ * gives for the same groups per lane.
+ * ---------------------------------------------------------------------- */
+typedef struct umt_batch umt_batch;
+
+/* One domain per problem (umt_problem_tiled), groups[d] groups each,
+ * precision 64 or 32 (FP64 phi and residual either way). */
+int umt_batch_create(int nDomains, const umt_problem *const *problems, const int *groups,
+                     int precision, umt_batch **out);
+void umt_batch_destroy(umt_batch *batch);
+/* dims[0..5] = domains, unknowns per batched sweep (corners x angles x groups
+ * summed), level launches per sweep, groups per lane, angles per block,
+ * angle blocks. */
+int umt_batch_dims(const umt_batch *batch, int64_t *dims);
+/* sigt, sigs, q (groups[d], nzones of domain d). */
+int umt_batch_set_cross_sections(umt_batch *batch, int domain, const double *sigt,
+                                 const double *sigs, const double *q);
+/* phi = 0 in every domain. */
+int umt_batch_reset(umt_batch *batch);
+/* One source-iteration step of every domain without the residual: S from
+ * phi, then phi from one sweep. */
+int umt_batch_sweep(umt_batch *batch);
+int umt_batch_get_phi(const umt_batch *batch, int domain, double *phi); /* (groups[d], C_d) */
+/* Source iteration of all domains together until every domain's residual
+ * (rt/ConvergenceTest.F90:150-168 semantics) is below tol: iters[d] is the
+ * iteration at which domain d first met it, residuals[d] its last residual,
+ * total the iterations run. Returns 0 when all converged, 1 otherwise. */
+int umt_batch_source_iteration(umt_batch *batch, double tol, int maxIters, int *iters,
+                               double *residuals, int *total);
+int umt_batch_set_graphs(umt_batch *batch, int on);
+/* Groups per lane K (1, 2, 4) and angles per block for the batched sweep. */
+int umt_batch_set_layout(umt_batch *batch, int groupsPerLane, int anglesPerBlock);
+/* Mean ms of one source iteration (with residuals) over reps, after warmup. */
+int umt_batch_time_iterations(umt_batch *batch, int warmup, int reps, double *msPerIteration);
+/* Autotuner: times every K in {1, 2, 4} with angles per block in {8, 16, 32,
+ * 64, all} (skipping blocks whose psi would not fit in free HBM), keeps the
+ * fastest and resets phi. */
+int umt_batch_autotune(umt_batch *batch, int reps, int *bestK, int *bestAnglesPerBlock,
+                       double *msPerIteration);
+
+/* ------------------------------------------------------------------------
  * Learned correction of source iteration. No counterpart in the reference:
  * it accelerates the iteration rt/rtmainsn.F90:167-235 drives, next to GTA.
  *

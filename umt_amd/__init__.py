@@ -7,6 +7,7 @@ from ._lib import LIB_PATH, UmtError, load  # noqa: F401
 from .sweep import (SweepContext, TiledProblem, draw_cross_sections, solve_mixed,  # noqa: F401
                     sweep_ucbxyz)
 from .learn import Learner, train_on_iterates  # noqa: F401
+from .batch import SweepBatch  # noqa: F401
 
 __all__ = ["SweepContext", "TiledProblem", "draw_cross_sections", "sweep_ucbxyz", "load",
-           "UmtError", "LIB_PATH", "Learner", "train_on_iterates", "solve_mixed"]
+           "UmtError", "LIB_PATH", "Learner", "train_on_iterates", "solve_mixed", "SweepBatch"]

@@ -69,6 +69,18 @@ SIGNATURES = {
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
     "umt_ctx_grey_sweep": (_i, [_vp, _pd, _pd, _pd]),
     "umt_ctx_grey_solve": (_i, [_vp, _pd, _pd, _d, _pd, _pi64]),
+    "umt_batch_create": (_i, [_i, C.POINTER(_vp), _pi, _i, C.POINTER(_vp)]),
+    "umt_batch_destroy": (None, [_vp]),
+    "umt_batch_dims": (_i, [_vp, _pi64]),
+    "umt_batch_set_cross_sections": (_i, [_vp, _i, _pd, _pd, _pd]),
+    "umt_batch_reset": (_i, [_vp]),
+    "umt_batch_sweep": (_i, [_vp]),
+    "umt_batch_get_phi": (_i, [_vp, _i, _pd]),
+    "umt_batch_source_iteration": (_i, [_vp, _d, _i, _pi, _pd, _pi]),
+    "umt_batch_set_graphs": (_i, [_vp, _i]),
+    "umt_batch_set_layout": (_i, [_vp, _i, _i]),
+    "umt_batch_time_iterations": (_i, [_vp, _i, _i, _pd]),
+    "umt_batch_autotune": (_i, [_vp, _i, _pi, _pi, _pd]),
     "umt_learner_num_params": (_i, []),
     "umt_learner_create": (_i, [_u64, C.POINTER(_vp)]),
     "umt_learner_destroy": (None, [_vp]),

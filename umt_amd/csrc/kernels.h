@@ -31,6 +31,38 @@ struct SweepLevelArgs {
 
 template <typename Real>
 hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s);
+
+// Many domains in one level launch (batches of uneven domains).
+template <typename Real>
+struct BatchDomain {
+  const Real* Afp;          // (3, 3, C) of this domain
+  const Real* Aez;
+  const Real* vol;
+  const int* cFP;           // (3, C), 1-based within the domain
+  const Real* S;            // (G, C)
+  const Real* sigt;         // (G, Z)
+  const double* omega;      // (3, A)
+  Real* psi;                // (G, C + nbelem, anglesPerBlock)
+  long psiAngleStride;      // (C + nbelem) * G
+  int G;                    // groups as stored (a multiple of the groups per lane)
+  int C;
+};
+struct BatchSeg {
+  long taskStart;           // first task (item, K groups) of the segment
+  int domain;
+  int itemStart;            // first item of the segment within the level
+};
+template <typename Real>
+struct BatchLevelArgs {
+  const BatchDomain<Real>* doms;
+  const int2* items;        // (zone, angle) of this level, grouped by domain
+  const BatchSeg* segs;     // segments of this level
+  int nSegs;
+  long nTasks;
+  int angle0;               // first angle of the block being swept
+};
+template <typename Real>
+hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStream_t s);
 // Nonzero when the sweep kernels can carry K groups per lane for G groups.
 int groups_per_lane_supported(int G, int K);
 template <typename Real>

@@ -13,13 +13,16 @@
 #include <vector>
 
 #include "../../include/umt_sweep.h"
+#include "common.h"
 #include "host.h"
 #include "kernels.h"
 
 namespace {
-
 thread_local std::string g_err;
 thread_local int g_dropin_status = 0;
+}  // namespace
+
+namespace umt {
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -31,18 +34,10 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                    \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return fail(100 + (int)e_, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                  __FILE__, __LINE__);                                                   \
-  } while (0)
-
 // Canonical hex corner topology (1-based EZ neighbours per corner-face slot),
 // the layout the sweep kernels are specialised for.
-const int kCanonicalEZ[8][3] = {{4, 2, 5}, {1, 3, 6}, {2, 4, 7}, {3, 1, 8},
-                                {6, 8, 1}, {7, 5, 2}, {8, 6, 3}, {5, 7, 4}};
+static const int kCanonicalEZ[8][3] = {{4, 2, 5}, {1, 3, 6}, {2, 4, 7}, {3, 1, 8},
+                                       {6, 8, 1}, {7, 5, 2}, {8, 6, 3}, {5, 7, 4}};
 
 int check_hex_topology(int Z, int C, const int* numCorner, const int* cOffSet,
                        const int* nCFaces, const int* cEZ) {
@@ -63,21 +58,12 @@ int check_hex_topology(int Z, int C, const int* numCorner, const int* cOffSet,
   return 0;
 }
 
-template <typename T>
-int dalloc(T** p, size_t n) {
-  *p = nullptr;
-  if (n == 0) return 0;
-  HIP_TRY(hipMalloc((void**)p, n * sizeof(T)));
-  return 0;
-}
+}  // namespace umt
 
-template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree(p);
-  p = nullptr;
-}
-
-}  // namespace
+using umt::check_hex_topology;
+using umt::dalloc;
+using umt::dfree;
+using umt::fail;
 
 struct umt_problem {
   umt::Mesh m;
@@ -384,6 +370,17 @@ int umt_problem_tiled(int nx, int ny, int nz, int npolar, int nazimuthal, int kd
 }
 
 void umt_problem_destroy(umt_problem* p) { delete p; }
+
+}  // extern "C"
+
+namespace umt {
+const Mesh& problem_mesh(const umt_problem* p) { return p->m; }
+const Geometry& problem_geometry(const umt_problem* p) { return p->g; }
+const Quadrature& problem_quadrature(const umt_problem* p) { return p->q; }
+const Schedule& problem_schedule(const umt_problem* p) { return p->s; }
+}  // namespace umt
+
+extern "C" {
 
 int umt_problem_dims(const umt_problem* p, int64_t* d) {
   if (!p || !d) return fail(1, "null argument");

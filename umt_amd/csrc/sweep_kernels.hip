@@ -113,19 +113,15 @@ __device__ __forceinline__ void store_groups(Real* p, const GroupVec<Real, K>& r
 // One lane: one corner c of one zone for one direction and K consecutive
 // groups. Geometry, dot products and the zone's EZ orientation are shared by
 // the K groups; everything group-dependent is a K-vector in registers.
+// Item `item` of the level, groups g0 .. g0+K-1, this lane's corner. Lanes
+// with valid == false still take part in the cross-lane moves and the wave
+// vote (they work on a valid item) but store nothing.
 template <typename Real, int K, bool kSingleAngle>
-__global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
+__device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int item, int g0,
+                                             bool valid) {
   const int c = threadIdx.x & 7;
   const bool bottom = c < 4;
-  const long pair = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
   const int G = p.G;
-  const int blocksPerItem = G / K;
-  int item = (int)(pair / blocksPerItem);
-  const int g0 = K * (int)(pair - (long)item * blocksPerItem);
-  // Lanes past the last item still take part in the cross-lane moves and the
-  // wave vote; they work on item 0 and store nothing.
-  const bool valid = item < p.nItems;
-  if (!valid) item = 0;
   const int2 it = p.items[item];
   const int z = it.x, a = it.y;
   const long cc = 8L * z + c;
@@ -254,6 +250,57 @@ __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
   }
 }
 
+template <typename Real, int K, bool kSingleAngle>
+__global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
+  const long pair = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const int blocksPerItem = p.G / K;
+  int item = (int)(pair / blocksPerItem);
+  const int g0 = K * (int)(pair - (long)item * blocksPerItem);
+  const bool valid = item < p.nItems;
+  if (!valid) item = 0;
+  sweep_corner<Real, K, kSingleAngle>(p, item, g0, valid);
+}
+
+// One level of many domains in one launch. The level's items are grouped by
+// domain into segments; segment s starts at task segs[s].taskStart, where a
+// task is one (item, K groups) pair of 8 lanes. Each 8-lane group finds its
+// segment by binary search, takes its domain's arrays, and runs the same
+// corner code as the one-domain kernel.
+template <typename Real, int K>
+__global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real> b) {
+  const long task = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const bool valid = task < b.nTasks;
+  const long t = valid ? task : 0;
+  int lo = 0, hi = b.nSegs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.segs[mid].taskStart <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchSeg sg = b.segs[lo];
+  const BatchDomain<Real>& d = b.doms[sg.domain];
+  SweepLevelArgs<Real> p{};
+  p.items = b.items;
+  p.G = d.G;
+  p.C = d.C;
+  p.Afp = d.Afp;
+  p.Aez = d.Aez;
+  p.vol = d.vol;
+  p.cFP = d.cFP;
+  p.S = d.S;
+  p.sigt = d.sigt;
+  p.omega = d.omega;
+  p.psi = d.psi;
+  p.psiAngleStride = d.psiAngleStride;
+  p.angle0 = b.angle0;
+  p.parTol = 0;
+  const int blocksPerItem = d.G / K;
+  const long local = t - sg.taskStart;
+  const int item = sg.itemStart + (int)(local / blocksPerItem);
+  const int g0 = K * (int)(local - (local / blocksPerItem) * blocksPerItem);
+  sweep_corner<Real, K, false>(p, item, g0, valid);
+}
+
 // phi(g,c) = sum_a w_a psi(g,c,a), accumulated in FP64 in angle order
 // (or added to phi when accumulate is set).
 template <typename Real>
@@ -380,6 +427,19 @@ static hipError_t launch_k(const SweepLevelArgs<Real>& a, bool singleAngle, hipS
 int groups_per_lane_supported(int G, int K) { return (K == 1 || K == 2 || K == 4) && G % K == 0; }
 
 template <typename Real>
+hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStream_t s) {
+  const long threads = 8L * b.nTasks;
+  if (threads == 0) return hipSuccess;
+  const dim3 grid(blocks_for(threads, 256)), block(256);
+  switch (K) {
+    case 4: hipLaunchKernelGGL((ucb_sweep_batch_level<Real, 4>), grid, block, 0, s, b); break;
+    case 2: hipLaunchKernelGGL((ucb_sweep_batch_level<Real, 2>), grid, block, 0, s, b); break;
+    default: hipLaunchKernelGGL((ucb_sweep_batch_level<Real, 1>), grid, block, 0, s, b); break;
+  }
+  return hipGetLastError();
+}
+
+template <typename Real>
 hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
   switch (a.groupsPerLane) {
     case 4: return launch_k<Real, 4>(a, singleAngle, s);
@@ -449,6 +509,8 @@ hipError_t launch_convert(const Src* s_, Dst* d, long n, hipStream_t s) {
   return hipGetLastError();
 }
 
+template hipError_t launch_sweep_batch_level<double>(const BatchLevelArgs<double>&, int, hipStream_t);
+template hipError_t launch_sweep_batch_level<float>(const BatchLevelArgs<float>&, int, hipStream_t);
 template hipError_t launch_sweep_level<double>(const SweepLevelArgs<double>&, bool, hipStream_t);
 template hipError_t launch_sweep_level<float>(const SweepLevelArgs<float>&, bool, hipStream_t);
 template hipError_t launch_phi_from_psi<double>(const double*, const double*, int, long, long,
