@@ -243,6 +243,72 @@ def learned_config4(tol, log):
     return out
 
 
+def batched_domains(n_domains, seed, log):
+    """Uneven domains in one batch: n_domains tiled meshes of n^3 tiles,
+    n ~ U{4..16}, quadrature 2x2, 3x3 or 4x4 per octant (32, 72, 128 angles),
+    G ~ U{8..64}, per-domain seeds, cross sections drawn like the workload's;
+    FP32 sweeps. Reports the batched sweep (autotuned groups per lane and
+    angles per block) against the same domains swept one context at a time."""
+    import umt_amd as U
+
+    rng = np.random.default_rng(seed)
+    spec = [(int(rng.integers(4, 17)), int(rng.choice([2, 3, 4])), int(rng.integers(8, 65)))
+            for _ in range(n_domains)]
+    t0 = time.perf_counter()
+    probs = []
+    for i, (n, a, _) in enumerate(spec):
+        probs.append(U.TiledProblem(n, n, n, a, a))
+        if i % 16 == 15:
+            log(f"batch: built {i + 1} of {n_domains} problems")
+    build_s = time.perf_counter() - t0
+    groups = [g for _, _, g in spec]
+    xs = [U.draw_cross_sections(seed + 1 + i, p.nzones, g, (1e-2, 1e3), (0.0, 0.95), (0.0, 1.0))
+          for i, (p, g) in enumerate(zip(probs, groups))]
+    log(f"batch: {n_domains} problems built in {build_s:.1f} s")
+    # One context at a time (the same sweeps without batching).
+    seq_ms, seq_launches = 0.0, 0
+    for i, (p, g, (st, ss, qq)) in enumerate(zip(probs, groups, xs)):
+        if i % 16 == 15:
+            log(f"batch: swept {i + 1} of {n_domains} domains one at a time")
+        ctx = U.SweepContext(p, g, precision=32)
+        ctx.set_sigt(st)
+        ctx.set_source(np.repeat(qq, 8, axis=0) / (4 * np.pi))
+        ctx.set_graphs(True)
+        k = 4 if g % 4 == 0 else (2 if g % 2 == 0 else 1)
+        ctx.set_groups_per_lane(k)
+        ctx.sweep_n(1)
+        t1 = time.perf_counter()
+        ctx.sweep_n(3)
+        seq_ms += (time.perf_counter() - t1) * 1e3 / 3
+        seq_launches += ctx.levels
+        ctx.close()
+    b = U.SweepBatch(probs, groups, precision=32)
+    for i, (st, ss, qq) in enumerate(xs):
+        b.set_cross_sections(i, st, ss, qq)
+    b.set_layout(4, 32)
+    K, n, ms_tuned = b.autotune(reps=2)
+    ms = b.time_iterations(1, 5)
+    d = b.dims()
+    out = {
+        "workload": (f"{n_domains} domains, n^3 tiles with n ~ U{{4..16}}, 32/72/128 angles, "
+                     f"G ~ U{{8..64}}, sigma_t logU[1e-2,1e3], c U[0,0.95], q U[0,1], FP32 sweeps, "
+                     f"seed {seed}"),
+        "domains": n_domains, "zones_total": int(sum(p.nzones for p in probs)),
+        "unknowns_per_sweep": d["unknowns"],
+        "batched": {"ms_per_iteration": ms, "unknowns_per_s": d["unknowns"] / (ms * 1e-3),
+                    "launches_per_sweep": d["launches"], "groups_per_lane": K,
+                    "angles_per_block": n, "angle_blocks": d["blocks"]},
+        "one_context_at_a_time": {"ms_per_sweep_sum": seq_ms,
+                                  "unknowns_per_s": d["unknowns"] / (seq_ms * 1e-3),
+                                  "level_launches_sum": seq_launches},
+        "speedup_batched": seq_ms / ms,
+        "problem_build_s": build_s,
+    }
+    log(f"batched domains: {out}")
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +323,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iters", type=int, default=3000)
+    ap.add_argument("--batch-domains", type=int, default=64,
+                    help="uneven domains in the batched-sweep section (0: skip it)")
     ap.add_argument("--learn-tiles", type=int, default=4,
                     help="train the learned correction on n^3-tile problems (0: skip it)")
     ap.add_argument("--calibrate", action="store_true",
@@ -358,6 +426,9 @@ def main():
 
         if args.learn_tiles:
             out["learned_config4"] = learned_config4(args.tol, log)
+
+    if args.batch_domains:
+        out["batched_domains"] = batched_domains(args.batch_domains, 20261205, log)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, G, st, S, args.cpu_seconds)

@@ -253,7 +253,8 @@ int umt_ctx_dims(const umt_ctx *ctx, int64_t *dims);
 typedef struct umt_batch umt_batch;
 
 /* One domain per problem (umt_problem_tiled), groups[d] groups each,
- * precision 64 or 32 (FP64 phi and residual either way). */
+ * precision 64 or 32 (FP64 phi and residual either way). Starts with 4
+ * groups per lane and blocks of up to 32 angles. */
 int umt_batch_create(int nDomains, const umt_problem *const *problems, const int *groups,
                      int precision, umt_batch **out);
 void umt_batch_destroy(umt_batch *batch);

@@ -64,6 +64,7 @@ def test_batched_steps_match_single_domain_contexts(domains, K, n):
 
 def test_batched_sweep_matches_oracle(domains):
     b = make_batch(domains)
+    assert b.dims()["angles_per_block"] == 32
     b.sweep()  # from phi = 0: S = q / 4 pi
     for i, (p, G, (st, ss, qq)) in enumerate(domains):
         S = np.repeat(qq, 8, axis=0) / (4 * np.pi)

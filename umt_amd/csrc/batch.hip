@@ -383,7 +383,9 @@ int umt_batch_create(int nDomains, const umt_problem* const* problems, const int
                                                                           : sizeof(umt::BatchDomain<double>)))) ||
       (rc = dalloc(&b->bits, 2 * (size_t)nDomains)))
     return bail(rc);
-  if ((rc = layout(b, b->maxA, 4))) return bail(rc);
+  // Start with blocks of at most 32 angles (psi for all angles of every
+  // domain may not fit); the autotuner tries larger blocks where they fit.
+  if ((rc = layout(b, std::min(b->maxA, 32), 4))) return bail(rc);
   *out = b;
   return 0;
 }
