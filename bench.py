@@ -102,6 +102,8 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
     import umt_amd as U
 
     ctx = U.SweepContext(p, G, precision=precision)
+    if APB[precision]:
+        ctx.set_angles_per_block(APB[precision])
     ctx.set_sigt(st)
     ctx.set_source(S)
     best_k, tune_ms = ctx.autotune(2)
@@ -119,7 +121,10 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
                 autotune_ms_per_sweep={str(k): v for k, v in tune_ms.items()})
 
 
-def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True):
+APB = {64: 0, 32: 0}  # angles per block of the FP64 / FP32 contexts (0: all)
+
+
+def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True, history=False):
     """Source iteration to `tol` from phi = 0: FP64 sweeps (mixed False) or
     FP32 sweeps to 1e-6 then FP64 sweeps (mixed True), with acceleration
     accel ("none", "gta" or "learned"), graph replay on. The timed solve is
@@ -132,6 +137,8 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True):
         ctx = U.SweepContext(p, G, precision=precision)
         ctx.set_sigt(st)
         ctx.set_scattering(ss, qq)
+        if APB[precision]:
+            ctx.set_angles_per_block(APB[precision])
         ctx.autotune(1)
         ctx.set_graphs(True)
         if accel in ("learned", "gta+learned"):
@@ -149,14 +156,16 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True):
                 c.set_phi(zero)
         t0 = time.perf_counter()
         if mixed:
-            ok, it32, it64, res, _ = U.solve_mixed(fast, exact, tol=tol, switch_tol=1e-6,
-                                                   max_iters=max_iters)
+            ok, it32, it64, res, hist = U.solve_mixed(fast, exact, tol=tol, switch_tol=1e-6,
+                                                      max_iters=max_iters)
         else:
-            ok, it64, res, _ = exact.source_iteration(tol=tol, max_iters=max_iters)
+            ok, it64, res, hist = exact.source_iteration(tol=tol, max_iters=max_iters)
             it32 = 0
         dt = time.perf_counter() - t0
         out = dict(converged=ok, iterations_fp32=it32, iterations_fp64=it64, residual=res,
                    seconds=dt)
+        if history:
+            out["history"] = [float(h) for h in hist]
     if accel in ("gta", "gta+learned"):
         gi, gs = exact.gta_stats()
         out["grey_iterations_fp64_total"] = gi
@@ -171,15 +180,26 @@ def rel_error(phi, ref):
     return float(np.max(np.abs(phi - ref) / np.maximum(np.abs(ref), 1e-300)))
 
 
-def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log):
+def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0.95)):
     """Plain FP64 source iteration (the reference method on the GPU) against
     GTA and the learned correction, FP64 and FP32->FP64."""
     import umt_amd as U
 
     res = {}
-    base, phi64 = solve(p, G, xs, "none", False, tol, max_iters)
+    base, phi64 = solve(p, G, xs, "none", False, tol, max_iters, repeat=False, history=True)
+    hist = base.pop("history")
+    if not base["converged"] and len(hist) > 20:
+        # Not converged within max_iters: extrapolate the iterations (and the
+        # time) from the contraction over the last 20 iterations.
+        rate = (hist[-1] / hist[-21]) ** (1.0 / 20)
+        if 0.0 < rate < 1.0:
+            more = np.log(tol / hist[-1]) / np.log(rate)
+            base["estimated_iterations"] = float(len(hist) + more)
+            base["estimated_seconds"] = base["seconds"] * base["estimated_iterations"] / len(hist)
+            base["contraction_per_iteration"] = float(rate)
     res["fp64_plain"] = base
     log(f"fp64 plain: {base}")
+    base_s = base.get("estimated_seconds", base["seconds"])
     cases = [("fp64_gta", "gta", False, None), ("fp32_then_fp64_gta", "gta", True, None)]
     nets = []
     if learn_tiles:
@@ -187,7 +207,7 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log):
         for key, on in (("plain", "none"), ("gta", "gta")):
             t0 = time.perf_counter()
             net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
-                                            c=(0.0, 0.95), q=(0.0, 1.0), rounds=3, batch=8192,
+                                            c=c_range, q=(0.0, 1.0), rounds=3, batch=8192,
                                             steps=1500, collect_stride=4 if on == "none" else 1,
                                             max_iters=max_iters, base=on)
             res[f"learned_training_{key}"] = {
@@ -205,7 +225,11 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log):
     for name, accel, mixed, net in cases:
         r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, net=net)
         r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
-        r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
+        r["speedup_vs_fp64_plain"] = base_s / r["seconds"]
+        if base["converged"]:
+            r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
+        else:
+            r.pop("max_rel_phi_error_vs_fp64_plain", None)
         res[name] = r
         log(f"{name}: {r}")
     for net in nets:
@@ -317,6 +341,11 @@ def main():
     ap.add_argument("--tiles", type=int, nargs=3, default=[20, 20, 20])
     ap.add_argument("--quad", type=int, nargs=2, default=[4, 4], help="npolar nazimuthal")
     ap.add_argument("--groups", type=int, default=16)
+    ap.add_argument("--c", type=float, nargs=2, default=[0.0, 0.95],
+                    help="scattering ratio range of the workload")
+    ap.add_argument("--seed", type=int, default=SEED)
+    ap.add_argument("--angles-per-block", type=int, nargs=2, default=[0, 0],
+                    metavar=("FP64", "FP32"), help="sweep angle blocks (0: all angles at once)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-converge", action="store_true", help="skip time-to-convergence")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 sweep")
@@ -356,7 +385,9 @@ def main():
     t0 = time.perf_counter()
     p = U.TiledProblem(nx, ny, nz, npol, nazi)
     setup_s = time.perf_counter() - t0
-    st, ss, qq = U.draw_cross_sections(SEED, p.nzones, G, (1e-2, 1e3), (0.0, 0.95), (0.0, 1.0))
+    c_range = tuple(args.c)
+    APB[64], APB[32] = args.angles_per_block
+    st, ss, qq = U.draw_cross_sections(args.seed, p.nzones, G, (1e-2, 1e3), c_range, (0.0, 1.0))
     S = np.repeat(qq, 8, axis=0) / (4.0 * np.pi)
     unknowns = p.unknowns(G)
 
@@ -395,7 +426,8 @@ def main():
         "config": {
             "workload": (f"UCB xyz sweep, {nx}x{ny}x{nz} tiles ({p.nzones} zones, {p.ncorner} "
                          f"corners), {npol}x{nazi} product quadrature per octant ({p.nangles} "
-                         f"angles), {G} groups, sigma_t logU[1e-2,1e3], c U[0,0.95], q U[0,1], "
+                         f"angles), {G} groups, sigma_t logU[1e-2,1e3], c U[{c_range[0]},{c_range[1]}], "
+                         f"q U[0,1], seed {args.seed}, "
                          f"vacuum"),
             "tiles": [nx, ny, nz], "zones": p.nzones, "corners": p.ncorner,
             "angles": p.nangles, "groups": G, "unknowns_per_sweep": unknowns,
@@ -422,7 +454,7 @@ def main():
 
     if not args.no_converge:
         out["time_to_converge"] = time_to_converge(
-            p, G, (st, ss, qq), args.tol, args.max_iters, args.learn_tiles, log)
+            p, G, (st, ss, qq), args.tol, args.max_iters, args.learn_tiles, log, c_range)
 
         if args.learn_tiles:
             out["learned_config4"] = learned_config4(args.tol, log)
