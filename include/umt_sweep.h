@@ -70,6 +70,19 @@ typedef struct umt_problem umt_problem;
  * rt/rtorder.F90 / snac/snnext.F90. kdtree != 0 orders zones by a k-d tree. */
 int umt_problem_tiled(int nx, int ny, int nz, int npolar, int nazimuthal, int kdtree,
                       umt_problem **out);
+/* A problem on any hex mesh given as explicit points coords(3, npoints) and
+ * 8 node ids per zone, connectivity(8, nzones), 0-based, in the Conduit
+ * Blueprint / VTK hexahedron order (nodes 1-4 one face, counter-clockwise
+ * seen from nodes 5-8 above them): the unstructured-topology form UMT hands
+ * to Teton (interface/TetonBlueprint.cc:774-890), turned into Teton's corner
+ * arrays as aux/setTetonZone.F90:143-191 does. kdtree != 0 reorders zones
+ * as the UMT driver does. Meshes whose S_N schedule has dependency cycles
+ * are refused; where only the S2 grey-acceleration schedule has cycles (S2
+ * directions lying in warped faces) the problem is built without GTA and
+ * umt_problem_gta says why. */
+int umt_problem_from_hex_mesh(int npoints, const double *coords, int nzones,
+                              const int *connectivity, int npolar, int nazimuthal, int kdtree,
+                              umt_problem **out);
 void umt_problem_destroy(umt_problem *p);
 
 /* dims[0..7] = nzones, ncorner, nbelem, nangles, maxPlanes, npoints,

@@ -32,6 +32,7 @@ SIGNATURES = {
     "gpu_sweepucbxyz_status": (_i, []),
     "umt_problem_tiled": (_i, [_i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
     "umt_problem_destroy": (None, [_vp]),
+    "umt_problem_from_hex_mesh": (_i, [_i, _pd, _i, _pi, _i, _i, _i, C.POINTER(_vp)]),
     "umt_problem_dims": (_i, [_vp, _pi64]),
     "umt_problem_mesh": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd]),
     "umt_problem_geometry": (_i, [_vp, _pd, _pd, _pd, _pd]),

@@ -7,6 +7,7 @@
 //   Geometry arrays: mods/Geometry_mod.F90:23-48
 //   Schedule arrays: mods/AngleSet_mod.F90:77-102
 #pragma once
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -57,6 +58,10 @@ struct Schedule {
 };
 
 void build_tiled_mesh(int nx, int ny, int nz, bool kdtree, Mesh& m);
+// Any hex mesh: points (3, npoints), 8 node ids per zone (0-based, Blueprint
+// / VTK hexahedron order), optional kd-tree zone order.
+void build_hex_mesh(int npoints, const std::vector<double>& points,
+                    const std::vector<std::array<int, 8>>& hexPts, bool kdtree, Mesh& m);
 void compute_geometry(const Mesh& m, Geometry& g);
 void product_quadrature(int npolar, int nazimuthal, Quadrature& q);
 // Level-symmetric S2 (8 directions), the grey-acceleration quadrature.

@@ -8,6 +8,12 @@
 // every consumer (sweep, oracle) receives the same arrays from here.
 // Each tile is one hex layer thick in z; nx*ny*nz tiles fill [0,1]^3.
 //
+// build_hex_mesh takes any hex mesh as explicit points and 8 node ids per
+// zone in the order of a Conduit Blueprint / VTK hexahedron (nodes 0-3 one
+// face, counter-clockwise seen from nodes 4-7 above them), the form in which
+// interface/TetonBlueprint.cc:774-890 receives UMT's meshes, and builds the
+// same Teton corner arrays from it.
+//
 // Corner connectivity follows aux/setTetonZone.F90:143-191: for each corner,
 // the corner faces are chained cyclically so that the EZ neighbour of corner
 // face i is the counter-clockwise neighbour on zone face i, and face i+1 is
@@ -143,6 +149,22 @@ void build_tiled_mesh(int nx, int ny, int nz, bool kdtree, Mesh& m) {
         h[4 + v] = (k + 1) * n2 + quads[q].p[v];
       }
     }
+  std::vector<double> points = std::move(m.points);
+  const int npoints = m.npoints;
+  build_hex_mesh(npoints, points, hexPts, kdtree, m);
+  m.nx = nx; m.ny = ny; m.nz = nz;
+}
+
+void build_hex_mesh(int npoints, const std::vector<double>& points,
+                    const std::vector<std::array<int, 8>>& hexPts, bool kdtree, Mesh& m) {
+  const int Z = (int)hexPts.size();
+  if (Z < 1) throw std::invalid_argument("mesh: no zones");
+  for (const auto& h : hexPts)
+    for (int v : h)
+      if (v < 0 || v >= npoints) throw std::invalid_argument("mesh: hex node index out of range");
+  m = Mesh();
+  m.npoints = npoints;
+  m.points = points;
 
   // Zone order: kd-tree over zone centroids (driver/test_driver.cc:247,1813).
   std::vector<int> order(Z);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -72,6 +73,7 @@ struct umt_problem {
   umt::Schedule s;
   umt::Quadrature q2;  // level-symmetric S2 of the grey-acceleration sweeps
   umt::Schedule s2;
+  std::string gtaError;  // why the S2 schedule is unavailable ("" when it is built)
 };
 
 // Grey-acceleration sweeps use directions that lie exactly in some faces
@@ -340,6 +342,61 @@ const char* umt_last_error(void) { return g_err.c_str(); }
 // ---------------------------------------------------------------------------
 // Problem construction
 
+}  // extern "C"
+
+// Geometry, quadrature and both schedules of a problem whose mesh is built.
+static int finish_problem(umt_problem* p, int npolar, int nazimuthal) {
+  try {
+    umt::compute_geometry(p->m, p->g);
+    umt::product_quadrature(npolar, nazimuthal, p->q);
+    std::string err;
+    int rc = umt::build_schedule(p->m, p->g, p->q, p->s, err);
+    if (rc) return fail(20 + rc, "schedule: %s", err.c_str());
+    // The S2 directions lie in planar diagonal faces; on warped meshes their
+    // schedule can have cycles, and then the problem has no GTA.
+    umt::level_symmetric_s2(p->q2);
+    if (umt::build_schedule(p->m, p->g, p->q2, p->s2, err, kGtaParallelTol)) {
+      p->gtaError = "grey-acceleration schedule: " + err;
+      p->s2 = umt::Schedule();
+    }
+  } catch (const std::exception& e) {
+    return fail(2, "problem construction: %s", e.what());
+  }
+  return 0;
+}
+
+extern "C" {
+
+int umt_problem_from_hex_mesh(int npoints, const double* coords, int nzones, const int* connectivity,
+                              int npolar, int nazimuthal, int kdtree, umt_problem** out) {
+  if (!out || !coords || !connectivity) return fail(1, "null argument");
+  *out = nullptr;
+  if (npoints < 8 || nzones < 1) return fail(1, "bad sizes");
+  for (long i = 0; i < 3L * npoints; ++i)
+    if (!std::isfinite(coords[i])) return fail(17, "coordinate %ld is not finite", i + 1);
+  std::vector<std::array<int, 8>> hexes((size_t)nzones);
+  for (long z = 0; z < nzones; ++z)
+    for (int v = 0; v < 8; ++v) {
+      const int id = connectivity[8 * z + v];
+      if (id < 0 || id >= npoints) return fail(18, "zone %ld node %d: id %d out of range", z + 1, v + 1, id);
+      hexes[z][v] = id;
+    }
+  umt_problem* p = new umt_problem();
+  try {
+    umt::build_hex_mesh(npoints, std::vector<double>(coords, coords + 3L * npoints), hexes,
+                        kdtree != 0, p->m);
+  } catch (const std::exception& e) {
+    delete p;
+    return fail(19, "mesh: %s", e.what());
+  }
+  if (int rc = finish_problem(p, npolar, nazimuthal)) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return 0;
+}
+
 int umt_problem_tiled(int nx, int ny, int nz, int npolar, int nazimuthal, int kdtree,
                       umt_problem** out) {
   if (!out) return fail(1, "null output pointer");
@@ -347,23 +404,13 @@ int umt_problem_tiled(int nx, int ny, int nz, int npolar, int nazimuthal, int kd
   umt_problem* p = new umt_problem();
   try {
     umt::build_tiled_mesh(nx, ny, nz, kdtree != 0, p->m);
-    umt::compute_geometry(p->m, p->g);
-    umt::product_quadrature(npolar, nazimuthal, p->q);
-    std::string err;
-    int rc = umt::build_schedule(p->m, p->g, p->q, p->s, err);
-    if (rc) {
-      delete p;
-      return fail(20 + rc, "schedule: %s", err.c_str());
-    }
-    umt::level_symmetric_s2(p->q2);
-    rc = umt::build_schedule(p->m, p->g, p->q2, p->s2, err, kGtaParallelTol);
-    if (rc) {
-      delete p;
-      return fail(20 + rc, "grey-acceleration schedule: %s", err.c_str());
-    }
   } catch (const std::exception& e) {
     delete p;
     return fail(2, "problem construction: %s", e.what());
+  }
+  if (int rc = finish_problem(p, npolar, nazimuthal)) {
+    delete p;
+    return rc;
   }
   *out = p;
   return 0;
@@ -447,6 +494,7 @@ int umt_problem_schedule(const umt_problem* p, int* nHyperPlanes, int* zonesInPl
 int umt_problem_gta(const umt_problem* p, int* nangles, double* omega, double* weight,
                     int* nHyperPlanes, int* zonesInPlane, int* nextZ) {
   if (!p) return fail(1, "null problem");
+  if (!p->gtaError.empty()) return fail(47, "%s", p->gtaError.c_str());
   if (nangles) *nangles = p->q2.nangles;
   copy_out(p->q2.omega, omega);
   copy_out(p->q2.weight, weight);
@@ -728,8 +776,10 @@ int umt_ctx_create_from_problem(const umt_problem* p, int groups, int precision,
                           p->q.weight.data(), p->s.nHyperPlanes.data(), p->s.zonesInPlane.data(),
                           p->s.nextZ.data(), out);
   if (rc) return rc;
-  rc = umt_ctx_enable_gta(*out, p->q2.nangles, p->q2.omega.data(), p->q2.weight.data(),
-                          p->s2.nHyperPlanes.data(), p->s2.zonesInPlane.data(), p->s2.nextZ.data());
+  if (p->gtaError.empty())
+    rc = umt_ctx_enable_gta(*out, p->q2.nangles, p->q2.omega.data(), p->q2.weight.data(),
+                            p->s2.nHyperPlanes.data(), p->s2.zonesInPlane.data(),
+                            p->s2.nextZ.data());
   if (!rc) rc = umt_ctx_set_zone_neighbors(*out, p->m.zoneOpp.data());
   if (rc) {
     std::string keep = g_err;

@@ -41,26 +41,48 @@ def _f64(a):
 
 
 class TiledProblem:
-    """Tiled hex mesh of [0,1]^3 with geometry, quadrature and schedule."""
+    """Hex mesh (UMT's tiled mesh of [0,1]^3, or any hex mesh through
+    from_hex_mesh) with geometry, quadrature and schedules."""
 
     def __init__(self, nx, ny, nz, npolar, nazimuthal, kdtree=True):
         lib = _lib.load()
         h = C.c_void_p()
         check("umt_problem_tiled",
               lib.umt_problem_tiled(nx, ny, nz, npolar, nazimuthal, int(kdtree), C.byref(h)))
+        self._init(h, (nx, ny, nz), npolar, nazimuthal)
+
+    @classmethod
+    def from_hex_mesh(cls, points, hexes, npolar, nazimuthal, kdtree=True):
+        """Problem on any hex mesh: points (npoints, 3), hexes (nzones, 8)
+        0-based node ids in Conduit Blueprint / VTK hexahedron order."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        hx = np.ascontiguousarray(hexes, dtype=np.int32)
+        h = C.c_void_p()
+        check("umt_problem_from_hex_mesh", _lib.load().umt_problem_from_hex_mesh(
+            pts.shape[0], _dp(pts), hx.shape[0], _ip(hx), npolar, nazimuthal, int(kdtree),
+            C.byref(h)))
+        self = cls.__new__(cls)
+        self._init(h, None, npolar, nazimuthal)
+        return self
+
+    def _init(self, h, tiles, npolar, nazimuthal):
+        lib = _lib.load()
         self._h = h
         d = np.zeros(8, dtype=np.int64)
         check("umt_problem_dims", lib.umt_problem_dims(h, d.ctypes.data_as(C.POINTER(C.c_int64))))
         (self.nzones, self.ncorner, self.nbelem, self.nangles, self.max_planes, self.npoints,
          self.total_planes, self.maxcf) = (int(x) for x in d)
-        self.tiles = (nx, ny, nz)
+        self.tiles = tiles
         self.npolar, self.nazimuthal = npolar, nazimuthal
         self._mesh = self._geom = self._quad = self._sched = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.load().umt_problem_destroy(h)
+            try:
+                _lib.load().umt_problem_destroy(h)
+            except Exception:
+                pass
             self._h = None
 
     @property
