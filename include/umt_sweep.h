@@ -166,6 +166,20 @@ int umt_ctx_set_phi(umt_ctx *ctx, const double *phi);   /* (G,C) */
  * maxIters was reached (the state is still valid). */
 int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
                              double *residual, double *residual_history);
+/* Time-dependent sweeps. tau = 1/(c dt) (control/initializeSets.F90:81):
+ * the sweep sees Sigt + tau (control/setTotalOpacity.F90) and the source
+ * STotal + tau Psi(g, c, Angle) (snac/SweepUCBxyz.F90:121), Psi being the
+ * previous cycle's angular flux held on the device for every angle. tau = 0
+ * (default) is the steady problem. umt_ctx_save_psi is the cycle's final
+ * sweep with savePsi (rt/rtmainsn.F90:238-246, SweepUCBxyz.F90:314-318):
+ * source from the current phi, one sweep, Psi = its psi, phi = its phi.
+ * Psi arrays on the host are Teton's Psi(G, C, A). GTA accounts for tau in
+ * its grey total cross section. */
+int umt_ctx_set_tau(umt_ctx *ctx, double tau);
+int umt_ctx_set_psi_prev(umt_ctx *ctx, const double *Psi);
+int umt_ctx_get_psi_prev(const umt_ctx *ctx, double *Psi);
+int umt_ctx_save_psi(umt_ctx *ctx);
+
 /* window > 0: source iteration also stops (returning 2) once `window`
  * iterations pass without a new smallest residual, as at the rounding floor
  * of FP32 sweeps; 0 (default) never stops early. No counterpart in the reference. */

@@ -177,3 +177,55 @@ def source_iteration(mesh, geom, quad, sched, Sigt, sigs, q, tol, max_iters, phi
         _ip(sched["nextZ"]), _ip(sched["nextC"]), nthreads, tol, max_iters, _dp(Phi),
         C.byref(res), _dp(hist))
     return Phi, it, res.value, hist[:it].copy()
+
+
+def time_step(mesh, geom, quad, sched, sigt, sigs, q, tau, psi_prev, tol, max_iters, phi0=None):
+    """One implicit time step: source iteration with the sweep of
+    snac/SweepUCBxyz.F90:119-130 for tau > 0 (total Sigt + tau,
+    control/setTotalOpacity.F90; source STotal + tau Psi(:,:,Angle) with Psi
+    the previous cycle's psi_prev (A, C, G)), residual as in
+    source_iteration, then the cycle's final sweep with savePsi
+    (rt/rtmainsn.F90:238-246). Angles are swept one after another.
+    Returns (phi (C, G), psi_new (A, C, G), iterations, residual, history)."""
+    Cn = mesh["nCFaces"].shape[0]
+    G = sigt.shape[1]
+    A = quad["weight"].shape[0]
+    nb = int(np.count_nonzero(mesh["cFP"] > Cn))
+    V, VZ = geom["Volume"], geom["VolumeZone"]
+    st = np.ascontiguousarray(sigt + tau)
+    sigs_c, q_c = np.repeat(sigs, 8, axis=0), np.repeat(q, 8, axis=0)
+    starts = np.concatenate([[0], np.cumsum(sched["nHyperPlanes"])])
+    psi_prev = np.ascontiguousarray(psi_prev, dtype=np.float64)
+
+    def sweep(phi, save):
+        S = np.ascontiguousarray((sigs_c * phi + q_c) / (4.0 * np.pi))
+        Phi = np.zeros((Cn, G))
+        new = np.zeros((A, Cn, G)) if save else None
+        for a in range(A):
+            Psi1 = np.zeros((Cn + nb, G))
+            PsiB = np.zeros((nb, G))
+            sweep_angle(mesh, geom, S, st, quad["omega"][a], quad["weight"][a], sched["nextZ"][a],
+                        sched["nextC"][a], sched["zonesInPlane"][starts[a]:starts[a + 1]], Psi1,
+                        PsiB, Phi, tau=tau, Psi=np.ascontiguousarray(psi_prev[a]))
+            if save:
+                new[a] = Psi1[:Cn]
+        return Phi, new
+
+    def zavg(x):
+        return (x * V[:, None]).reshape(-1, 8, G).sum(1) / VZ[:, None]
+
+    phi = np.zeros((Cn, G)) if phi0 is None else np.array(phi0, dtype=np.float64)
+    old = zavg(phi)
+    hist, res, it = [], 1.0, 0
+    while it < max_iters:
+        phi, _ = sweep(phi, False)
+        cur = zavg(phi)
+        mask = cur > 1e-6 * cur.max()
+        res = float(np.max(np.abs((cur[mask] - old[mask]) / cur[mask]))) if mask.any() else 0.0
+        old = cur
+        hist.append(res)
+        it += 1
+        if res < tol:
+            break
+    phi, new = sweep(phi, True)
+    return phi, new, it, res, np.array(hist)

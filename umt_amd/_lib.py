@@ -67,6 +67,10 @@ SIGNATURES = {
     "umt_ctx_get_angles_per_block": (_i, [_vp, _pi, _pi]),
     "umt_ctx_copy_phi": (_i, [_vp, _vp]),
     "umt_ctx_set_stall_window": (_i, [_vp, _i]),
+    "umt_ctx_set_tau": (_i, [_vp, _d]),
+    "umt_ctx_set_psi_prev": (_i, [_vp, _pd]),
+    "umt_ctx_get_psi_prev": (_i, [_vp, _pd]),
+    "umt_ctx_save_psi": (_i, [_vp]),
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
     "umt_ctx_grey_sweep": (_i, [_vp, _pd, _pd, _pd]),
     "umt_ctx_grey_solve": (_i, [_vp, _pd, _pd, _d, _pd, _pi64]),

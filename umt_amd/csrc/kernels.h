@@ -25,7 +25,8 @@ struct SweepLevelArgs {
   // Single-angle (drop-in) extras.
   double* phi;              // (G, C), phi += quadwt * psi
   double quadwt;
-  const Real* tauPsi;       // (G, C) or null: source += tau * Psi
+  const Real* tauPsi;       // null, or source += tau * Psi: (G, C) of this angle (single
+                            // angle) or the previous cycle's (G, C + nbelem, A) (all angles)
   Real tau;
 };
 

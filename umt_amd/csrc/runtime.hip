@@ -213,6 +213,8 @@ struct umt_ctx {
   int acceleration = 0;   // 0 plain source iteration, 1 GTA, 2 learned correction, 3 both
   std::vector<double> sigtHost, sigsHost;  // last uploaded cross sections (GTA opacities)
   double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
+  double tau = 0.0;       // 1/(c dt): time-dependent sweeps when > 0
+  void* psiPrev = nullptr;  // previous cycle's psi, (G, C + nbelem, A), sweep precision
   GtaState gta;
   LearnState learn;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -243,6 +245,10 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
   a.psiAngleStride = x->psiStride();
   a.angle0 = angle0;
   a.parTol = (Real)set.parTol;
+  if (&set == &x->sn && x->tau > 0.0) {  // tau Psi source of the S_N sweep only
+    a.tauPsi = (const Real*)x->psiPrev;
+    a.tau = (Real)x->tau;
+  }
   return a;
 }
 
@@ -250,7 +256,7 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
 // phi (FP64, angle order).
 template <typename Real>
 int launch_sweep(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
-                 bool accumulate, bool timeLevels) {
+                 bool accumulate, bool timeLevels, void* savePsi = nullptr) {
   for (int b = 0; b < set.blocks(); ++b) {
     const int a0 = b * set.anglesPerBlock;
     const int nA = std::min(set.anglesPerBlock, set.A - a0);
@@ -259,6 +265,10 @@ int launch_sweep(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, dou
       HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h, a0), false, x->stream));
       if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h + 1], x->stream));
     }
+    if (savePsi)  // Psi(:,:,Angle) = Psi1 for this block's angles (SweepUCBxyz.F90:314-318)
+      HIP_TRY(hipMemcpyAsync((char*)savePsi + (size_t)a0 * x->psiStride() * sizeof(Real), set.psi,
+                             (size_t)nA * x->psiStride() * sizeof(Real), hipMemcpyDeviceToDevice,
+                             x->stream));
     HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)set.psi, set.weight + a0, nA,
                                                x->psiStride(), (long)x->C * x->G, phiOut,
                                                accumulate || b > 0, x->stream));
@@ -566,6 +576,7 @@ void umt_ctx_destroy(umt_ctx* x) {
   if (!x) return;
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   free_learn(x->learn);
+  dfree(x->psiPrev);
   dfree(x->Afp); dfree(x->Aez); dfree(x->vol); dfree(x->cFP); dfree(x->vol64); dfree(x->volZ);
   dfree(x->S); dfree(x->sigt); dfree(x->sigs); dfree(x->q); dfree(x->phi); dfree(x->phiPrev);
   dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
@@ -798,15 +809,79 @@ int umt_ctx_dims(const umt_ctx* x, int64_t* d) {
   return 0;
 }
 
+// The sweep's total cross section: sigma_t + tau (control/setTotalOpacity.F90).
+static int upload_sigt(umt_ctx* x) {
+  std::vector<double> st(x->sigtHost);
+  for (double& v : st) v += x->tau;
+  int rc = upload_real(x, x->sigt, st.data(), st.size());
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+static void drop_graphs(SweepSet& set) {
+  for (auto& g : set.graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  set.graphs.clear();
+}
+
 int umt_ctx_set_sigt(umt_ctx* x, const double* sigt) {
   if (!x || !sigt) return fail(1, "null argument");
   for (long i = 0; i < (long)x->G * x->Z; ++i)
     if (!(sigt[i] > 0.0) || !std::isfinite(sigt[i])) return fail(30, "Sigt entry %ld is not positive and finite", i + 1);
-  int rc = upload_real(x, x->sigt, sigt, (size_t)x->G * x->Z);
+  x->sigtHost.assign(sigt, sigt + (size_t)x->G * x->Z);
+  if (int rc = upload_sigt(x)) return rc;
+  x->haveSigt = true;
+  return 0;
+}
+
+int umt_ctx_set_tau(umt_ctx* x, double tau) {
+  if (!x) return fail(1, "null context");
+  if (!(tau >= 0.0) || !std::isfinite(tau)) return fail(48, "tau must be finite and >= 0");
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  if (tau > 0.0 && !x->psiPrev) {
+    const size_t n = (size_t)x->psiStride() * x->sn.A;
+    if (int rc = dalloc((char**)&x->psiPrev, n * x->rsize())) return rc;
+    if (x->precision == 32) HIP_TRY(umt::launch_fill_zero<float>((float*)x->psiPrev, (long)n, x->stream));
+    else HIP_TRY(umt::launch_fill_zero<double>((double*)x->psiPrev, (long)n, x->stream));
+  }
+  x->tau = tau;
+  drop_graphs(x->sn);  // the level launches carry tau and the Psi pointer
+  if (x->haveSigt)
+    if (int rc = upload_sigt(x)) return rc;
+  return 0;
+}
+
+int umt_ctx_set_psi_prev(umt_ctx* x, const double* psi) {
+  if (!x || !psi) return fail(1, "null argument");
+  if (!x->psiPrev) return fail(49, "umt_ctx_set_tau with tau > 0 first");
+  const size_t GC = (size_t)x->G * x->C, stride = (size_t)x->psiStride();
+  std::vector<double> h(stride * x->sn.A, 0.0);
+  for (int a = 0; a < x->sn.A; ++a) std::memcpy(&h[a * stride], psi + a * GC, GC * sizeof(double));
+  int rc = upload_real(x, x->psiPrev, h.data(), h.size());
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(x->stream));
-  x->haveSigt = true;
-  x->sigtHost.assign(sigt, sigt + (size_t)x->G * x->Z);
+  return 0;
+}
+
+int umt_ctx_get_psi_prev(const umt_ctx* x, double* psi) {
+  if (!x || !psi) return fail(1, "null argument");
+  if (!x->psiPrev) return fail(49, "umt_ctx_set_tau with tau > 0 first");
+  const size_t GC = (size_t)x->G * x->C, stride = (size_t)x->psiStride();
+  const size_t n = stride * x->sn.A;
+  std::vector<double> h(n);
+  if (x->precision == 64) {
+    HIP_TRY(hipMemcpyAsync(h.data(), x->psiPrev, n * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  } else {
+    std::vector<float> f(n);
+    HIP_TRY(hipMemcpyAsync(f.data(), x->psiPrev, n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+    for (size_t i = 0; i < n; ++i) h[i] = f[i];
+  }
+  for (int a = 0; a < x->sn.A; ++a) std::memcpy(psi + a * GC, &h[a * stride], GC * sizeof(double));
   return 0;
 }
 
@@ -1052,7 +1127,7 @@ int gta_prepare(umt_ctx* x) {
     return fail(38, "GTA needs sigma_t and sigma_s (umt_ctx_set_sigt, umt_ctx_set_scattering)");
   std::vector<double> st(2 * GZ), ep(2 * GZ);
   for (size_t i = 0; i < GZ; ++i) {
-    const double sigt = x->sigtHost[i], sigs = x->sigsHost[i], siga = sigt - sigs;
+    const double sigt = x->sigtHost[i] + x->tau, sigs = x->sigsHost[i], siga = sigt - sigs;
     double greyScat = sigs, greyTot = sigt;
     if (sigs / sigt <= 1.0e-10) {
       greyScat = 0.0;
@@ -1264,8 +1339,10 @@ int learn_prepare(umt_ctx* x) {
         (rc = dalloc(&L.X, GZ * umt::kMlpFeatures)) || (rc = dalloc(&L.r, GZ)))
       return rc;
   }
-  HIP_TRY(hipMemcpyAsync(L.sigt64, x->sigtHost.data(), GZ * sizeof(double), hipMemcpyHostToDevice,
-                         x->stream));
+  std::vector<double> st(x->sigtHost);
+  for (double& v : st) v += x->tau;
+  HIP_TRY(hipMemcpyAsync(L.sigt64, st.data(), GZ * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
 }
 
@@ -1425,6 +1502,35 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
   if (iters) *iters = it;
   if (residual) *residual = res;
   return res < tol ? 0 : (stalled ? 2 : 1);
+}
+
+int umt_ctx_save_psi(umt_ctx* x) {
+  if (!x) return fail(1, "null context");
+  if (!x->psiPrev) return fail(49, "umt_ctx_set_tau with tau > 0 first");
+  if (!x->haveScattering) return fail(33, "umt_ctx_save_psi needs umt_ctx_set_scattering first");
+  // The final sweep of a cycle (rt/rtmainsn.F90:238-246): source from the
+  // converged phi, sweep, Psi = psi of every angle. The tau Psi source of
+  // this sweep still reads the old Psi, so psi goes to a scratch copy first.
+  const long nGC = (long)x->G * x->C;
+  const size_t n = (size_t)x->psiStride() * x->sn.A;
+  void* fresh = nullptr;
+  if (int rc = dalloc((char**)&fresh, n * x->rsize())) return rc;
+  int rc = 0;
+  if (x->precision == 32) {
+    if (umt::launch_source_update<float>(x->phi, x->sigs, x->q, x->G, nGC, (float*)x->S, x->stream) != hipSuccess)
+      rc = fail(101, "source update failed");
+    if (!rc) rc = launch_sweep<float>(x, x->sn, x->S, x->sigt, x->phi, false, false, fresh);
+  } else {
+    if (umt::launch_source_update<double>(x->phi, x->sigs, x->q, x->G, nGC, (double*)x->S, x->stream) != hipSuccess)
+      rc = fail(101, "source update failed");
+    if (!rc) rc = launch_sweep<double>(x, x->sn, x->S, x->sigt, x->phi, false, false, fresh);
+  }
+  if (!rc && hipMemcpyAsync(x->psiPrev, fresh, n * x->rsize(), hipMemcpyDeviceToDevice, x->stream) != hipSuccess)
+    rc = fail(101, "psi copy failed");
+  if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = fail(101, "synchronize failed");
+  (void)hipStreamSynchronize(x->stream);
+  dfree(fresh);
+  return rc;
 }
 
 int umt_ctx_set_stall_window(umt_ctx* x, int window) {

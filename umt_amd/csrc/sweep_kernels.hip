@@ -160,8 +160,12 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   }
   const GroupVec<Real, K> sigv_ = load_groups<Real, K>(p.sigt + (long)z * G + g0);
   GroupVec<Real, K> Q = load_groups<Real, K>(p.S + cc * G + g0);
-  if (kSingleAngle && p.tauPsi) {
-    const GroupVec<Real, K> tp = load_groups<Real, K>(p.tauPsi + cc * G + g0);
+  if (p.tauPsi) {
+    // Time-dependent source tau Psi(g, c, Angle) (SweepUCBxyz.F90:121): the
+    // drop-in passes this angle's Psi (G, C); the all-angle sweep reads the
+    // previous cycle's psi of angle a, stored like psi with every angle.
+    const Real* tpA = kSingleAngle ? p.tauPsi : p.tauPsi + (long)a * p.psiAngleStride;
+    const GroupVec<Real, K> tp = load_groups<Real, K>(tpA + cc * G + g0);
 #pragma unroll
     for (int k = 0; k < K; ++k) Q.v[k] += p.tau * tp.v[k];
   }

@@ -344,6 +344,25 @@ class SweepContext:
             check("umt_ctx_source_iteration", rc)
         return rc == 0, it.value, res.value, hist[:it.value].copy()
 
+    def set_tau(self, tau):
+        """Time-dependent sweeps with tau = 1/(c dt) (0: steady)."""
+        check("umt_ctx_set_tau", _lib.load().umt_ctx_set_tau(self._h, tau))
+
+    def set_psi_prev(self, psi):
+        """Previous cycle's Psi as (A, C, G) = Teton Psi(G, C, A)."""
+        a = _f64(psi)
+        check("umt_ctx_set_psi_prev", _lib.load().umt_ctx_set_psi_prev(self._h, _dp(a)))
+
+    def psi_prev(self):
+        p = self.problem
+        out = np.zeros((p.nangles, p.ncorner, self.G))
+        check("umt_ctx_get_psi_prev", _lib.load().umt_ctx_get_psi_prev(self._h, _dp(out)))
+        return out
+
+    def save_psi(self):
+        """The cycle's final sweep: Psi = psi of a sweep from the current phi."""
+        check("umt_ctx_save_psi", _lib.load().umt_ctx_save_psi(self._h))
+
     def set_stall_window(self, window):
         """Stop source iteration after `window` iterations without a new best residual."""
         check("umt_ctx_set_stall_window", _lib.load().umt_ctx_set_stall_window(self._h, window))
