@@ -140,6 +140,8 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True, history
         if APB[precision]:
             ctx.set_angles_per_block(APB[precision])
         ctx.autotune(1)
+        if accel in ("gta", "gta+learned"):
+            ctx.autotune_grey(2)
         ctx.set_graphs(True)
         if accel in ("learned", "gta+learned"):
             ctx.set_learner(net, every=2)

@@ -73,6 +73,7 @@ SIGNATURES = {
     "umt_ctx_save_psi": (_i, [_vp]),
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
     "umt_ctx_grey_sweep": (_i, [_vp, _pd, _pd, _pd]),
+    "umt_ctx_autotune_grey": (_i, [_vp, _i, _pi, _pd]),
     "umt_ctx_grey_solve": (_i, [_vp, _pd, _pd, _d, _pd, _pi64]),
     "umt_batch_create": (_i, [_i, C.POINTER(_vp), _pi, _i, C.POINTER(_vp)]),
     "umt_batch_destroy": (None, [_vp]),

@@ -92,6 +92,7 @@ static const double kGtaParallelTolFP32 = 1e-5;
 struct SweepSet {
   int A = 0;
   int anglesPerBlock = 0;
+  int groupsPerLane = 0;     // K of this set's sweeps (0: the context's)
   double parTol = 0.0;       // parallel-face tolerance of schedule and sweep
   double* omega = nullptr;   // (3, A)
   double* weight = nullptr;  // (A)
@@ -232,7 +233,7 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
   a.items = set.items + set.levelOffset[level];
   a.nItems = set.levelOffset[level + 1] - set.levelOffset[level];
   a.G = x->G;
-  a.groupsPerLane = x->groupsPerLane;
+  a.groupsPerLane = set.groupsPerLane ? set.groupsPerLane : x->groupsPerLane;
   a.C = x->C;
   a.Afp = (const Real*)x->Afp;
   a.Aez = (const Real*)x->Aez;
@@ -283,12 +284,13 @@ template <typename Real>
 int sweep_graph(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
                 bool accumulate, hipGraphExec_t* out) {
   for (auto& g : set.graphs)
-    if (g.S == S && g.sigt == sigt && g.K == x->groupsPerLane && g.phi == phiOut &&
+    if (g.S == S && g.sigt == sigt && g.K == (set.groupsPerLane ? set.groupsPerLane : x->groupsPerLane) && g.phi == phiOut &&
         g.accumulate == accumulate) {
       *out = g.exec;
       return 0;
     }
-  SweepSet::Graph g{S, sigt, x->groupsPerLane, phiOut, accumulate, nullptr, nullptr};
+  SweepSet::Graph g{S, sigt, set.groupsPerLane ? set.groupsPerLane : x->groupsPerLane, phiOut,
+                    accumulate, nullptr, nullptr};
   HIP_TRY(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
   if (int rc = launch_sweep<Real>(x, set, S, sigt, phiOut, accumulate, false)) {
     hipGraph_t dummy = nullptr;
@@ -1366,6 +1368,40 @@ int learn_record(umt_ctx* x, double res) {
 }  // namespace
 
 extern "C" {
+
+int umt_ctx_autotune_grey(umt_ctx* x, int reps, int* bestK, double* msPerK) {
+  if (!x || reps < 1) return fail(1, "bad argument");
+  if (!x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
+  if (int rc = gta_prepare(x)) return rc;
+  GtaState& g = x->gta;
+  const long GC = (long)x->G * x->C;
+  if (x->precision == 32) HIP_TRY(umt::launch_fill_zero<float>((float*)g.S2, GC, x->stream));
+  else HIP_TRY(umt::launch_fill_zero<double>((double*)g.S2, GC, x->stream));
+  static const int cand[3] = {1, 2, 4};
+  double best = 1e300;
+  int bk = x->s2.groupsPerLane;
+  for (int i = 0; i < 3; ++i) {
+    if (msPerK) msPerK[i] = -1.0;
+    if (!umt::groups_per_lane_supported(x->G, cand[i])) continue;
+    x->s2.groupsPerLane = cand[i];
+    if (int rc = sweep_set(x, x->s2, g.S2, g.sigt2, g.As, false, false)) return rc;  // warm-up
+    HIP_TRY(hipEventRecord(x->ev0, x->stream));
+    for (int r = 0; r < reps; ++r)
+      if (int rc = sweep_set(x, x->s2, g.S2, g.sigt2, g.As, false, false)) return rc;
+    HIP_TRY(hipEventRecord(x->ev1, x->stream));
+    HIP_TRY(hipEventSynchronize(x->ev1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, x->ev0, x->ev1));
+    if (msPerK) msPerK[i] = t / reps;
+    if (t / reps < best) {
+      best = t / reps;
+      bk = cand[i];
+    }
+  }
+  x->s2.groupsPerLane = bk;
+  if (bestK) *bestK = bk;
+  return 0;
+}
 
 int umt_ctx_grey_sweep(umt_ctx* x, const double* P, const double* gsrc, double* out) {
   if (!x || !out) return fail(1, "null argument");
