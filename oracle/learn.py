@@ -16,8 +16,8 @@ How each part is pinned (tests/test_oracle_learn.py):
   adam_step          torch.optim.Adam with the same hyperparameters
   features           closed forms on uniform and equilibrium states
   source_iteration   zero network = plain source iteration exactly; the guard
-                     falls back on a diverging correction and still reaches the
-                     plain fixed point
+                     falls back on a diverging correction and on a non-finite
+                     one, and still reaches the plain fixed point
 """
 import numpy as np
 
@@ -192,6 +192,7 @@ def source_iteration(mesh, geom, quad, sched, sigt, sigs, q, params, tol, max_it
     correcting = correct and params is not None
     track = correcting or collect
     since, have_dp, last_res = 0, False, -1.0
+    pending_bad = False
     dp = None
     hist, states = [], []
     fell_back, corrections = 0, 0
@@ -207,6 +208,10 @@ def source_iteration(mesh, geom, quad, sched, sigt, sigs, q, params, tol, max_it
         phi = half
         hist.append(res)
         it += 1
+        if pending_bad and correcting:  # a non-finite correction: stop correcting
+            correcting = False
+            fell_back = it
+        pending_bad = False
         if res < tol or not np.isfinite(res):
             break
         if track:
@@ -223,8 +228,13 @@ def source_iteration(mesh, geom, quad, sched, sigt, sigs, q, params, tol, max_it
                     states.append((X, ph.copy(), d.copy(), res))
                 if correcting:
                     r, _ = forward(params, X.reshape(-1, NF))
-                    r = r.astype(np.float32).astype(np.float64).reshape(ph.shape)
-                    phi = half + np.repeat(r, 8, axis=0) * (half - prev)
+                    with np.errstate(over="ignore", invalid="ignore"):
+                        r = r.astype(np.float32).astype(np.float64).reshape(ph.shape)
+                        new = half + np.repeat(r, 8, axis=0) * (half - prev)
+                    ok = np.isfinite(new)
+                    phi = np.where(ok, new, half)  # non-finite corrections are not applied
+                    if not ok.all():
+                        pending_bad = True
                     old = zone_average(phi, V, VZ)
                     since = 0
                     last_res = res

@@ -306,3 +306,25 @@ def test_learned_correction_on_top_of_gta(cfg4):
     assert np.max(np.abs(a - b) / b) < 1e-6
     ctx.close()
     net.close()
+
+
+def test_non_finite_correction_falls_back_on_the_gpu(cfg4):
+    """Finite weights whose FP32 output overflows: the correction is not
+    applied, corrections stop, and plain iteration reaches its fixed point."""
+    from test_oracle_learn import overflowing_params
+    p = cfg4[0]
+    ctx, _ = context(p, 2, 24)
+    conv, _, _, _ = ctx.source_iteration(tol=1e-10, max_iters=500)
+    ref = ctx.phi()
+    net = U.Learner(0)
+    net.set_params(overflowing_params().astype(np.float32))
+    ctx.set_phi(np.zeros_like(ref))
+    ctx.set_learner(net, every=2)
+    ctx.set_acceleration("learned")
+    conv, it, res, _ = ctx.source_iteration(tol=1e-10, max_iters=500)
+    assert conv and ctx.learn_stats()[1] > 0
+    V, VZ = p.geometry()["Volume"], p.geometry()["VolumeZone"]
+    a, b = OL.zone_average(ctx.phi(), V, VZ), OL.zone_average(ref, V, VZ)
+    assert np.max(np.abs(a - b) / b) < 1e-8
+    ctx.close()
+    net.close()

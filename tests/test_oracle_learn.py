@@ -221,3 +221,22 @@ def test_collected_states_match_the_iteration(tiny):
     for k, (X, ph, d, r) in enumerate(states):
         assert r == res["history"][k + 1]
         assert X.shape == ph.shape + (16,)
+
+
+def overflowing_params():
+    """Finite weights whose FP32 output overflows to infinity."""
+    par = np.zeros(OL.NPARAMS)
+    par[OL.B2:OL.W3] = 1.0          # h2 = tanh(1) everywhere
+    par[OL.W3:OL.B3] = 3.0e38       # 64 terms of ~2.3e38 overflow FP32
+    return par
+
+
+def test_non_finite_correction_falls_back(tiny):
+    p, m, g, q, s = tiny
+    st, ss, qq = problem_xs(p, 15)
+    plain = O.source_iteration(m, g, q, s, st, ss, qq, 1e-10, 500)
+    res = OL.source_iteration(m, g, q, s, st, ss, qq, overflowing_params(), 1e-10, 500)
+    assert res["fell_back"] > 0 and res["residual"] < 1e-10
+    pz = OL.zone_average(plain[0], g["Volume"], g["VolumeZone"])
+    rz = OL.zone_average(res["phi"], g["Volume"], g["VolumeZone"])
+    assert np.max(np.abs(rz - pz) / pz) < 1e-8

@@ -140,7 +140,7 @@ hipError_t launch_learn_features(const double* ph, const double* pp, const doubl
                                  const int* zoneNbr, int G, long n, double* rel, double* ratio,
                                  unsigned long long* maxRel, float* X, hipStream_t s);
 hipError_t launch_learn_apply(double* phi, const double* phiPrev, const float* r, int G, long n,
-                              hipStream_t s);
+                              unsigned long long* bad, hipStream_t s);
 hipError_t launch_learn_delta(const double* ph, const double* pp, long n, double* d, hipStream_t s);
 hipError_t launch_learn_targets(const double* ph, const double* d, const double* pstar, long n,
                                 double* a, double* b, double* partial, float* aOut, float* bOut,

@@ -452,16 +452,21 @@ __global__ __launch_bounds__(256) void learn_features(
   for (int k = 10; k < kMlpFeatures; ++k) x[k] = 0.f;
 }
 
-// phi <- phi + r (phi - phiPrev), r per (zone, group); phi holds the sweep's result.
+// phi <- phi + r (phi - phiPrev), r per (zone, group); phi holds the sweep's
+// result. A non-finite r (or corrected value) leaves that entry uncorrected
+// and raises *bad, so the iteration can stop correcting.
 __global__ __launch_bounds__(256) void learn_apply(double* __restrict__ phi,
                                                    const double* __restrict__ phiPrev,
-                                                   const float* __restrict__ r, int G, long n) {
+                                                   const float* __restrict__ r, int G, long n,
+                                                   unsigned long long* __restrict__ bad) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long c = i / G;
   const int g = (int)(i - c * G);
   const double h = phi[i];
-  phi[i] = h + (double)r[(c >> 3) * G + g] * (h - phiPrev[i]);
+  const double v = h + (double)r[(c >> 3) * G + g] * (h - phiPrev[i]);
+  if (isfinite(v)) phi[i] = v;
+  else *bad = 1ull;
 }
 
 __global__ __launch_bounds__(256) void learn_delta(const double* __restrict__ ph,
@@ -572,8 +577,9 @@ hipError_t launch_learn_features(const double* ph, const double* pp, const doubl
 }
 
 hipError_t launch_learn_apply(double* phi, const double* phiPrev, const float* r, int G, long n,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(learn_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, phi, phiPrev, r, G, n);
+                              unsigned long long* bad, hipStream_t s) {
+  hipLaunchKernelGGL(learn_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, phi, phiPrev, r, G, n,
+                     bad);
   return hipGetLastError();
 }
 

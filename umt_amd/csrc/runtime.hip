@@ -170,7 +170,7 @@ struct LearnState {
   double *pp = nullptr, *d = nullptr, *dp = nullptr, *rel = nullptr, *ratio = nullptr;
   double *ta = nullptr, *tb = nullptr, *tpart = nullptr;
   unsigned long long* maxRel = nullptr;
-  unsigned long long* scratchBits = nullptr;
+  unsigned long long* scratchBits = nullptr;  // [0] zone-average max, [1] non-finite correction flag
   float* X = nullptr;
   float* r = nullptr;
   struct Pending {
@@ -1344,6 +1344,7 @@ int learn_prepare(umt_ctx* x) {
   std::vector<double> st(x->sigtHost);
   for (double& v : st) v += x->tau;
   HIP_TRY(hipMemcpyAsync(L.sigt64, st.data(), GZ * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(umt::launch_fill_zero<double>((double*)L.scratchBits, 2, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
 }
@@ -1487,9 +1488,17 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
       HIP_TRY(umt::launch_learn_features(x->phibar, L.pp, haveDp ? L.dp : nullptr, L.sigt64, x->sigs,
                                          x->q, L.zoneNbr, x->G, nGZ, L.rel, L.ratio, L.maxRel, L.X,
                                          x->stream));
-    unsigned long long b[2];
+    unsigned long long b[2], badFlag = 0;
     HIP_TRY(hipMemcpyAsync(b, x->bits, sizeof b, hipMemcpyDeviceToHost, x->stream));
+    if (track && L.scratchBits)
+      HIP_TRY(hipMemcpyAsync(&badFlag, L.scratchBits + 1, sizeof badFlag, hipMemcpyDeviceToHost, x->stream));
     HIP_TRY(hipStreamSynchronize(x->stream));
+    if (badFlag && correcting) {
+      // A correction came out non-finite (those entries were left uncorrected):
+      // stop correcting, plain iteration carries on.
+      correcting = false;
+      L.fellBack = it + 1;
+    }
     double r;
     std::memcpy(&r, &b[1], sizeof r);
     if (!std::isfinite(r)) return fail(34, "source iteration residual is not finite at iteration %d", it + 1);
@@ -1522,7 +1531,8 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
           if (int rc = learn_record(x, res)) return rc;
         if (correcting) {
           HIP_TRY(umt::launch_mlp_forward(L.X, nGZ, L.net->packed, L.net->params, L.r, x->stream));
-          HIP_TRY(umt::launch_learn_apply(x->phi, x->phiPrev, L.r, x->G, nGC, x->stream));
+          HIP_TRY(umt::launch_learn_apply(x->phi, x->phiPrev, L.r, x->G, nGC, L.scratchBits + 1,
+                                          x->stream));
           // The corrected state's zone averages are the "old" values of the next residual.
           HIP_TRY(umt::launch_zone_average(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibarOld,
                                            L.scratchBits, x->stream));
