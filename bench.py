@@ -122,6 +122,7 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
 
 
 APB = {64: 0, 32: 0}  # angles per block of the FP64 / FP32 contexts (0: all)
+GRAPHS = [True]       # replay captured sweeps (off for profiler runs)
 
 
 def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True, history=False):
@@ -142,7 +143,7 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True, history
         ctx.autotune(1)
         if accel in ("gta", "gta+learned"):
             ctx.autotune_grey(2)
-        ctx.set_graphs(True)
+        ctx.set_graphs(GRAPHS[0])
         if accel in ("learned", "gta+learned"):
             ctx.set_learner(net, every=2)
         ctx.set_acceleration(accel)
@@ -211,7 +212,7 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0
             net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
                                             c=c_range, q=(0.0, 1.0), rounds=3, batch=8192,
                                             steps=1500, collect_stride=4 if on == "none" else 1,
-                                            max_iters=max_iters, base=on)
+                                            max_iters=max_iters, base=on, graphs=GRAPHS[0])
             res[f"learned_training_{key}"] = {
                 "seconds": time.perf_counter() - t0,
                 "problem": f"{learn_tiles}^3 tiles, {p.nangles} angles, {G} groups, seeds 1-3 "
@@ -249,7 +250,8 @@ def learned_config4(tol, log):
     p = U.TiledProblem(4, 4, 4, 2, 2)
     G = 2
     t0 = time.perf_counter()
-    net, info = U.train_on_iterates(p, G, seeds=[1, 2, 3, 4], rounds=4, epochs=40, **cfg)
+    net, info = U.train_on_iterates(p, G, seeds=[1, 2, 3, 4], rounds=4, epochs=40,
+                                    graphs=GRAPHS[0], **cfg)
     train_s = time.perf_counter() - t0
     xs = U.draw_cross_sections(20261206, p.nzones, G, **cfg)
     out = {"workload": "4x4x4 tiles (1536 zones), 32 angles, 2 groups, sigma_t logU[0.1,10], "
@@ -299,9 +301,9 @@ def batched_domains(n_domains, seed, log):
         ctx = U.SweepContext(p, g, precision=32)
         ctx.set_sigt(st)
         ctx.set_source(np.repeat(qq, 8, axis=0) / (4 * np.pi))
-        ctx.set_graphs(True)
         k = 4 if g % 4 == 0 else (2 if g % 2 == 0 else 1)
         ctx.set_groups_per_lane(k)
+        ctx.set_graphs(GRAPHS[0])
         ctx.sweep_n(1)
         t1 = time.perf_counter()
         ctx.sweep_n(3)
@@ -309,6 +311,7 @@ def batched_domains(n_domains, seed, log):
         seq_launches += ctx.levels
         ctx.close()
     b = U.SweepBatch(probs, groups, precision=32)
+    b.set_graphs(GRAPHS[0])
     for i, (st, ss, qq) in enumerate(xs):
         b.set_cross_sections(i, st, ss, qq)
     b.set_layout(4, 32)
@@ -358,6 +361,8 @@ def main():
                     help="uneven domains in the batched-sweep section (0: skip it)")
     ap.add_argument("--learn-tiles", type=int, default=4,
                     help="train the learned correction on n^3-tile problems (0: skip it)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every sweep eagerly instead of replaying captured graphs")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the HBM copy probe (for rocprofv3 --pmc calibration)")
     args = ap.parse_args()
@@ -389,6 +394,7 @@ def main():
     setup_s = time.perf_counter() - t0
     c_range = tuple(args.c)
     APB[64], APB[32] = args.angles_per_block
+    GRAPHS[0] = not args.no_graphs
     st, ss, qq = U.draw_cross_sections(args.seed, p.nzones, G, (1e-2, 1e3), c_range, (0.0, 1.0))
     S = np.repeat(qq, 8, axis=0) / (4.0 * np.pi)
     unknowns = p.unknowns(G)
