@@ -183,7 +183,8 @@ def rel_error(phi, ref):
     return float(np.max(np.abs(phi - ref) / np.maximum(np.abs(ref), 1e-300)))
 
 
-def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0.95)):
+def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0.95),
+                     gta_only=False):
     """Plain FP64 source iteration (the reference method on the GPU) against
     GTA and the learned correction, FP64 and FP32->FP64."""
     import umt_amd as U
@@ -207,7 +208,7 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0
     nets = []
     if learn_tiles:
         small = U.TiledProblem(learn_tiles, learn_tiles, learn_tiles, p.npolar, p.nazimuthal)
-        for key, on in (("plain", "none"), ("gta", "gta")):
+        for key, on in ((("gta", "gta"),) if gta_only else (("plain", "none"), ("gta", "gta"))):
             t0 = time.perf_counter()
             net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
                                             c=c_range, q=(0.0, 1.0), rounds=3, batch=8192,
@@ -221,9 +222,12 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0
                 "losses": info["losses"], "samples": info["samples"]}
             log(f"training on {key} iterates: {res[f'learned_training_{key}']}")
             nets.append(net)
-        cases += [("fp64_learned", "learned", False, nets[0]),
-                  ("fp32_then_fp64_learned", "learned", True, nets[0]),
-                  ("fp64_gta_learned", "gta+learned", False, nets[1]),
+        if gta_only:
+            nets.insert(0, None)
+        if not gta_only:
+            cases += [("fp64_learned", "learned", False, nets[0]),
+                      ("fp32_then_fp64_learned", "learned", True, nets[0])]
+        cases += [("fp64_gta_learned", "gta+learned", False, nets[1]),
                   ("fp32_then_fp64_gta_learned", "gta+learned", True, nets[1])]
     for name, accel, mixed, net in cases:
         r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, net=net)
@@ -236,7 +240,8 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0
         res[name] = r
         log(f"{name}: {r}")
     for net in nets:
-        net.close()
+        if net is not None:
+            net.close()
     return res
 
 
@@ -361,6 +366,9 @@ def main():
                     help="uneven domains in the batched-sweep section (0: skip it)")
     ap.add_argument("--learn-tiles", type=int, default=4,
                     help="train the learned correction on n^3-tile problems (0: skip it)")
+    ap.add_argument("--learned-with-gta-only", action="store_true",
+                    help="run the learned correction only on top of GTA (for workloads where "
+                         "iteration without GTA does not converge within --max-iters)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every sweep eagerly instead of replaying captured graphs")
     ap.add_argument("--calibrate", action="store_true",
@@ -462,9 +470,10 @@ def main():
 
     if not args.no_converge:
         out["time_to_converge"] = time_to_converge(
-            p, G, (st, ss, qq), args.tol, args.max_iters, args.learn_tiles, log, c_range)
+            p, G, (st, ss, qq), args.tol, args.max_iters, args.learn_tiles, log, c_range,
+            args.learned_with_gta_only)
 
-        if args.learn_tiles:
+        if args.learn_tiles and not args.learned_with_gta_only:
             out["learned_config4"] = learned_config4(args.tol, log)
 
     if args.batch_domains:

@@ -16,6 +16,13 @@ namespace {
 constexpr double kWtiso = 0.07957747154594767;  // 1/(4 pi)
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// max that keeps a NaN once one is seen (the host checks for it).
+__device__ inline double nan_max(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return fmax(a, b);
+}
 }  // namespace
 
 // Scattering-rate residual after a sweep: sigma_s (phi_half - phi_prev)
@@ -95,13 +102,23 @@ __global__ void gta_wdot_partial(const double* __restrict__ x, const double* __r
   }
 }
 
-__global__ void gta_reduce_partials(const double* __restrict__ partial, int nblocks, int G,
-                                    double* __restrict__ out) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
+// One block per group: each thread sums a fixed stride of the block
+// partials, then a fixed-shape tree in LDS, so the result has the same bits
+// on every run.
+__global__ __launch_bounds__(256) void gta_reduce_partials(const double* __restrict__ partial,
+                                                           int nblocks, int G,
+                                                           double* __restrict__ out) {
+  __shared__ double red[256];
+  const int g = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += partial[(long)b * G + g];
-  out[g] = s;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[(long)b * G + g];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[g] = red[0];
 }
 
 // Grey convergence measure (rt/GTASolver.F90:318-356 per group): zone
@@ -144,7 +161,7 @@ __global__ void gta_zone_error(const double* __restrict__ F, const double* __res
     double m = 0.0, a = 0.0, b = 0.0;
     for (int t = 0; t < (int)blockDim.x; ++t) {
       const double v = smax[t];
-      m = (v != v || m != m) ? v : fmax(m, v);
+      m = nan_max(m, v);
       a += serr[t];
       b += sphi[t];
     }
@@ -154,19 +171,32 @@ __global__ void gta_zone_error(const double* __restrict__ F, const double* __res
   }
 }
 
-__global__ void gta_reduce_error(const double* __restrict__ partial, int nblocks,
-                                 double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void gta_reduce_error(const double* __restrict__ partial,
+                                                        int nblocks, double* __restrict__ out) {
+  __shared__ double sm[256], sa[256], sb[256];
   double m = 0.0, a = 0.0, b = 0.0;
-  for (int k = 0; k < nblocks; ++k) {
-    const double v = partial[3 * k];
-    m = (v != v || m != m) ? v : fmax(m, v);
+  for (int k = threadIdx.x; k < nblocks; k += blockDim.x) {
+    m = nan_max(m, partial[3 * k]);
     a += partial[3 * k + 1];
     b += partial[3 * k + 2];
   }
-  out[0] = m;
-  out[1] = a;
-  out[2] = b;
+  sm[threadIdx.x] = m;
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sm[threadIdx.x] = nan_max(sm[threadIdx.x], sm[threadIdx.x + w]);
+      sa[threadIdx.x] += sa[threadIdx.x + w];
+      sb[threadIdx.x] += sb[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sm[0];
+    out[1] = sa[0];
+    out[2] = sb[0];
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -202,8 +232,7 @@ hipError_t launch_gta_wdot(const double* x, const double* y, const double* sigs,
   const int bs = gta_dot_block(G);
   hipLaunchKernelGGL(gta_wdot_partial, dim3(nblocks), dim3(bs), bs * sizeof(double), s, x, y, sigs,
                      vol, G, n, partial);
-  hipLaunchKernelGGL(gta_reduce_partials, dim3(blocks_for(G, 64)), dim3(64), 0, s, partial, nblocks,
-                     G, out);
+  hipLaunchKernelGGL(gta_reduce_partials, dim3(G), dim3(256), 0, s, partial, nblocks, G, out);
   return hipGetLastError();
 }
 
@@ -212,7 +241,7 @@ hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const d
                                  double* partial, double* out, hipStream_t s) {
   hipLaunchKernelGGL(gta_zone_error, dim3(nblocks), dim3(256), 0, s, F, phiHalf, vol, volZ, G, nzg,
                      pzOld, partial);
-  hipLaunchKernelGGL(gta_reduce_error, dim3(1), dim3(64), 0, s, partial, nblocks, out);
+  hipLaunchKernelGGL(gta_reduce_error, dim3(1), dim3(256), 0, s, partial, nblocks, out);
   return hipGetLastError();
 }
 

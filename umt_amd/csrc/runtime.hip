@@ -1622,7 +1622,7 @@ int umt_ctx_time_sweeps(umt_ctx* x, int warmup, int reps, double* msSweep, doubl
 namespace {
 
 constexpr float kAdamBeta1 = 0.9f, kAdamBeta2 = 0.999f, kAdamEps = 1e-8f;
-constexpr int kTrainStepsPerWave = 4;
+constexpr int kTrainStepsPerWave = 1;  // 32 samples per wave: a batch spreads over more CUs
 
 // splitmix64 stream of uniform doubles in [0, 1).
 double next_unit(uint64_t& state) {
