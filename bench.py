@@ -229,14 +229,14 @@ def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0
                       ("fp32_then_fp64_learned", "learned", True, nets[0])]
         cases += [("fp64_gta_learned", "gta+learned", False, nets[1]),
                   ("fp32_then_fp64_gta_learned", "gta+learned", True, nets[1])]
+    ref_phi, ref_name = (phi64, "fp64_plain") if base["converged"] else (None, "fp64_gta")
     for name, accel, mixed, net in cases:
         r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, net=net)
-        r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
         r["speedup_vs_fp64_plain"] = base_s / r["seconds"]
-        if base["converged"]:
-            r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
-        else:
-            r.pop("max_rel_phi_error_vs_fp64_plain", None)
+        if ref_phi is None and name == "fp64_gta" and r["converged"]:
+            ref_phi = phi  # plain iteration did not converge: FP64 GTA is the reference answer
+        if ref_phi is not None and name != ref_name:
+            r[f"max_rel_phi_error_vs_{ref_name}"] = rel_error(phi, ref_phi)
         res[name] = r
         log(f"{name}: {r}")
     for net in nets:
