@@ -107,6 +107,15 @@ int umt_problem_quadrature(const umt_problem *p, double *omega, double *weight);
 int umt_problem_schedule(const umt_problem *p, int *nHyperPlanes, int *zonesInPlane, int *nextZ,
                          int *nextC);
 
+/* The S_N schedule's cycle lists (ASet%numCycles, ASet%cycleList of
+ * mods/AngleSet_mod.F90:80-82, built as rt/rtorder.F90:62-90 builds them):
+ * *totalCycles, numCycles(A), cycleList(totalCycles) with 1-based corners,
+ * angle by angle. Any output may be NULL (query totalCycles first). Meshes
+ * whose schedule has no dependency cycles have totalCycles == 0; zones with
+ * cyclic corner dependencies appear negated in nextZ
+ * (snac/snnext.F90:164-168). */
+int umt_problem_cycles(const umt_problem *p, int *totalCycles, int *numCycles, int *cycleList);
+
 /* Grey-acceleration quadrature (level-symmetric S2, rt/quadxyz.F90 order 2,
  * the GTA order of driver/test_driver.cc:1206) and its schedule. Faces that
  * contain an S2 direction (the tile's diagonal faces) carry no dependency
@@ -130,7 +139,9 @@ int umt_draw_cross_sections(uint64_t seed, int nzones, int groups, double sigt_l
 typedef struct umt_ctx umt_ctx;
 
 /* From Teton-layout arrays (as a Fortran host holds them). zonesInPlane is
- * the concatenation over angles of each angle's nHyperPlanes entries. */
+ * the concatenation over angles of each angle's nHyperPlanes entries.
+ * nextZ may hold cyclic zones (negative entries, snac/snnext.F90:164-168);
+ * sweeps then need the schedule's cycle lists (umt_ctx_set_cycles). */
 int umt_ctx_create(int nzones, int ncorner, int nbelem, int groups, int nangles, int precision,
                    const int *numCorner, const int *cOffSet, const int *nCFaces, const int *cFP,
                    const int *cEZ, const double *A_fp, const double *A_ez, const double *Volume,
@@ -139,6 +150,20 @@ int umt_ctx_create(int nzones, int ncorner, int nbelem, int groups, int nangles,
                    umt_ctx **out);
 int umt_ctx_create_from_problem(const umt_problem *p, int groups, int precision, umt_ctx **out);
 void umt_ctx_destroy(umt_ctx *ctx);
+
+/* The schedule's cycle lists: numCycles(A) and cycleList (1-based corners,
+ * angle by angle; ASet%numCycles / ASet%cycleList, mods/AngleSet_mod.F90:80-82,
+ * rt/rtorder.F90:62-90). Each sweep loads the listed corners' psi from the
+ * previous sweep of their angle before the angle is swept and stores them
+ * after it, as SetSweep.F90:149,155 do with initFromCycleList /
+ * updateCycleList (control/constructDynMemory.F90:118-189); a cyclic zone
+ * is solved once from its own corners' previous values
+ * (SweepUCBxyz.F90:281-295). The lagged values start at zero;
+ * umt_ctx_set_psi_prev reloads them from the previous cycle's Psi
+ * (initCyclePsi, control/constructDynMemory.F90:75-114) and
+ * umt_ctx_copy_phi carries them between contexts with the same lists.
+ * umt_ctx_create_from_problem sets them from the problem's schedule. */
+int umt_ctx_set_cycles(umt_ctx *ctx, const int *numCycles, const int *cycleList);
 
 /* Sigt(G,Z) = total cross section seen by the sweep (setTotalOpacity with tau = 0). */
 int umt_ctx_set_sigt(umt_ctx *ctx, const double *sigt);
@@ -243,7 +268,8 @@ int umt_ctx_set_graphs(umt_ctx *ctx, int on);
 int umt_ctx_set_angles_per_block(umt_ctx *ctx, int n);
 int umt_ctx_get_angles_per_block(const umt_ctx *ctx, int *n, int *blocks);
 
-/* dst's phi = src's phi, device to device (same groups and corners). */
+/* dst's phi = src's phi, device to device (same groups and corners), and
+ * the lagged cycle-list psi when both have the same cycle lists. */
 int umt_ctx_copy_phi(umt_ctx *dst, const umt_ctx *src);
 
 /* Autotuner: times reps sweeps with each supported groups-per-lane value on

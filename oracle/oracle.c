@@ -205,13 +205,188 @@ int oracle_schedule_angle_tol(int nzones, int ncorner, const int *numCorner, con
                               const double *A_fp, const double *A_ez, const double *omega,
                               double parTol, int *nextZ, int *nextC, int *zonesInPlane,
                               int *nHyperPlanesOut) {
-  (void)ncorner;
-  int *needZ = calloc(nzones, sizeof(int));
+  int *cycleList = malloc(sizeof(int) * (size_t)ncorner);
+  int numCycles = 0;
+  int rc = oracle_schedule_angle_cyc(nzones, ncorner, numCorner, cOffSet, zoneFaces, nCFaces, cEZ,
+                                     CToFace, zoneOpp, faceOpp, A_fp, A_ez, omega, parTol, nextZ,
+                                     nextC, zonesInPlane, nHyperPlanesOut, cycleList, &numCycles);
+  free(cycleList);
+  if (rc == 0 && numCycles > 0) rc = 2;
+  return rc;
+}
+
+/* State shared by snnext and the routines it calls (their argument lists). */
+typedef struct {
+  int nzones, ncorner;
+  const int *numCorner, *cOffSet, *zoneFaces, *zoneOpp, *faceOpp;
+  int *needZ;
+  char *exitFace;    /* exitFace(MAXFACES, nzones) */
+  char *onCycleList; /* (nzones) */
+  int *cycleList;    /* 1-based corners, capacity cycleCap */
+  int meshCycles, cycleCap, overflow;
+  /* cyclebreaker / sccsearch work arrays */
+  int *dfnum, *lowlink, *stack, *tempList, *zoneBreakList;
+  char *isNew, *onstack;
+  int ncount, stackindex, nBreaks;
+} SchedState;
+
+/* The corners of one zone onto the cycle list (snac/fixZone.F90:24-30 and the
+ * same loop in snneed, findseeds and sccsearch). */
+static void add_zone_corners(SchedState *st, int zone) {
+  for (int c = 1; c <= st->numCorner[zone - 1]; ++c) {
+    if (st->meshCycles >= st->cycleCap) { st->overflow = 1; return; }
+    st->cycleList[st->meshCycles++] = st->cOffSet[zone - 1] + c;
+  }
+}
+
+/* snac/sccsearch.F90:54-183 (Tarjan's strongly connected components; on a
+ * component of more than one zone, break the links into its root). */
+static void sccsearch(SchedState *st, int zone) {
+  st->ncount += 1;
+  st->dfnum[zone - 1] = st->ncount;
+  st->lowlink[zone - 1] = st->ncount;
+  st->isNew[zone - 1] = 0;
+  st->stackindex += 1;
+  st->stack[st->stackindex - 1] = zone;
+  st->onstack[zone - 1] = 1;
+
+  const int nFaces = st->zoneFaces[zone - 1];
+  for (int face = 1; face <= nFaces; ++face) { /* :96-134 */
+    if (I2(st->exitFace, face, zone, MAXFACES)) {
+      const int zone2 = I2(st->zoneOpp, face, zone, MAXFACES);
+      if (zone2 > 0) {
+        if (st->isNew[zone2 - 1]) {
+          sccsearch(st, zone2);
+          if (st->lowlink[zone2 - 1] < st->lowlink[zone - 1])
+            st->lowlink[zone - 1] = st->lowlink[zone2 - 1];
+        } else {
+          if (st->dfnum[zone2 - 1] < st->dfnum[zone - 1] && st->onstack[zone2 - 1] &&
+              st->lowlink[zone2 - 1] < st->lowlink[zone - 1])
+            st->lowlink[zone - 1] = st->lowlink[zone2 - 1];
+        }
+      }
+    }
+  }
+
+  if (st->lowlink[zone - 1] == st->dfnum[zone - 1]) { /* :138-181 */
+    int zone2 = st->stack[st->stackindex - 1];
+    st->stackindex -= 1;
+    st->onstack[zone2 - 1] = 0;
+    if (zone2 != zone) {
+      int cyclesize = 0;
+      while (zone2 != zone) {
+        cyclesize += 1;
+        st->tempList[cyclesize - 1] = zone2;
+        zone2 = st->stack[st->stackindex - 1];
+        st->stackindex -= 1;
+      }
+      cyclesize += 1;
+      st->tempList[cyclesize - 1] = zone2;
+      st->onstack[st->tempList[0] - 1] = 1;
+
+      const int lowlinkZ = st->tempList[cyclesize - 1];
+      const int nF = st->zoneFaces[lowlinkZ - 1];
+      for (int face = 1; face <= nF; ++face) {
+        const int zoneBreak = I2(st->zoneOpp, face, lowlinkZ, MAXFACES);
+        const int faceBreak = I2(st->faceOpp, face, lowlinkZ, MAXFACES);
+        if (zoneBreak > 0) {
+          if (st->onstack[zoneBreak - 1] && I2(st->exitFace, faceBreak, zoneBreak, MAXFACES)) {
+            if (!st->onCycleList[zoneBreak - 1]) {
+              add_zone_corners(st, zoneBreak);
+              st->onCycleList[zoneBreak - 1] = 1;
+            }
+            st->needZ[lowlinkZ - 1] -= 1;
+            I2(st->exitFace, faceBreak, zoneBreak, MAXFACES) = 0;
+            if (st->needZ[lowlinkZ - 1] == 0) {
+              st->nBreaks += 1;
+              st->zoneBreakList[st->nBreaks - 1] = lowlinkZ;
+            }
+          }
+        }
+      }
+      for (int i = 1; i <= cyclesize; ++i) st->onstack[st->tempList[i - 1] - 1] = 0;
+    }
+  }
+}
+
+/* snac/cyclebreaker.F90:50-150. Returns addedZones (> 0) or a negative error. */
+static int cyclebreaker(SchedState *st, int ndoneZ, int *nextZone, int *listZone) {
+  const int nzones = st->nzones;
+  const int ngraph = nzones - ndoneZ;
+  int *listZ = malloc(sizeof(int) * (size_t)(ngraph > 0 ? ngraph : 1));
+  for (int z = 0; z < nzones; ++z) {
+    st->isNew[z] = 1;
+    st->onstack[z] = 0;
+    st->dfnum[z] = 0;
+    st->lowlink[z] = 0;
+  }
+  st->nBreaks = 0;
+  st->ncount = 0;
+  st->stackindex = 0;
+  int nleft = 0;
+  for (int zone = 1; zone <= nzones; ++zone) {
+    if (st->needZ[zone - 1] == 0) st->isNew[zone - 1] = 0;
+    else {
+      if (nleft < ngraph) listZ[nleft] = zone;
+      nleft += 1;
+    }
+  }
+  int rc = 0;
+  if (nleft != ngraph) { rc = -1; goto out; }
+  for (int i = 1; i <= ngraph; ++i) {
+    const int zone = listZ[i - 1];
+    if (st->isNew[zone - 1]) sccsearch(st, zone);
+  }
+  if (st->nBreaks == 0) { rc = -2; goto out; }
+  int addedZones = 0;
+  for (int i = 1; i <= st->nBreaks; ++i) {
+    const int zone = st->zoneBreakList[i - 1];
+    if (st->needZ[zone - 1] == 0) {
+      *nextZone += 1;
+      addedZones += 1;
+      listZone[*nextZone - 1] = zone;
+    } else if (st->needZ[zone - 1] < 0) {
+      rc = -3;
+      goto out;
+    }
+  }
+  rc = addedZones == 0 ? -4 : addedZones;
+out:
+  free(listZ);
+  return rc;
+}
+
+int oracle_schedule_angle_cyc(int nzones, int ncorner, const int *numCorner, const int *cOffSet,
+                              const int *zoneFaces, const int *nCFaces, const int *cEZ,
+                              const int *CToFace, const int *zoneOpp, const int *faceOpp,
+                              const double *A_fp, const double *A_ez, const double *omega,
+                              double parTol, int *nextZ, int *nextC, int *zonesInPlane,
+                              int *nHyperPlanesOut, int *cycleList, int *numCyclesOut) {
+  SchedState st = {0};
+  st.nzones = nzones;
+  st.ncorner = ncorner;
+  st.numCorner = numCorner;
+  st.cOffSet = cOffSet;
+  st.zoneFaces = zoneFaces;
+  st.zoneOpp = zoneOpp;
+  st.faceOpp = faceOpp;
+  st.needZ = calloc(nzones, sizeof(int));
+  st.exitFace = calloc((size_t)MAXFACES * nzones, 1);
+  st.onCycleList = calloc(nzones, 1);
+  /* fixZone appends without the onCycleList test, so a zone can appear twice */
+  st.cycleCap = 2 * ncorner + MAXFACES * MAXCORNER;
+  st.cycleList = malloc(sizeof(int) * (size_t)st.cycleCap);
+  st.dfnum = calloc(nzones, sizeof(int));
+  st.lowlink = calloc(nzones, sizeof(int));
+  st.stack = calloc(nzones, sizeof(int));
+  st.tempList = calloc(nzones, sizeof(int));
+  st.zoneBreakList = calloc(nzones, sizeof(int));
+  st.isNew = calloc(nzones, 1);
+  st.onstack = calloc(nzones, 1);
   int *listZone = calloc(nzones, sizeof(int));
-  char *exitFace = calloc((size_t)MAXFACES * nzones, 1);
-  char *onCycleList = calloc(nzones, 1);
+  char *badZone = calloc(nzones, 1);
   char *doneZ = calloc(nzones, 1);
-  int meshCycles = 0, rc = 0;
+  int rc = 0;
 
   /* snac/snneed.F90:97-191 */
   for (int zone = 1; zone <= nzones; ++zone) {
@@ -236,32 +411,57 @@ int oracle_schedule_angle_tol(int nzones, int ncorner, const int *numCorner, con
       if (zOpp > zone) {
         const int fOpp = I2(faceOpp, face, zone, MAXFACES);
         if (afpm_Face[face - 1] < 0.0) {
-          needZ[zone - 1] += 1;
-          I2(exitFace, fOpp, zOpp, MAXFACES) = 1;
-          if (nExit[face - 1] > 0 && !onCycleList[zone - 1]) {
-            meshCycles += nCorner;
-            onCycleList[zone - 1] = 1;
+          st.needZ[zone - 1] += 1;
+          I2(st.exitFace, fOpp, zOpp, MAXFACES) = 1;
+          if (nExit[face - 1] > 0 && !st.onCycleList[zone - 1]) {
+            add_zone_corners(&st, zone);
+            st.onCycleList[zone - 1] = 1;
           }
         } else if (afpm_Face[face - 1] > 0.0) {
-          needZ[zOpp - 1] += 1;
-          I2(exitFace, face, zone, MAXFACES) = 1;
-          if (nInc[face - 1] > 0 && !onCycleList[zOpp - 1]) {
-            meshCycles += numCorner[zOpp - 1];
-            onCycleList[zOpp - 1] = 1;
+          st.needZ[zOpp - 1] += 1;
+          I2(st.exitFace, face, zone, MAXFACES) = 1;
+          if (nInc[face - 1] > 0 && !st.onCycleList[zOpp - 1]) {
+            add_zone_corners(&st, zOpp);
+            st.onCycleList[zOpp - 1] = 1;
           }
         }
       }
     }
   }
-  if (meshCycles > 0) { rc = 2; goto done; }
 
-  /* snac/findseeds.F90:58-66 */
+  /* snac/findseeds.F90:58-114 */
   int newZones = 0;
   for (int zone = 1; zone <= nzones; ++zone)
-    if (needZ[zone - 1] == 0) listZone[newZones++] = zone;
-  if (newZones == 0) { rc = 3; goto done; }
+    if (st.needZ[zone - 1] == 0) listZone[newZones++] = zone;
+  if (newZones == 0) {
+    int minNeed = nzones, zoneID = 0;
+    for (int zone = 1; zone <= nzones; ++zone) {
+      int boundaryZone = 0; /* aux/setOppositeFace.F90:41-45 */
+      for (int face = 1; face <= zoneFaces[zone - 1]; ++face)
+        if (I2(zoneOpp, face, zone, MAXFACES) < 0) boundaryZone = 1;
+      if (boundaryZone && st.needZ[zone - 1] < minNeed) {
+        zoneID = zone;
+        minNeed = st.needZ[zone - 1];
+      }
+    }
+    if (zoneID == 0) { rc = 3; goto done; }
+    newZones = 1;
+    listZone[0] = zoneID;
+    st.needZ[zoneID - 1] = 0;
+    for (int face = 1; face <= zoneFaces[zoneID - 1]; ++face) {
+      if (!I2(st.exitFace, face, zoneID, MAXFACES)) {
+        const int zOpp = I2(zoneOpp, face, zoneID, MAXFACES);
+        const int fOpp = I2(faceOpp, face, zoneID, MAXFACES);
+        if (zOpp > 0) {
+          add_zone_corners(&st, zOpp);
+          I2(st.exitFace, fOpp, zOpp, MAXFACES) = 0;
+          st.onCycleList[zOpp - 1] = 1;
+        }
+      }
+    }
+  }
 
-  /* snac/getDownStreamData.F90:88-149 */
+  /* snac/getDownStreamData.F90:88-158 */
   for (int zone = 1; zone <= nzones; ++zone) {
     int need[MAXCORNER] = {0}, nDSC[MAXCORNER] = {0}, DownStreamC[MAXCORNER][MAXCF];
     const int nCorner = numCorner[zone - 1], c0 = cOffSet[zone - 1];
@@ -281,20 +481,22 @@ int oracle_schedule_angle_tol(int nzones, int ncorner, const int *numCorner, con
         }
       }
     }
-    int badZone = 0;
     for (int i = 1; i <= nCorner; ++i) {
       int c = 1; /* minloc: first index of the minimum */
       for (int k = 2; k <= nCorner; ++k)
         if (need[k - 1] < need[c - 1]) c = k;
-      if (need[c - 1] != 0) badZone = 1;
+      if (need[c - 1] != 0) badZone[zone - 1] = 1;
       nextC[c0 + i - 1] = c;
       for (int cf = 0; cf < nDSC[c - 1]; ++cf) need[DownStreamC[c - 1][cf] - 1] -= 1;
       need[c - 1] = 99;
     }
-    if (badZone) { rc = 4; goto done; }
+    if (badZone[zone - 1]) {
+      for (int i = 1; i <= nCorner; ++i) nextC[c0 + i - 1] = i;
+      add_zone_corners(&st, zone); /* snac/fixZone.F90 */
+    }
   }
 
-  /* snac/snnext.F90:103-195 */
+  /* snac/snnext.F90:103-213 */
   {
     int ndoneZ = 0, nextZone = 0, lastZone = 0, nHyperPlanes = 0;
     for (;;) {
@@ -307,32 +509,43 @@ int oracle_schedule_angle_tol(int nzones, int ncorner, const int *numCorner, con
         ndoneZ += 1;
         doneZ[zone - 1] = 1;
         for (int face = 1; face <= zoneFaces[zone - 1]; ++face) {
-          if (I2(exitFace, face, zone, MAXFACES)) {
+          if (I2(st.exitFace, face, zone, MAXFACES)) {
             const int Zexit = I2(zoneOpp, face, zone, MAXFACES);
             if (Zexit > 0 && !doneZ[Zexit - 1]) {
-              needZ[Zexit - 1] -= 1;
-              if (needZ[Zexit - 1] == 0) {
+              st.needZ[Zexit - 1] -= 1;
+              if (st.needZ[Zexit - 1] == 0) {
                 nextZone += 1;
                 addedZones += 1;
                 listZone[nextZone - 1] = Zexit;
-              } else if (needZ[Zexit - 1] < 0) {
+              } else if (st.needZ[Zexit - 1] < 0) {
                 rc = 5;
                 goto done;
               }
             }
           }
         }
-        nextZ[ndoneZ - 1] = zone;
+        nextZ[ndoneZ - 1] = badZone[zone - 1] ? -zone : zone;
       }
       lastZone += newZones;
       if (lastZone == nzones) break;
-      if (addedZones == 0) { rc = 6; goto done; } /* cyclebreaker not restated */
-      newZones = addedZones;
+      if (addedZones > 0) {
+        newZones = addedZones;
+      } else {
+        addedZones = cyclebreaker(&st, ndoneZ, &nextZone, listZone);
+        if (addedZones < 0) { rc = 6; goto done; }
+        newZones = addedZones;
+      }
     }
     *nHyperPlanesOut = nHyperPlanes;
   }
+  if (st.overflow || st.meshCycles > ncorner) { rc = 7; goto done; }
+  memcpy(cycleList, st.cycleList, sizeof(int) * (size_t)st.meshCycles);
+  *numCyclesOut = st.meshCycles;
 done:
-  free(needZ); free(listZone); free(exitFace); free(onCycleList); free(doneZ);
+  free(st.needZ); free(st.exitFace); free(st.onCycleList); free(st.cycleList);
+  free(st.dfnum); free(st.lowlink); free(st.stack); free(st.tempList); free(st.zoneBreakList);
+  free(st.isNew); free(st.onstack);
+  free(listZone); free(badZone); free(doneZ);
   return rc;
 }
 
@@ -344,8 +557,8 @@ static void sweep_angle_tol(int nzones, int ncorner, int nbelem, int Groups, con
                             const double *Volume, const double *STotal, const double *Sigt,
                             double tau, const double *Psi, const double *omega, double quadwt,
                             int nHyperPlanes, const int *zonesInPlane, const int *nextZ,
-                            const int *nextC, double parTol, double *Psi1, double *PsiB,
-                            double *Phi) {
+                            const int *nextC, double parTol, int numCycles, const int *cycleList,
+                            double *cyclePsi, double *Psi1, double *PsiB, double *Phi) {
   (void)nzones;
   const double fouralpha = 1.82;
   const int G = Groups;
@@ -359,6 +572,10 @@ static void sweep_angle_tol(int nzones, int ncorner, int nbelem, int Groups, con
 
   for (int ib = 1; ib <= nbelem; ++ib)
     for (int g = 1; g <= G; ++g) I2(Psi1, g, ncorner + ib, G) = I2(PsiB, g, ib, G);
+  /* initFromCycleList (control/constructDynMemory.F90:118-152): corners read
+   * before they are computed hold the previous sweep's values. */
+  for (int m = 1; m <= numCycles; ++m)
+    for (int g = 1; g <= G; ++g) I2(Psi1, g, cycleList[m - 1], G) = I2(cyclePsi, g, m, G);
 
   int ndoneZ = 0;
   for (int hyperPlane = 1; hyperPlane <= nHyperPlanes; ++hyperPlane) {
@@ -501,6 +718,9 @@ static void sweep_angle_tol(int nzones, int ncorner, int nbelem, int Groups, con
     }
     ndoneZ += nz;
   }
+  /* updateCycleList (control/constructDynMemory.F90:156-189) */
+  for (int m = 1; m <= numCycles; ++m)
+    for (int g = 1; g <= G; ++g) I2(cyclePsi, g, m, G) = I2(Psi1, g, cycleList[m - 1], G);
   free(psi_opp); free(SigtVol); free(src); free(Q); free(psifp);
 }
 
@@ -513,7 +733,20 @@ void oracle_sweep_angle(int nzones, int ncorner, int nbelem, int Groups, const i
                         const int *nextC, double *Psi1, double *PsiB, double *Phi) {
   sweep_angle_tol(nzones, ncorner, nbelem, Groups, numCorner, cOffSet, nCFacesArray, cFP, cEZ, A_fp,
                   A_ez, Volume, STotal, Sigt, tau, Psi, omega, quadwt, nHyperPlanes, zonesInPlane,
-                  nextZ, nextC, 0.0, Psi1, PsiB, Phi);
+                  nextZ, nextC, 0.0, 0, NULL, NULL, Psi1, PsiB, Phi);
+}
+
+void oracle_sweep_angle_cyc(int nzones, int ncorner, int nbelem, int Groups, const int *numCorner,
+                            const int *cOffSet, const int *nCFacesArray, const int *cFP,
+                            const int *cEZ, const double *A_fp, const double *A_ez,
+                            const double *Volume, const double *STotal, const double *Sigt,
+                            double tau, const double *Psi, const double *omega, double quadwt,
+                            int nHyperPlanes, const int *zonesInPlane, const int *nextZ,
+                            const int *nextC, int numCycles, const int *cycleList,
+                            double *cyclePsi, double *Psi1, double *PsiB, double *Phi) {
+  sweep_angle_tol(nzones, ncorner, nbelem, Groups, numCorner, cOffSet, nCFacesArray, cFP, cEZ, A_fp,
+                  A_ez, Volume, STotal, Sigt, tau, Psi, omega, quadwt, nHyperPlanes, zonesInPlane,
+                  nextZ, nextC, 0.0, numCycles, cycleList, cyclePsi, Psi1, PsiB, Phi);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -537,10 +770,28 @@ void oracle_sweep_all_tol(int nzones, int ncorner, int nbelem, int G, int A, con
                           const double *weight, const int *nHyperPlanes, const int *zonesInPlane,
                           const int *nextZ, const int *nextC, int nthreads, double parTol,
                           double *PsiB, double *Phi) {
+  oracle_sweep_all_cyc(nzones, ncorner, nbelem, G, A, numCorner, cOffSet, nCFaces, cFP, cEZ, A_fp,
+                       A_ez, Volume, STotal, Sigt, omega, weight, nHyperPlanes, zonesInPlane, nextZ,
+                       nextC, nthreads, parTol, NULL, NULL, NULL, PsiB, Phi);
+}
+
+void oracle_sweep_all_cyc(int nzones, int ncorner, int nbelem, int G, int A, const int *numCorner,
+                          const int *cOffSet, const int *nCFaces, const int *cFP, const int *cEZ,
+                          const double *A_fp, const double *A_ez, const double *Volume,
+                          const double *STotal, const double *Sigt, const double *omega,
+                          const double *weight, const int *nHyperPlanes, const int *zonesInPlane,
+                          const int *nextZ, const int *nextC, int nthreads, double parTol,
+                          const int *numCycles, const int *cycleList, double *cyclePsi,
+                          double *PsiB, double *Phi) {
   const size_t nGC = (size_t)G * ncorner, nPsi1 = (size_t)G * (ncorner + nbelem);
   int *planeStart = malloc(sizeof(int) * (A + 1));
+  int *cycleOffSet = malloc(sizeof(int) * (A + 1)); /* rt/rtorder.F90:62-69 */
   planeStart[0] = 0;
-  for (int a = 0; a < A; ++a) planeStart[a + 1] = planeStart[a] + nHyperPlanes[a];
+  cycleOffSet[0] = 0;
+  for (int a = 0; a < A; ++a) {
+    planeStart[a + 1] = planeStart[a] + nHyperPlanes[a];
+    cycleOffSet[a + 1] = cycleOffSet[a] + (numCycles ? numCycles[a] : 0);
+  }
   if (nthreads < 1) nthreads = 1;
   /* One phase-space set per thread, each with its own Phi (Set%Phi), summed
    * afterwards in set order like getPhiTotal (control/getPhiTotal_OMPOL.F90:134-162). */
@@ -560,10 +811,14 @@ void oracle_sweep_all_tol(int nzones, int ncorner, int nbelem, int G, int A, con
 #pragma omp for schedule(static)
 #endif
     for (int a = 0; a < A; ++a) {
+      const int nc = numCycles ? numCycles[a] : 0;
       sweep_angle_tol(nzones, ncorner, nbelem, G, numCorner, cOffSet, nCFaces, cFP, cEZ, A_fp,
                       A_ez, Volume, STotal, Sigt, 0.0, NULL, &omega[3 * (size_t)a], weight[a],
                       nHyperPlanes[a], &zonesInPlane[planeStart[a]], &nextZ[(size_t)a * nzones],
-                      &nextC[(size_t)a * ncorner], parTol, Psi1, &PsiB[(size_t)a * G * nbelem], phiT);
+                      &nextC[(size_t)a * ncorner], parTol, nc,
+                      nc ? &cycleList[cycleOffSet[a]] : NULL,
+                      nc ? &cyclePsi[(size_t)G * cycleOffSet[a]] : NULL, Psi1,
+                      &PsiB[(size_t)a * G * nbelem], phiT);
     }
     free(Psi1);
   }
@@ -574,6 +829,7 @@ void oracle_sweep_all_tol(int nzones, int ncorner, int nbelem, int G, int A, con
   }
   free(phiSets);
   free(planeStart);
+  free(cycleOffSet);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -586,6 +842,23 @@ int oracle_source_iteration(int nzones, int ncorner, int nbelem, int G, int A,
                             const int *zonesInPlane, const int *nextZ, const int *nextC,
                             int nthreads, double tol, int maxIters, double *Phi,
                             double *residual, double *hist) {
+  return oracle_source_iteration_cyc(nzones, ncorner, nbelem, G, A, numCorner, cOffSet, nCFaces,
+                                     cFP, cEZ, A_fp, A_ez, Volume, VolumeZone, Sigt, sigs, q,
+                                     omega, weight, nHyperPlanes, zonesInPlane, nextZ, nextC,
+                                     NULL, NULL, NULL, nthreads, tol, maxIters, Phi, residual,
+                                     hist);
+}
+
+int oracle_source_iteration_cyc(int nzones, int ncorner, int nbelem, int G, int A,
+                                const int *numCorner, const int *cOffSet, const int *nCFaces,
+                                const int *cFP, const int *cEZ, const double *A_fp,
+                                const double *A_ez, const double *Volume, const double *VolumeZone,
+                                const double *Sigt, const double *sigs, const double *q,
+                                const double *omega, const double *weight, const int *nHyperPlanes,
+                                const int *zonesInPlane, const int *nextZ, const int *nextC,
+                                const int *numCycles, const int *cycleList, double *cyclePsi,
+                                int nthreads, double tol, int maxIters, double *Phi,
+                                double *residual, double *hist) {
   const size_t nGC = (size_t)G * ncorner, nGZ = (size_t)G * nzones;
   const double wtiso = 1.0 / (4.0 * M_PI);
   double *STotal = malloc(sizeof(double) * nGC);
@@ -608,9 +881,10 @@ int oracle_source_iteration(int nzones, int ncorner, int nbelem, int G, int A,
           I2(STotal, g, cOffSet[z - 1] + c, G) =
               wtiso * (I2(sigs, g, z, G) * I2(Phi, g, cOffSet[z - 1] + c, G) + I2(q, g, z, G));
     memset(PsiB, 0, sizeof(double) * (size_t)G * nbelem * A); /* vacuum */
-    oracle_sweep_all(nzones, ncorner, nbelem, G, A, numCorner, cOffSet, nCFaces, cFP, cEZ, A_fp,
-                     A_ez, Volume, STotal, Sigt, omega, weight, nHyperPlanes, zonesInPlane, nextZ,
-                     nextC, nthreads, PsiB, Phi);
+    oracle_sweep_all_cyc(nzones, ncorner, nbelem, G, A, numCorner, cOffSet, nCFaces, cFP, cEZ,
+                         A_fp, A_ez, Volume, STotal, Sigt, omega, weight, nHyperPlanes,
+                         zonesInPlane, nextZ, nextC, nthreads, 0.0, numCycles, cycleList, cyclePsi,
+                         PsiB, Phi);
     double mx = 0.0;
     for (int z = 1; z <= nzones; ++z)
       for (int g = 1; g <= G; ++g) {

@@ -104,9 +104,10 @@ def test_ctx_rejects_non_hex_corner_order(prob):
     assert rc == 13 and "corner-face order" in msg
 
 
-def test_ctx_rejects_cyclic_zone_in_schedule(prob):
+def test_ctx_rejects_out_of_range_zone_in_schedule(prob):
+    """Negative nextZ entries are cyclic zones (snnext); beyond -Z they are invalid."""
     nextZ = prob.schedule()["nextZ"].copy()
-    nextZ[3, 10] = -nextZ[3, 10]
+    nextZ[3, 10] = -(prob.nzones + 1)
     rc, msg = _create(prob, _ctx_args(prob, nextZ=nextZ))
     assert rc == 15 and "angle 4" in msg
 
@@ -118,7 +119,7 @@ def test_ctx_rejects_out_of_range_connectivity(prob):
     assert rc == 14
 
 
-def test_dropin_rejects_cycles_and_reflection_without_touching_outputs(prob):
+def test_dropin_rejects_bad_schedule_and_cycle_list_without_touching_outputs(prob):
     """gpu_sweepucbxyz returns through its status code (the reference exits)."""
     p = prob
     G = 2
@@ -129,10 +130,19 @@ def test_dropin_rejects_cycles_and_reflection_without_touching_outputs(prob):
     Phi = np.full((p.ncorner, G), 3.0)
     s = p.schedule()
     saved = s["nextZ"][0, 0]
-    s["nextZ"][0, 0] = -saved
+    s["nextZ"][0, 0] = -(p.nzones + 1)
     try:
         status = U.sweep_ucbxyz(p, 1, S, st, Psi1, PsiB, Phi)
     finally:
         s["nextZ"][0, 0] = saved
     assert status == 45
+    # a cycle list naming a corner outside the mesh
+    s["numCycles"] = np.array([1] + [0] * (p.nangles - 1), np.int32)
+    s["cycleList"] = np.array([p.ncorner + 1], np.int32)
+    try:
+        status = U.sweep_ucbxyz(p, 1, S, st, Psi1, PsiB, Phi)
+    finally:
+        s["numCycles"] = np.zeros(p.nangles, np.int32)
+        s["cycleList"] = np.zeros(0, np.int32)
+    assert status == 42
     assert np.all(Phi == 3.0)

@@ -147,23 +147,39 @@ def test_dropin_per_angle_matches_oracle(small_problem):
             assert np.array_equal(Psi, Psi1[:p.ncorner])
 
 
-def test_dropin_rejects_unsupported_input(small_problem):
-    """A cyclic zone in nextZ must leave outputs untouched and set a status."""
+def test_dropin_cyclic_zone_uses_lagged_corner_values(small_problem):
+    """A zone marked cyclic in nextZ (negative) is solved once from the
+    values its corners hold in Psi1 on entry (SweepUCBxyz.F90:281-295), in
+    the drop-in as in the oracle; the result differs from the ordinary
+    triangular solve."""
     p = small_problem
     G = 2
     st, _, _, S = _inputs(p, G)
-    s = p.schedule()
-    saved = s["nextZ"][0, 3]
-    s["nextZ"][0, 3] = -abs(saved)
-    try:
-        Psi1 = np.zeros((p.ncorner + p.nbelem, G))
-        PsiB = np.zeros((p.nbelem, G))
-        Phi = np.full((p.ncorner, G), 7.0)
-        status = U.sweep_ucbxyz(p, 1, S, st, Psi1, PsiB, Phi)
-    finally:
-        s["nextZ"][0, 3] = saved
-    assert status != 0
-    assert np.all(Phi == 7.0)
+    m, g, q, s = p.mesh(), p.geometry(), p.quadrature(), p.schedule()
+    rng = np.random.default_rng(9)
+    Psi1_in = rng.uniform(0.0, 1.0, (p.ncorner + p.nbelem, G))
+    out = []
+    for mark in (False, True):
+        saved = s["nextZ"][0, 3]
+        if mark:
+            s["nextZ"][0, 3] = -abs(saved)
+        try:
+            Psi1 = Psi1_in.copy()
+            PsiB = np.zeros((p.nbelem, G))
+            Phi = np.zeros((p.ncorner, G))
+            assert U.sweep_ucbxyz(p, 1, S, st, Psi1, PsiB, Phi) == 0
+            ref1 = Psi1_in.copy()
+            refB = np.zeros((p.nbelem, G))
+            refP = np.zeros((p.ncorner, G))
+            nhp = int(s["nHyperPlanes"][0])
+            O.sweep_angle(m, g, S, st, q["omega"][0], q["weight"][0], s["nextZ"][0],
+                          s["nextC"][0], s["zonesInPlane"][:nhp], ref1, refB, refP)
+        finally:
+            s["nextZ"][0, 3] = saved
+        assert rel_err(Phi, refP) < 1e-12
+        assert rel_err(Psi1[:p.ncorner], ref1[:p.ncorner]) < 1e-12
+        out.append(Phi)
+    assert rel_err(out[1], out[0]) > 1e-6
 
 
 def test_fp32_sweep_error_against_fp64_oracle(config1_problem):

@@ -99,9 +99,11 @@ def test_corner_order_is_lexicographically_smallest_topological_order(prob):
             np.testing.assert_array_equal(nextC[8 * z:8 * z + 8] - 1, best)
 
 
-def test_warped_face_is_reported_as_cycle(prob):
-    """Flip one corner-face area so a face has mixed Omega.A signs: the
-    reference puts the zone on the cycle list; the schedule must refuse."""
+def test_warped_face_puts_downstream_zone_on_cycle_list(prob):
+    """Flip one corner-face area so a face has mixed Omega.A signs: snneed
+    (snac/snneed.F90:165-186) keeps the face-summed dependency and puts the
+    downstream zone's corners on the cycle list, since one of its corners
+    feeds the upstream zone."""
     p, m, g, q = prob
     A_fp = g["A_fp"].copy()
     om = q["omega"][0]
@@ -109,10 +111,16 @@ def test_warped_face_is_reported_as_cycle(prob):
         z = c // 8
         for f in range(3):
             face = m["CToFace"][c, f]
+            zo = m["zoneOpp"][z, face - 1]
             # snneed reads the face from its lower-numbered zone
-            if m["zoneOpp"][z, face - 1] > z + 1 and abs(A_fp[c, f] @ om) > 0:
+            if zo > z + 1 and abs(A_fp[c, f] @ om) > 0:
+                out = A_fp[c, f] @ om > 0  # zone z is upstream across this face
                 A_fp[c, f] *= -0.5  # opposite sign, face sum keeps its sign
-                rc, *_ = O.schedule_angle(m, A_fp, g["A_ez"], om)
-                assert rc != 0
+                rc, nextZ, _, _ = O.schedule_angle(m, A_fp, g["A_ez"], om)
+                assert rc == 2  # a cycle list: the plain call refuses
+                rc, nextZ2, _, _, cyc = O.schedule_angle(m, A_fp, g["A_ez"], om, cycles=True)
+                assert rc == 0
+                down = zo - 1 if out else z
+                np.testing.assert_array_equal(cyc, 8 * down + np.arange(1, 9))
                 return
     pytest.fail("no interior corner face found")

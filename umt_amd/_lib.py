@@ -38,6 +38,7 @@ SIGNATURES = {
     "umt_problem_geometry": (_i, [_vp, _pd, _pd, _pd, _pd]),
     "umt_problem_quadrature": (_i, [_vp, _pd, _pd]),
     "umt_problem_schedule": (_i, [_vp, _pi, _pi, _pi, _pi]),
+    "umt_problem_cycles": (_i, [_vp, _pi, _pi, _pi]),
     "umt_draw_cross_sections": (_i, [_u64, _i, _i, _d, _d, _d, _d, _d, _d, _pd, _pd, _pd]),
     "umt_ctx_create": (_i, [_i, _i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
                             _pd, _pd, _pi, _pi, _pi, C.POINTER(_vp)]),
@@ -66,6 +67,7 @@ SIGNATURES = {
     "umt_ctx_set_angles_per_block": (_i, [_vp, _i]),
     "umt_ctx_get_angles_per_block": (_i, [_vp, _pi, _pi]),
     "umt_ctx_copy_phi": (_i, [_vp, _vp]),
+    "umt_ctx_set_cycles": (_i, [_vp, _pi, _pi]),
     "umt_ctx_set_stall_window": (_i, [_vp, _i]),
     "umt_ctx_set_tau": (_i, [_vp, _d]),
     "umt_ctx_set_psi_prev": (_i, [_vp, _pd]),

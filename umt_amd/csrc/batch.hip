@@ -313,6 +313,9 @@ int create_domain(umt_batch* b, const umt_problem* p, int G, Domain& d) {
     return rc;
   d.Z = m.nzones; d.C = m.ncorner; d.nb = m.nbelem; d.G = G; d.A = q.nangles;
   d.Gp = (G + kGroupPad - 1) / kGroupPad * kGroupPad;
+  if (s.totalCycles() > 0)
+    return fail(51, "batch: a domain's schedule has dependency cycles (%ld cycle-list corners); "
+                    "sweep it with its own context", s.totalCycles());
   if (int rc = domain_levels(s, d.Z, d.A, d)) return rc;
   const size_t rs = b->rsize(), C = d.C, Z = d.Z, Gp = d.Gp;
   int rc;

@@ -53,8 +53,10 @@ struct Schedule {
   std::vector<int> zonesInPlane;   // concatenated per angle
   std::vector<int> nextZ;          // (Z, A) sign = non-cyclic(+)/cyclic(-)
   std::vector<int> nextC;          // (C, A) local corner order per zone
-  std::vector<int> numCycles;      // (A)
+  std::vector<int> numCycles;      // (A) corners on each angle's cycle list
+  std::vector<int> cycleList;      // 1-based corners, angle by angle (rt/rtorder.F90:62-90)
   int maxPlanes = 0;
+  long totalCycles() const { return (long)cycleList.size(); }
 };
 
 void build_tiled_mesh(int nx, int ny, int nz, bool kdtree, Mesh& m);
@@ -66,7 +68,9 @@ void compute_geometry(const Mesh& m, Geometry& g);
 void product_quadrature(int npolar, int nazimuthal, Quadrature& q);
 // Level-symmetric S2 (8 directions), the grey-acceleration quadrature.
 void level_symmetric_s2(Quadrature& q);
-// Returns 0 on success, nonzero (and a message) on an unsupported mesh cycle.
+// Returns 0 on success, nonzero (and a message) when the reference's
+// schedule would stop with a fatal error. Dependency cycles are broken as
+// the reference breaks them (Schedule::numCycles, cycleList).
 // Corner faces with |Omega.A| <= parallelTol * |A|_1 are treated as parallel
 // to Omega (no dependency); parallelTol = 0 is the reference's exact test.
 int build_schedule(const Mesh& m, const Geometry& g, const Quadrature& q,

@@ -6,7 +6,8 @@ namespace umt {
 
 template <typename Real>
 struct SweepLevelArgs {
-  const int2* items;        // (zone, angle) pairs of this level, 0-based
+  const int2* items;        // (zone, angle) pairs of this level, 0-based; a zone
+                            // with cyclic corner dependencies comes as ~zone
   int nItems;
   int G;                    // groups
   int groupsPerLane;        // K: groups one lane carries (1, 2 or 4; divides G)
@@ -100,6 +101,12 @@ hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const d
                                  double* partial, double* out, hipStream_t s);
 template <typename Dst, typename Src>
 hipError_t launch_convert(const Src* s, Dst* d, long n, hipStream_t s_);
+// Cycle-list psi: entries (corner, angle) first .. first + n - 1; toPsi loads
+// cyclePsi (G, entries) into psi (G, C + nbelem, angles from angle0), else
+// stores psi into cyclePsi.
+template <typename Real>
+hipError_t launch_cycle_copy(const int2* entries, long first, long n, int G, Real* cyclePsi,
+                             Real* psi, long angleStride, int angle0, bool toPsi, hipStream_t s);
 // Zone averages phibar(g,z) = sum_c V_c phi(g,c) / V_z (maxBits: scratch running max).
 hipError_t launch_zone_average(const double* phi, const double* vol, const double* volZ, int G,
                                long nzg, double* phibar, unsigned long long* maxBits,

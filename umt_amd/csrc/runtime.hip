@@ -102,6 +102,15 @@ struct SweepSet {
   std::vector<int2> allItems;    // the unblocked levels (plane h of every angle)
   std::vector<int> allLevelOffset;
   void* psi = nullptr;       // (G, C + nbelem, anglesPerBlock), sweep precision
+  // Dependency cycles: cyclic zones in the schedule (nextZ < 0), and the
+  // corners whose psi is read before it is computed, per angle
+  // (AngleSet numCycles / cycleOffSet / cycleList), with their psi from the
+  // previous sweep (SetData cyclePsi).
+  bool cyclicZones = false;
+  std::vector<int> cycleOffset;  // (A + 1), empty without cycle lists
+  int2* cycleEntries = nullptr;  // (corner, angle) 0-based, angle by angle
+  void* cyclePsi = nullptr;      // (G, total entries), sweep precision
+  long totalCycles() const { return cycleOffset.empty() ? 0 : cycleOffset.back(); }
   std::vector<hipEvent_t> evLevel;
   // Captured sweeps, keyed by source, cross section, groups per lane and
   // the phi they write.
@@ -216,6 +225,7 @@ struct umt_ctx {
   double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
   double tau = 0.0;       // 1/(c dt): time-dependent sweeps when > 0
   void* psiPrev = nullptr;  // previous cycle's psi, (G, C + nbelem, A), sweep precision
+  std::vector<int> cFPHost;  // (3, C) corner-face neighbours, for splitting cyclic planes
   GtaState gta;
   LearnState learn;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -258,14 +268,28 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
 template <typename Real>
 int launch_sweep(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, double* phiOut,
                  bool accumulate, bool timeLevels, void* savePsi = nullptr) {
+  if (set.cyclicZones && set.cycleOffset.empty())
+    return fail(15, "the schedule has cyclic zones (nextZ < 0) but no cycle lists: "
+                    "umt_ctx_set_cycles first");
   for (int b = 0; b < set.blocks(); ++b) {
     const int a0 = b * set.anglesPerBlock;
     const int nA = std::min(set.anglesPerBlock, set.A - a0);
+    long c0 = 0, nc = 0;
+    if (!set.cycleOffset.empty()) {
+      c0 = set.cycleOffset[a0];
+      nc = set.cycleOffset[a0 + nA] - c0;
+    }
+    // initFromCycleList for the block's angles (SetSweep.F90:149)
+    HIP_TRY(umt::launch_cycle_copy<Real>(set.cycleEntries, c0, nc, x->G, (Real*)set.cyclePsi,
+                                         (Real*)set.psi, x->psiStride(), a0, true, x->stream));
     for (int h = set.blockLevel[b]; h < set.blockLevel[b + 1]; ++h) {
       if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h], x->stream));
       HIP_TRY(umt::launch_sweep_level<Real>(level_args<Real>(x, set, S, sigt, h, a0), false, x->stream));
       if (timeLevels) HIP_TRY(hipEventRecord(set.evLevel[2 * h + 1], x->stream));
     }
+    // updateCycleList (SetSweep.F90:155)
+    HIP_TRY(umt::launch_cycle_copy<Real>(set.cycleEntries, c0, nc, x->G, (Real*)set.cyclePsi,
+                                         (Real*)set.psi, x->psiStride(), a0, false, x->stream));
     if (savePsi)  // Psi(:,:,Angle) = Psi1 for this block's angles (SweepUCBxyz.F90:314-318)
       HIP_TRY(hipMemcpyAsync((char*)savePsi + (size_t)a0 * x->psiStride() * sizeof(Real), set.psi,
                              (size_t)nA * x->psiStride() * sizeof(Real), hipMemcpyDeviceToDevice,
@@ -369,6 +393,11 @@ static int finish_problem(umt_problem* p, int npolar, int nazimuthal) {
     umt::level_symmetric_s2(p->q2);
     if (umt::build_schedule(p->m, p->g, p->q2, p->s2, err, kGtaParallelTol)) {
       p->gtaError = "grey-acceleration schedule: " + err;
+      p->s2 = umt::Schedule();
+    } else if (p->s2.totalCycles() > 0) {
+      // The grey solve is a Krylov method on a fixed linear operator; lagged
+      // cycle fluxes would make the operator depend on the previous sweep.
+      p->gtaError = "grey-acceleration schedule: the S2 directions have dependency cycles";
       p->s2 = umt::Schedule();
     }
   } catch (const std::exception& e) {
@@ -503,6 +532,14 @@ int umt_problem_schedule(const umt_problem* p, int* nHyperPlanes, int* zonesInPl
   return 0;
 }
 
+int umt_problem_cycles(const umt_problem* p, int* totalCycles, int* numCycles, int* cycleList) {
+  if (!p) return fail(1, "null problem");
+  if (totalCycles) *totalCycles = (int)p->s.totalCycles();
+  copy_out(p->s.numCycles, numCycles);
+  copy_out(p->s.cycleList, cycleList);
+  return 0;
+}
+
 int umt_problem_gta(const umt_problem* p, int* nangles, double* omega, double* weight,
                     int* nHyperPlanes, int* zonesInPlane, int* nextZ) {
   if (!p) return fail(1, "null problem");
@@ -547,6 +584,8 @@ static void free_set(SweepSet& set) {
   dfree(set.weight);
   dfree(set.items);
   dfree(set.psi);
+  dfree(set.cycleEntries);
+  dfree(set.cyclePsi);
   for (auto e : set.evLevel) (void)hipEventDestroy(e);
   set.evLevel.clear();
 }
@@ -594,7 +633,9 @@ void umt_ctx_destroy(umt_ctx* x) {
 // Hyperplane zone batches of one quadrature: level h holds plane h of every
 // angle, angles in order, zones in each plane's nextZ order.
 static int batch_levels(int Z, int A, const int* nHyperPlanes, const int* zonesInPlane,
-                        const int* nextZ, std::vector<int2>& items, std::vector<int>& levelOffset) {
+                        const int* nextZ, std::vector<int2>& items, std::vector<int>& levelOffset,
+                        bool* cyclicZones = nullptr) {
+  if (cyclicZones) *cyclicZones = false;
   std::vector<int> planeStart(A + 1, 0);
   int maxPlanes = 0;
   for (int a = 0; a < A; ++a) {
@@ -610,8 +651,10 @@ static int batch_levels(int Z, int A, const int* nHyperPlanes, const int* zonesI
       if (n < 0 || done + n > Z) return fail(16, "angle %d: hyperplane sizes exceed the mesh", a + 1);
       for (int i = 0; i < n; ++i) {
         const int z = nextZ[(long)a * Z + done + i];
-        if (z <= 0 || z > Z) return fail(15, "angle %d: nextZ entry %d (cyclic or out of range)", a + 1, z);
-        lv[h].push_back(make_int2(z - 1, a));
+        if (z == 0 || z > Z || z < -Z) return fail(15, "angle %d: nextZ entry %d out of range", a + 1, z);
+        // A cyclic zone (nextZ < 0, snac/snnext.F90:164-168) goes to the kernel as ~zone.
+        lv[h].push_back(make_int2(z > 0 ? z - 1 : ~(-z - 1), a));
+        if (z < 0 && cyclicZones) *cyclicZones = true;
       }
       done += n;
     }
@@ -676,10 +719,72 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
   return 0;
 }
 
+// Levels of angles with cycle lists. Zones of one plane can then be face
+// neighbours (a link broken by the cycle breaker whose two zones were
+// released together, or a face with no net flow), and the serial sweep order
+// inside the plane decides whether a zone reads its neighbour's new or
+// lagged psi. Such a plane is split into sub-levels in nextZ order: a zone
+// goes one sub-level after the latest earlier zone of its plane that it
+// touches, so every read sees what the serial sweep sees. Level L of the
+// result is sub-level L of every angle. Idempotent.
+static void split_cyclic_planes(int Z, int C, const int* cFP, int A,
+                                const std::vector<int>& numCycles, std::vector<int2>& items,
+                                std::vector<int>& levelOffset) {
+  std::vector<std::vector<int2>> out;
+  std::vector<int> next(A, 0);  // next output level of each angle
+  std::vector<int> mark(Z, -1), sub;
+  auto put = [&](int level, int2 it) {
+    if ((int)out.size() <= level) out.resize(level + 1);
+    out[level].push_back(it);
+  };
+  const int L = (int)levelOffset.size() - 1;
+  for (int h = 0; h < L; ++h) {
+    int i = levelOffset[h];
+    while (i < levelOffset[h + 1]) {
+      const int a = items[i].y;
+      int j = i;
+      while (j < levelOffset[h + 1] && items[j].y == a) ++j;
+      if (numCycles[a] == 0) {
+        for (int k = i; k < j; ++k) put(next[a], items[k]);
+        next[a] += 1;
+      } else {
+        sub.assign(j - i, 0);
+        int maxSub = 0;
+        for (int k = i; k < j; ++k) {
+          const int z = items[k].x < 0 ? ~items[k].x : items[k].x;
+          int sl = 0;
+          for (int c = 8 * z; c < 8 * z + 8; ++c)
+            for (int f = 0; f < 3; ++f) {
+              const int nb = cFP[3 * (size_t)c + f] - 1;
+              if (nb >= C) continue;
+              const int zn = nb / 8;
+              if (zn != z && mark[zn] >= 0) sl = std::max(sl, sub[mark[zn]] + 1);
+            }
+          sub[k - i] = sl;
+          mark[z] = k - i;
+          maxSub = std::max(maxSub, sl);
+        }
+        for (int s = 0; s <= maxSub; ++s)
+          for (int k = i; k < j; ++k)
+            if (sub[k - i] == s) put(next[a] + s, items[k]);
+        for (int k = i; k < j; ++k) mark[items[k].x < 0 ? ~items[k].x : items[k].x] = -1;
+        next[a] += maxSub + 1;
+      }
+      i = j;
+    }
+  }
+  items.clear();
+  levelOffset.assign(1, 0);
+  for (auto& lv : out) {
+    items.insert(items.end(), lv.begin(), lv.end());
+    levelOffset.push_back((int)items.size());
+  }
+}
+
 static int upload_set(umt_ctx* x, SweepSet& set, int A, const double* omega, const double* weight,
                       const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ) {
   if (int rc = batch_levels(x->Z, A, nHyperPlanes, zonesInPlane, nextZ, set.allItems,
-                            set.allLevelOffset))
+                            set.allLevelOffset, &set.cyclicZones))
     return rc;
   set.A = A;
   int rc;
@@ -740,6 +845,7 @@ static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, co
       (rc = upload_real(x, x->vol, Volume, (size_t)C)))
     return rc;
   HIP_TRY(hipMemcpyAsync(x->cFP, cFP, 3 * (size_t)C * sizeof(int), hipMemcpyHostToDevice, x->stream));
+  x->cFPHost.assign(cFP, cFP + 3 * (size_t)C);
   HIP_TRY(hipMemcpyAsync(x->vol64, Volume, (size_t)C * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipMemcpyAsync(x->volZ, VolumeZone, (size_t)Z * sizeof(double), hipMemcpyHostToDevice, x->stream));
   if (precision == 32) {
@@ -789,7 +895,9 @@ int umt_ctx_create_from_problem(const umt_problem* p, int groups, int precision,
                           p->q.weight.data(), p->s.nHyperPlanes.data(), p->s.zonesInPlane.data(),
                           p->s.nextZ.data(), out);
   if (rc) return rc;
-  if (p->gtaError.empty())
+  if (p->s.totalCycles() > 0)
+    rc = umt_ctx_set_cycles(*out, p->s.numCycles.data(), p->s.cycleList.data());
+  if (!rc && p->gtaError.empty())
     rc = umt_ctx_enable_gta(*out, p->q2.nangles, p->q2.omega.data(), p->q2.weight.data(),
                             p->s2.nHyperPlanes.data(), p->s2.zonesInPlane.data(),
                             p->s2.nextZ.data());
@@ -864,6 +972,17 @@ int umt_ctx_set_psi_prev(umt_ctx* x, const double* psi) {
   for (int a = 0; a < x->sn.A; ++a) std::memcpy(&h[a * stride], psi + a * GC, GC * sizeof(double));
   int rc = upload_real(x, x->psiPrev, h.data(), h.size());
   if (rc) return rc;
+  // initCyclePsi (control/constructDynMemory.F90:75-114): the lagged values
+  // of a new time step start from the previous cycle's psi.
+  const SweepSet& sn = x->sn;
+  if (x->precision == 32)
+    HIP_TRY(umt::launch_cycle_copy<float>(sn.cycleEntries, 0, sn.totalCycles(), x->G,
+                                          (float*)sn.cyclePsi, (float*)x->psiPrev,
+                                          x->psiStride(), 0, false, x->stream));
+  else
+    HIP_TRY(umt::launch_cycle_copy<double>(sn.cycleEntries, 0, sn.totalCycles(), x->G,
+                                           (double*)sn.cyclePsi, (double*)x->psiPrev,
+                                           x->psiStride(), 0, false, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
 }
@@ -959,8 +1078,65 @@ int umt_ctx_copy_phi(umt_ctx* dst, const umt_ctx* src) {
   HIP_TRY(hipStreamSynchronize(src->stream));
   HIP_TRY(hipMemcpyAsync(dst->phi, src->phi, (size_t)src->G * src->C * sizeof(double),
                          hipMemcpyDeviceToDevice, dst->stream));
+  // The lagged cycle psi continues too, converted to the destination's precision.
+  const long nc = (long)src->G * src->sn.totalCycles();
+  if (nc > 0 && src->sn.cycleOffset == dst->sn.cycleOffset) {
+    const void* a = src->sn.cyclePsi;
+    void* b = dst->sn.cyclePsi;
+    if (src->precision == 64 && dst->precision == 64)
+      HIP_TRY((umt::launch_convert<double, double>((const double*)a, (double*)b, nc, dst->stream)));
+    else if (src->precision == 64)
+      HIP_TRY((umt::launch_convert<float, double>((const double*)a, (float*)b, nc, dst->stream)));
+    else if (dst->precision == 64)
+      HIP_TRY((umt::launch_convert<double, float>((const float*)a, (double*)b, nc, dst->stream)));
+    else
+      HIP_TRY((umt::launch_convert<float, float>((const float*)a, (float*)b, nc, dst->stream)));
+  }
   HIP_TRY(hipStreamSynchronize(dst->stream));
   return 0;
+}
+
+int umt_ctx_set_cycles(umt_ctx* x, const int* numCycles, const int* cycleList) {
+  if (!x || !numCycles) return fail(1, "null argument");
+  SweepSet& set = x->sn;
+  const int A = set.A;
+  std::vector<int> offset(A + 1, 0);
+  for (int a = 0; a < A; ++a) {
+    if (numCycles[a] < 0 || numCycles[a] > x->C)
+      return fail(50, "angle %d: numCycles = %d out of range", a + 1, numCycles[a]);
+    offset[a + 1] = offset[a] + numCycles[a];
+  }
+  const long total = offset[A];
+  if (total > 0 && !cycleList) return fail(1, "null cycle list");
+  std::vector<int2> entries((size_t)total);
+  for (int a = 0; a < A; ++a)
+    for (int m = offset[a]; m < offset[a + 1]; ++m) {
+      const int c = cycleList[m];
+      if (c < 1 || c > x->C) return fail(50, "angle %d: cycle list corner %d out of range", a + 1, c);
+      entries[m] = make_int2(c - 1, a);
+    }
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  dfree(set.cycleEntries);
+  dfree(set.cyclePsi);
+  set.cycleOffset.clear();
+  if (total > 0) {
+    int rc;
+    if ((rc = dalloc(&set.cycleEntries, (size_t)total)) ||
+        (rc = dalloc((char**)&set.cyclePsi, (size_t)total * x->G * x->rsize())))
+      return rc;
+    HIP_TRY(hipMemcpyAsync(set.cycleEntries, entries.data(), (size_t)total * sizeof(int2),
+                           hipMemcpyHostToDevice, x->stream));
+    // Lagged values start at zero, as SetData's cyclePsi does before
+    // initCyclePsi loads a previous cycle's psi.
+    if (x->precision == 32)
+      HIP_TRY(umt::launch_fill_zero<float>((float*)set.cyclePsi, total * x->G, x->stream));
+    else
+      HIP_TRY(umt::launch_fill_zero<double>((double*)set.cyclePsi, total * x->G, x->stream));
+    set.cycleOffset = offset;
+  }
+  std::vector<int> nc(numCycles, numCycles + A);
+  split_cyclic_planes(x->Z, x->C, x->cFPHost.data(), A, nc, set.allItems, set.allLevelOffset);
+  return block_set(x, set, set.anglesPerBlock);
 }
 
 int umt_ctx_get_groups_per_lane(const umt_ctx* x, int* K) {
@@ -2022,11 +2198,13 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
                  const int* nCFaces, int ndim, int maxcf, int C, const double* A_fp,
                  const double* omega, const int* cFP, double* Psi1, int nb, const double* A_ez,
                  const int* cEZ, double quadwt, double* Phi, double* PsiB, int maxCorner,
-                 int streamId, int savePsi, int numCycles, int nBdyElem, int Mref,
-                 const int* numCorner, const int* cOffSet) {
+                 int streamId, int savePsi, int numCycles, int cycleOffSet, double* cyclePsi,
+                 const int* cycleList, int nBdyElem, int Mref, const int* numCorner,
+                 const int* cOffSet) {
   if (ndim != 3) return fail(40, "only 3D (ndim = 3) is supported, got %d", ndim);
   if (maxcf != 3 || maxCorner != 8) return fail(41, "maxcf = %d, maxCorner = %d (need 3 and 8)", maxcf, maxCorner);
-  if (numCycles != 0) return fail(42, "angle %d has %d cycle corners (unsupported)", Angle, numCycles);
+  if (numCycles < 0 || cycleOffSet < 0 || (numCycles > 0 && (!cyclePsi || !cycleList)))
+    return fail(42, "angle %d: bad cycle list (numCycles %d, cycleOffSet %d)", Angle, numCycles, cycleOffSet);
   if (nBdyElem > 0 && Mref >= 1) return fail(43, "reflecting boundaries are not supported");
   if (nHP < 1 || G < 1 || C < 1 || nb < 0) return fail(44, "bad sizes");
   long Zl = 0;
@@ -2035,11 +2213,23 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   if (int rc = check_hex_topology(Z, C, numCorner, cOffSet, nCFaces, cEZ)) return rc;
   std::vector<int2> items(Z);
   for (int i = 0; i < Z; ++i) {
-    if (nextZ[i] <= 0 || nextZ[i] > Z) return fail(45, "nextZ(%d) = %d (cyclic or out of range)", i + 1, nextZ[i]);
-    items[i] = make_int2(nextZ[i] - 1, 0);
+    const int z = nextZ[i];
+    if (z == 0 || z > Z || z < -Z) return fail(45, "nextZ(%d) = %d out of range", i + 1, z);
+    items[i] = make_int2(z > 0 ? z - 1 : ~(-z - 1), 0);  // cyclic zone: ~zone
   }
   for (long i = 0; i < 3L * C; ++i)
     if (cFP[i] < 1 || cFP[i] > C + nb) return fail(46, "cFP entry %ld out of range", i + 1);
+  for (int m = 0; m < numCycles; ++m)
+    if (cycleList[cycleOffSet + m] < 1 || cycleList[cycleOffSet + m] > C)
+      return fail(42, "angle %d: cycle list entry %d = corner %d out of range", Angle,
+                  cycleOffSet + m + 1, cycleList[cycleOffSet + m]);
+  std::vector<int> planeOffset(nHP + 1, 0);
+  for (int h = 0; h < nHP; ++h) planeOffset[h + 1] = planeOffset[h] + zip[h];
+  if (numCycles > 0) {
+    const int one = numCycles;
+    split_cyclic_planes(Z, C, cFP, 1, std::vector<int>(1, one), items, planeOffset);
+  }
+  const int nLevels = (int)planeOffset.size() - 1;
 
   DropIn& d = g_drop[streamId % kMaxStreams];
   if (!d.stream) HIP_TRY(hipStreamCreate(&d.stream));
@@ -2057,6 +2247,10 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   HIP_TRY(hipMemcpyAsync(d.items, items.data(), (size_t)Z * sizeof(int2), hipMemcpyHostToDevice, s));
   // The boundary tail of Psi1 starts from PsiB(:,:,Angle) (SweepUCBxyz.F90:95-97).
   if (nb > 0) std::memcpy(Psi1 + GC, PsiB, (size_t)G * nb * 8);
+  // initFromCycleList (control/constructDynMemory.F90:118-152), on the host
+  // copy of Psi1 that is uploaded whole.
+  for (int m = cycleOffSet; m < cycleOffSet + numCycles; ++m)
+    std::memcpy(Psi1 + (size_t)(cycleList[m] - 1) * G, cyclePsi + (size_t)m * G, (size_t)G * 8);
   HIP_TRY(hipMemcpyAsync(d.psi1, Psi1, (GC + (size_t)G * nb) * 8, hipMemcpyHostToDevice, s));
   const bool useTau = tau != 0.0;
   if (useTau) HIP_TRY(hipMemcpyAsync(d.tauPsi, Psi, GC * 8, hipMemcpyHostToDevice, s));
@@ -2078,18 +2272,19 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   a.quadwt = quadwt;
   a.tauPsi = useTau ? d.tauPsi : nullptr;
   a.tau = tau;
-  long done = 0;
-  for (int h = 0; h < nHP; ++h) {
-    a.items = d.items + done;
-    a.nItems = zip[h];
+  for (int h = 0; h < nLevels; ++h) {
+    a.items = d.items + planeOffset[h];
+    a.nItems = planeOffset[h + 1] - planeOffset[h];
     HIP_TRY(umt::launch_sweep_level<double>(a, true, s));
-    done += zip[h];
   }
   HIP_TRY(hipMemcpyAsync(Psi1, d.psi1, GC * 8, hipMemcpyDeviceToHost, s));
   if (nb > 0) HIP_TRY(hipMemcpyAsync(PsiB, d.psi1 + GC, (size_t)G * nb * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(Phi, d.phi, GC * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (savePsi) std::memcpy(Psi, Psi1, GC * 8);
+  // updateCycleList (control/constructDynMemory.F90:156-189)
+  for (int m = cycleOffSet; m < cycleOffSet + numCycles; ++m)
+    std::memcpy(cyclePsi + (size_t)m * G, Psi1 + (size_t)(cycleList[m] - 1) * G, (size_t)G * 8);
   return 0;
 }
 
@@ -2104,14 +2299,14 @@ void gpu_sweepucbxyz(int* Angle, int* nHyperPlanes, int* nZonesInPlane, int* nex
                      int* savePsi, int* numCycles, int* cycleOffSet, double* cyclePsi,
                      int* cycleList, int* b0, int* nBdyElem, double* PsiBMref, int* Mref,
                      int* Geom_numCorner, int* Geom_cOffSet) {
-  (void)nextC; (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)cycleOffSet;
-  (void)cyclePsi; (void)cycleList; (void)b0; (void)PsiBMref;
+  (void)nextC; (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)b0; (void)PsiBMref;
   std::lock_guard<std::mutex> lock(g_dropMutex);
   g_dropin_status = dropin_sweep(*Angle, *nHyperPlanes, nZonesInPlane, nextZ, STotal, *tau, Psi,
                                  *Groups, Volume, Sigt, nCFacesArray, *ndim, *maxcf, *ncorner, A_fp,
                                  omega, cFP, Psi1, *nbelem, A_ez, cEZ, *quadwt, Phi, PsiB,
-                                 *maxCorner, *streamIdPtr, *savePsi, *numCycles, *nBdyElem, *Mref,
-                                 Geom_numCorner, Geom_cOffSet);
+                                 *maxCorner, *streamIdPtr, *savePsi, *numCycles, *cycleOffSet,
+                                 cyclePsi, cycleList, *nBdyElem, *Mref, Geom_numCorner,
+                                 Geom_cOffSet);
   if (g_dropin_status) fprintf(stderr, "gpu_sweepucbxyz: %s\n", g_err.c_str());
 }
 

@@ -123,7 +123,9 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   const bool bottom = c < 4;
   const int G = p.G;
   const int2 it = p.items[item];
-  const int z = it.x, a = it.y;
+  // A zone whose corners depend on each other in a cycle comes as ~zone.
+  const bool cyclic = it.x < 0;
+  const int z = cyclic ? ~it.x : it.x, a = it.y;
   const long cc = 8L * z + c;
   const Real ox = (Real)p.omega[3 * a + 0];
   const Real oy = (Real)p.omega[3 * a + 1];
@@ -229,6 +231,23 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
 #pragma unroll
   for (int k = 0; k < K; ++k) psi.v[k] = src[k] * rden[k];
   int fin = (k0 == 0 && k1 == 0 && k2 == 0) ? 1 : 0;
+  if (__any(cyclic)) {
+    // A cyclic zone is solved once from the previous values of its own
+    // corners (SweepUCBxyz.F90:281-295), which psi holds until the zone is
+    // computed: its corners are on the angle's cycle list. The 8 lanes of
+    // an item are all cyclic or all not, so the moves stay inside the item.
+    GroupVec<Real, K> old{};
+    if (cyclic) old = load_groups<Real, K>(psiA + cc * G + g0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const Real x = cyclic ? old.v[k] : (Real)0;
+      const Real n0 = from_ez<0>(x, bottom), n1 = from_ez<1>(x, bottom),
+                 n2 = from_ez<2>(x, bottom);
+      const Real once = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
+      psi.v[k] = cyclic ? once : psi.v[k];
+    }
+    fin = cyclic ? 1 : fin;
+  }
   for (int iter = 0; iter < 8; ++iter) {
     if (__all(fin)) break;
     const int f0 = from_ez<0>(fin, bottom), f1 = from_ez<1>(fin, bottom),
@@ -238,9 +257,10 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
       const Real x = psi.v[k];
       const Real n0 = from_ez<0>(x, bottom), n1 = from_ez<1>(x, bottom),
                  n2 = from_ez<2>(x, bottom);
-      psi.v[k] = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
+      const Real upd = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
+      psi.v[k] = cyclic ? psi.v[k] : upd;  // a cyclic zone keeps its single solve
     }
-    fin = (k0 == 0 || f0) && (k1 == 0 || f1) && (k2 == 0 || f2);
+    fin = cyclic || ((k0 == 0 || f0) && (k1 == 0 || f1) && (k2 == 0 || f2));
   }
 
   if (valid) {
@@ -303,6 +323,26 @@ __global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real
   const int item = sg.itemStart + (int)(local / blocksPerItem);
   const int g0 = K * (int)(local - (local / blocksPerItem) * blocksPerItem);
   sweep_corner<Real, K, false>(p, item, g0, valid);
+}
+
+// Cycle-list corners (control/constructDynMemory.F90:118-189): entry m is
+// (corner, angle), 0-based; toPsi copies cyclePsi(:, m) into psi before the
+// angle block is swept (initFromCycleList), otherwise psi into cyclePsi after
+// it (updateCycleList). Entries first .. first + n - 1, one thread per group.
+template <typename Real>
+__global__ __launch_bounds__(256) void cycle_copy(const int2* __restrict__ entries, long first,
+                                                  long n, int G, Real* __restrict__ cyclePsi,
+                                                  Real* __restrict__ psi, long angleStride,
+                                                  int angle0, int toPsi) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * G) return;
+  const long m = first + i / G;
+  const int g = (int)(i - (i / G) * G);
+  const int2 e = entries[m];
+  Real* p = psi + (long)(e.y - angle0) * angleStride + (long)e.x * G + g;
+  Real* q = cyclePsi + m * G + g;
+  if (toPsi) *p = *q;
+  else *q = *p;
 }
 
 // phi(g,c) = sum_a w_a psi(g,c,a), accumulated in FP64 in angle order
@@ -506,6 +546,15 @@ hipError_t launch_copy_probe(const double* src, double* dst, long n, hipStream_t
   return hipGetLastError();
 }
 
+template <typename Real>
+hipError_t launch_cycle_copy(const int2* entries, long first, long n, int G, Real* cyclePsi,
+                             Real* psi, long angleStride, int angle0, bool toPsi, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((cycle_copy<Real>), dim3(blocks_for(n * G, 256)), dim3(256), 0, s, entries,
+                     first, n, G, cyclePsi, psi, angleStride, angle0, toPsi ? 1 : 0);
+  return hipGetLastError();
+}
+
 template <typename Dst, typename Src>
 hipError_t launch_convert(const Src* s_, Dst* d, long n, hipStream_t s) {
   if (n == 0) return hipSuccess;
@@ -532,6 +581,12 @@ template hipError_t launch_source_update<float>(const double*, const double*, co
 template hipError_t launch_fill_zero<double>(double*, long, hipStream_t);
 template hipError_t launch_fill_zero<float>(float*, long, hipStream_t);
 template hipError_t launch_convert<float, double>(const double*, float*, long, hipStream_t);
+template hipError_t launch_convert<double, float>(const float*, double*, long, hipStream_t);
+template hipError_t launch_convert<float, float>(const float*, float*, long, hipStream_t);
+template hipError_t launch_cycle_copy<double>(const int2*, long, long, int, double*, double*, long,
+                                              int, bool, hipStream_t);
+template hipError_t launch_cycle_copy<float>(const int2*, long, long, int, float*, float*, long,
+                                             int, bool, hipStream_t);
 template hipError_t launch_convert<double, double>(const double*, double*, long, hipStream_t);
 
 }  // namespace umt

@@ -31,7 +31,7 @@ module umt_sweep_mod
             umt_ctx_set_stall_window, umt_ctx_set_zone_neighbors,             &
             umt_learner_create, umt_learner_destroy, umt_learner_train,       &
             umt_ctx_set_learner, umt_ctx_learn_collect, umt_ctx_learn_commit, &
-            umt_ctx_learn_stats
+            umt_ctx_learn_stats, umt_problem_cycles, umt_ctx_set_cycles
 
   interface
 
@@ -130,6 +130,17 @@ module umt_sweep_mod
       integer(c_int) :: nHyperPlanes(*), zonesInPlane(*), nextZ(*), nextC(*)
     end function umt_problem_schedule
 
+!   ASet%numCycles(A) and ASet%cycleList (1-based corners, angle by angle);
+!   pass totalCycles alone first (the arrays may be c_null_ptr via the
+!   c_ptr form below) to size cycleList.
+    integer(c_int) function umt_problem_cycles(prob, totalCycles, numCycles, cycleList) &
+        bind(c, name="umt_problem_cycles")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: prob
+      integer(c_int) :: totalCycles
+      type(c_ptr), value :: numCycles, cycleList
+    end function umt_problem_cycles
+
     integer(c_int) function umt_draw_cross_sections(seed, nzones, groups, sigt_lo, sigt_hi, &
                                                     c_lo, c_hi, q_lo, q_hi, sigt, sigs, q)  &
         bind(c, name="umt_draw_cross_sections")
@@ -225,6 +236,15 @@ module umt_sweep_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: dst, src
     end function umt_ctx_copy_phi
+
+!   The schedule's cycle lists (ASet%numCycles, ASet%cycleList) for meshes
+!   whose sweeps have dependency cycles.
+    integer(c_int) function umt_ctx_set_cycles(ctx, numCycles, cycleList) &
+        bind(c, name="umt_ctx_set_cycles")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: numCycles(*), cycleList(*)
+    end function umt_ctx_set_cycles
 
 !   mode 0: plain source iteration, 1: GTA, 2: learned correction
     integer(c_int) function umt_ctx_set_acceleration(ctx, mode) &

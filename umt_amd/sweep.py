@@ -132,6 +132,13 @@ class TiledProblem:
             check("umt_problem_schedule", _lib.load().umt_problem_schedule(
                 self._h, _ip(s["nHyperPlanes"]), _ip(s["zonesInPlane"]), _ip(s["nextZ"]),
                 _ip(s["nextC"])))
+            total = C.c_int(0)
+            lib = _lib.load()
+            check("umt_problem_cycles", lib.umt_problem_cycles(self._h, C.byref(total), None, None))
+            s["numCycles"] = np.zeros(A, np.int32)
+            s["cycleList"] = np.zeros(total.value, np.int32)
+            check("umt_problem_cycles", lib.umt_problem_cycles(
+                self._h, None, _ip(s["numCycles"]), _ip(s["cycleList"])))
             self._sched = s
         return self._sched
 
@@ -332,8 +339,15 @@ class SweepContext:
         return n.value, b.value
 
     def copy_phi_from(self, other):
-        """phi = other's phi (device to device)."""
+        """phi = other's phi (device to device), with the lagged cycle-list psi."""
         check("umt_ctx_copy_phi", _lib.load().umt_ctx_copy_phi(self._h, other._h))
+
+    def set_cycles(self, num_cycles, cycle_list):
+        """Cycle lists of the schedule (numCycles (A), cycleList 1-based);
+        contexts made from a problem already hold the problem's."""
+        nc = np.ascontiguousarray(num_cycles, dtype=np.int32)
+        cl = np.ascontiguousarray(cycle_list, dtype=np.int32)
+        check("umt_ctx_set_cycles", _lib.load().umt_ctx_set_cycles(self._h, _ip(nc), _ip(cl)))
 
     def gta_stats(self):
         """(BiCGStab iterations, S2 sweeps) done by GTA so far."""
@@ -393,12 +407,14 @@ def hbm_copy_probe(n_doubles, reps=10):
 
 
 def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, Psi_angle=None,
-                 save_psi=False, stream_id=0):
+                 save_psi=False, stream_id=0, cycle_psi=None):
     """One angle through the drop-in C-ABI ``gpu_sweepucbxyz``.
 
     ``angle`` is 1-based. Psi1 (C+nbelem, G), PsiB_angle (nbelem, G) and Phi
     (C, G) are updated in place, as the Fortran caller's arrays would be.
-    Returns the library status (0 on success).
+    When the problem's schedule has cycle lists, cycle_psi (total, G) holds
+    the lagged psi of every angle's cycle corners (Teton's cyclePsi) and is
+    updated in place; None uses zeros. Returns the library status (0 on success).
     """
     lib = _lib.load()
     m, g, q, s = problem.mesh(), problem.geometry(), problem.quadrature(), problem.schedule()
@@ -410,16 +426,20 @@ def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, P
     nextC = _i32(s["nextC"][a])
     G = STotal.shape[1]
     omega = _f64(q["omega"][a])
+    ncyc = s["numCycles"]
+    cyc_list = _i32(s["cycleList"]) if len(s["cycleList"]) else np.zeros(1, np.int32)
+    if cycle_psi is None:
+        cycle_psi = np.zeros((max(len(s["cycleList"]), 1), STotal.shape[1]))
+    assert cycle_psi.dtype == np.float64 and cycle_psi.flags["C_CONTIGUOUS"]
     ints = {k: C.c_int(v) for k, v in dict(
         Angle=angle, nHP=nhp, Groups=G, ndim=3, maxcf=3, ncorner=problem.ncorner,
         nbelem=problem.nbelem, NumAngles=problem.nangles, maxCorner=8, mem0solve1=0,
-        streamId=stream_id, totalStreams=1, savePsi=int(save_psi), numCycles=0, cycleOffSet=0,
-        b0=0, nBdyElem=0, Mref=-1).items()}
+        streamId=stream_id, totalStreams=1, savePsi=int(save_psi), numCycles=int(ncyc[a]),
+        cycleOffSet=int(np.sum(ncyc[:a])), b0=0, nBdyElem=0, Mref=-1).items()}
     tau_c = C.c_double(tau)
     wt = C.c_double(q["weight"][a])
     Psi = Psi_angle if Psi_angle is not None else np.zeros((problem.ncorner, G))
     dummy_d = np.zeros(1)
-    dummy_i = np.zeros(1, np.int32)
     for arr in (STotal, Sigt, Psi1, PsiB_angle, Phi, Psi):
         assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"]
     lib.gpu_sweepucbxyz(
@@ -431,7 +451,7 @@ def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, P
         C.byref(wt), _dp(Phi), _dp(PsiB_angle), C.byref(ints["maxCorner"]),
         C.byref(ints["mem0solve1"]), C.byref(ints["streamId"]), C.byref(ints["totalStreams"]),
         C.byref(ints["savePsi"]), C.byref(ints["numCycles"]), C.byref(ints["cycleOffSet"]),
-        _dp(dummy_d), _ip(dummy_i), C.byref(ints["b0"]), C.byref(ints["nBdyElem"]),
+        _dp(cycle_psi), _ip(cyc_list), C.byref(ints["b0"]), C.byref(ints["nBdyElem"]),
         _dp(dummy_d), C.byref(ints["Mref"]), _ip(m["numCorner"]), _ip(m["cOffSet"]))
     sid = C.c_int(stream_id)
     lib.gpu_streamsynchronize(C.byref(sid))
