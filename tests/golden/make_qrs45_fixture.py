@@ -1,7 +1,8 @@
 """Write tests/golden/qrs45_n1to4.json: the Spence QRS45 product-quadrature
 tables for orders N = 1..4 as the reference tabulates them
 (src/teton/mods/QuadratureData_mod.F90:1144-2857: `first`, `last`,
-cosTheta, weightTheta, cosPhiXYZ, weightPhiXYZ). Reads the reference source
+cosTheta, weightTheta, cosPhiXYZ, weightPhiXYZ, and the RZ azimuthal tables
+cosPhiRZ, weightPhiRZ that rt/quadrz.F90 uses). Reads the reference source
 as text; run once where /root/reference exists:
 
     python tests/golden/make_qrs45_fixture.py /root/reference
@@ -27,9 +28,10 @@ def main():
     text = open(path).read()
     first = [int(x) for x in data_block(text, "first")]
     last = [int(x) for x in data_block(text, "last")]
-    tables = {k: data_block(text, k) for k in ("cosTheta", "weightTheta", "cosPhiXYZ", "weightPhiXYZ")}
+    tables = {k: data_block(text, k) for k in ("cosTheta", "weightTheta", "cosPhiXYZ",
+                                               "weightPhiXYZ", "cosPhiRZ", "weightPhiRZ")}
     out = {"source": "src/teton/mods/QuadratureData_mod.F90 (data first/last/cosTheta/"
-                     "weightTheta/cosPhiXYZ/weightPhiXYZ)", "orders": {}}
+                     "weightTheta/cosPhiXYZ/weightPhiXYZ/cosPhiRZ/weightPhiRZ)", "orders": {}}
     for n in range(1, 5):
         lo, hi = first[n - 1], last[n - 1]
         out["orders"][str(n)] = {k: v[lo - 1:hi] for k, v in tables.items()}

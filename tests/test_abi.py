@@ -11,11 +11,12 @@ import pytest
 import umt_amd as U
 from umt_amd import _lib
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "umt_sweep.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include")
 
 
 def _declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(os.path.join(INCLUDE, f)).read() for f in sorted(os.listdir(INCLUDE))
+                   if f.endswith(".h"))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", text, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "while")))
@@ -25,6 +26,7 @@ def test_every_declared_symbol_is_exported():
     lib = _lib.load()
     names = _declared_functions()
     assert "gpu_sweepucbxyz" in names and "umt_ctx_create" in names
+    assert "umt_rz_ctx_create" in names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     # and the Python binding covers the whole header

@@ -1,4 +1,5 @@
-"""ctypes binding of libumt_amd.so (declarations in include/umt_sweep.h).
+"""ctypes binding of libumt_amd.so (declarations in include/umt_sweep.h and
+include/umt_rz.h).
 
 The library is built in-tree (``python -c "import __graft_entry__ as g; g.build()"``
 or ``make -C umt_amd/csrc``). Loading fails loudly when it is missing: there
@@ -20,7 +21,7 @@ _pd = C.POINTER(C.c_double)
 _pf = C.POINTER(C.c_float)
 _vp = C.c_void_p
 
-# name -> (restype, argtypes)
+# name -> (restype, argtypes); include/umt_sweep.h and include/umt_rz.h
 SIGNATURES = {
     "umt_last_error": (C.c_char_p, []),
     "gpu_sweepucbxyz": (None, [_pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi, _pd, _pd, _pi, _pi,
@@ -109,6 +110,29 @@ SIGNATURES = {
     "umt_ctx_learn_discard": (_i, [_vp]),
     "umt_ctx_learn_features": (_i, [_vp, _pd, _pd, _pd, _pf]),
     "umt_ctx_learn_stats": (_i, [_vp, _pi64, _pi]),
+    # RZ (include/umt_rz.h)
+    "umt_rz_problem_structured": (_i, [_i, _i, _d, _d, _d, _d, _d, _u64, _i, _i, C.POINTER(_vp)]),
+    "umt_rz_problem_from_quad_mesh": (_i, [_i, _pd, _i, _pi, _i, _i, C.POINTER(_vp)]),
+    "umt_rz_problem_destroy": (None, [_vp]),
+    "umt_rz_problem_dims": (_i, [_vp, _pi64]),
+    "umt_rz_problem_mesh": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd]),
+    "umt_rz_problem_points": (_i, [_vp, _pd, _pi]),
+    "umt_rz_problem_geometry": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "umt_rz_problem_quadrature": (_i, [_vp, _pd, _pd, _pi, _pi, _pd, _pd, _pd]),
+    "umt_rz_problem_schedule": (_i, [_vp, _pi, _pi, _pi, _pi]),
+    "umt_rz_ctx_create": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
+    "umt_rz_ctx_destroy": (None, [_vp]),
+    "umt_rz_ctx_dims": (_i, [_vp, _pi64]),
+    "umt_rz_ctx_set_sigt": (_i, [_vp, _pd]),
+    "umt_rz_ctx_set_source": (_i, [_vp, _pd]),
+    "umt_rz_ctx_set_scattering": (_i, [_vp, _pd, _pd]),
+    "umt_rz_ctx_sweep": (_i, [_vp]),
+    "umt_rz_ctx_get_phi": (_i, [_vp, _pd]),
+    "umt_rz_ctx_set_phi": (_i, [_vp, _pd]),
+    "umt_rz_ctx_get_psi": (_i, [_vp, _pd]),
+    "umt_rz_ctx_source_iteration": (_i, [_vp, _d, _i, _pi, _pd, _pd]),
+    "umt_rz_ctx_set_groups_per_lane": (_i, [_vp, _i]),
+    "umt_rz_ctx_time_sweeps": (_i, [_vp, _i, _i, _pd, _pd]),
 }
 
 _lib = None

@@ -11,6 +11,7 @@
 #include <stdexcept>
 
 #include "host.h"
+#include "qrs45.h"
 
 namespace umt {
 
@@ -115,33 +116,6 @@ void compute_geometry(const Mesh& m, Geometry& g) {
 // rt/quadProduct.F90:73-182 does with the polar axis along x (the driver's
 // default, driver/test_driver.cc:1198), then normalised so that the weights
 // sum to 4*pi (rt/rtquad.F90:95-105).
-namespace {
-const double kCosTheta[10] = {
-    5.498591989407761726e-01,
-    8.606674142130846805e-01, 2.793189648948431469e-01,
-    9.448055658016270009e-01, 6.290348537782320948e-01, 1.655001936344792923e-01,
-    9.740231629785398582e-01, 8.013996426135227003e-01, 4.580037624025995457e-01,
-    1.088125311505435677e-01};
-const double kWeightTheta[10] = {
-    1.000000000000000000e+00,
-    3.830774779020281184e-01, 6.169225220979718260e-01,
-    1.621784854610573423e-01, 4.493341854321756257e-01, 3.884873291067670320e-01,
-    7.906110909238366069e-02, 2.709462552109827005e-01, 3.870283006221111322e-01,
-    2.629643350745224928e-01};
-const double kCosPhi[10] = {
-    7.071067811865475727e-01,
-    3.382470672948211798e-01, 9.410573422839081381e-01,
-    1.915304654341140334e-01, 7.071067811865475727e-01, 9.814866686871460688e-01,
-    1.224171014312665645e-01, 5.098778812535227356e-01, 8.602467937797958486e-01,
-    9.924787419774626640e-01};
-const double kWeightPhi[10] = {
-    1.570796326794896558e+00,
-    7.853981633974482790e-01, 7.853981633974482790e-01,
-    4.573462752141532750e-01, 6.561037763665900080e-01, 4.573462752141532750e-01,
-    2.979649884419325878e-01, 4.874331749555156912e-01, 4.874331749555156912e-01,
-    2.979649884419325878e-01};
-const int kFirst[4] = {0, 1, 3, 6};
-}  // namespace
 
 void product_quadrature(int npolar, int nazimuthal, Quadrature& q) {
   if (npolar < 1 || npolar > 4 || nazimuthal < 1 || nazimuthal > 4)

@@ -112,6 +112,43 @@ hipError_t launch_zone_average(const double* phi, const double* vol, const doubl
                                long nzg, double* phibar, unsigned long long* maxBits,
                                hipStream_t s);
 
+// 2D cylindrical (RZ) sweep (rz_kernels.hip).
+struct RzAngle {
+  double mu, xi;        // direction cosines along r and z
+  double fac;           // angDerivFac
+  double w1, w2;        // quadTauW1, quadTauW2: PsiM <- w1 psi - w2 PsiM
+  int level;            // xi level (index of its half-angle flux buffer)
+  int flags;            // 1: starting direction, 2: the next direction is finishing
+};
+template <typename Real>
+struct RzLevelArgs {
+  const int2* items;    // (zone, direction) pairs of this level, 0-based
+  int nItems;
+  int G;                // groups (a multiple of the groups per lane)
+  int C;                // corners; cFP values above C are boundary elements
+  const Real* Afp;      // (2, 2, C)
+  const Real* Aez;      // (2, 2, C)
+  const Real* Rfp;      // (2, C) RadiusFP
+  const Real* Rez;      // (2, C) RadiusEZ
+  const Real* area;     // (C)
+  const Real* vol;      // (C) r-weighted corner volume
+  const int* cFP;       // (2, C), 1-based
+  const Real* S;        // (G, C)
+  const Real* sigt;     // (G, Z)
+  const RzAngle* angles;
+  Real* psi;            // (G, C + nbelem, A)
+  long psiAngleStride;  // (C + nbelem) * G
+  Real* psiM;           // (G, C, xi levels)
+};
+template <typename Real>
+hipError_t launch_rz_sweep_level(const RzLevelArgs<Real>& a, int K, hipStream_t s);
+template <typename Real>
+hipError_t launch_rz_source_update(const double* phi, const double* sigs, const double* q, int G,
+                                   long n, Real* S, hipStream_t s);
+hipError_t launch_rz_residual(const double* phi, const double* vol, const double* volZ, int G,
+                              long nzg, double* phibar, double* phibarOld, double cutoff,
+                              unsigned long long* bits, hipStream_t s);
+
 // Learned correction network (learn_kernels.hip): 16 feature slots -> 64 ->
 // 64 -> 1 with tanh. FP32 parameter layout: W1 [feature][hidden], b1,
 // W2 [in][out], b2, w3, b3.

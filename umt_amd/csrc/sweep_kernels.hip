@@ -23,6 +23,7 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "lane_moves.h"
 
 namespace umt {
 
@@ -37,32 +38,10 @@ __device__ __forceinline__ Real dot3(Real ox, Real oy, Real oz, const Real* a) {
 }
 
 // Cross-lane moves inside the 8-lane group of one (item, group) pair, lane
-// c = corner c. kEZ[c][0] is the previous corner of the bottom quad
-// (quad_perm [3,0,1,2]) but the next corner of the top quad
+// c = corner c (lane_moves.h). kEZ[c][0] is the previous corner of the
+// bottom quad (quad_perm [3,0,1,2]) but the next corner of the top quad
 // (quad_perm [1,2,3,0]); kEZ[c][1] is the inverse; kEZ[c][2] = c ^ 4
 // (row_shl:4 for the bottom quad, row_shr:4 for the top quad).
-template <int kCtrl>
-__device__ __forceinline__ int dpp_i32(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
-}
-template <int kCtrl>
-__device__ __forceinline__ double dpp_move(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = dpp_i32<kCtrl>((int)b), hi = dpp_i32<kCtrl>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <int kCtrl>
-__device__ __forceinline__ float dpp_move(float x) {
-  return __int_as_float(dpp_i32<kCtrl>(__float_as_int(x)));
-}
-template <int kCtrl>
-__device__ __forceinline__ int dpp_move(int x) {
-  return dpp_i32<kCtrl>(x);
-}
-constexpr int kQuadPrev = 0x93;  // quad_perm [3,0,1,2]
-constexpr int kQuadNext = 0x39;  // quad_perm [1,2,3,0]
-constexpr int kRowShl4 = 0x104;
-constexpr int kRowShr4 = 0x114;
 
 // Value held by this lane's EZ neighbour in corner-face slot F.
 template <int F, typename T>
@@ -92,22 +71,6 @@ __device__ __forceinline__ Real sez_term(Real A, Real sig, Real v, Real halfSigI
       (v * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd)) / (gnum + gden * sig);
   const Real noOpp = A * halfSigInv * (Qu - Qd);
   return areaOpp > 0 ? withOpp : noOpp;
-}
-
-// K consecutive groups of one corner as one aligned vector access.
-template <typename Real, int K>
-struct GroupVec {
-  Real v[K];
-};
-template <typename Real, int K>
-__device__ __forceinline__ GroupVec<Real, K> load_groups(const Real* p) {
-  GroupVec<Real, K> r;
-  __builtin_memcpy(&r, __builtin_assume_aligned(p, K * sizeof(Real)), sizeof r);
-  return r;
-}
-template <typename Real, int K>
-__device__ __forceinline__ void store_groups(Real* p, const GroupVec<Real, K>& r) {
-  __builtin_memcpy(__builtin_assume_aligned(p, K * sizeof(Real)), &r, sizeof r);
 }
 
 // One lane: one corner c of one zone for one direction and K consecutive
