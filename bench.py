@@ -13,7 +13,9 @@ resident in HBM. Also reported: the FP32 sweep (FP64 phi), the wall time of
 source iteration to residual < 1e-8 (plain FP64 against grey-transport
 acceleration and the learned correction, FP64 and FP32 then FP64), the
 sweep-level kernel against the HBM roofline, and the oracle's CPU sweep
-timed on this host's cores.
+timed on this host's cores; and the 2D cylindrical (RZ) sweep on a regular
+512x512 quad mesh with the RZ product quadrature 4x4 (FP64, FP32, roofline,
+CPU oracle).
 """
 import argparse
 import json
@@ -119,6 +121,99 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
     return dict(dt=dt, ms_levels=ms_levels, launches=launches, n_in=n_in,
                 ms_sweep_events=ms_sweep_ev, groups_per_lane=best_k,
                 autotune_ms_per_sweep={str(k): v for k, v in tune_ms.items()})
+
+
+def rz_bytes_per_unknown(precision, n_in, G, start_frac):
+    """Algorithmic HBM bytes per unknown of the RZ level kernel: source read,
+    sigma_t shared by 4 corners, psi write, n_in upwind psi reads, half-angle
+    flux read (not for starting directions) and write, and the per-corner
+    geometry (A_fp, A_ez, RadiusFP, RadiusEZ, Area, Volume in the sweep
+    precision, cFP as two int32) shared by the G groups."""
+    r = 8 if precision == 64 else 4
+    geo = 15 * r + 8
+    return r + r / 4 + r + r * n_in + r * (1 - start_frac) + r + geo / G
+
+
+def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds, log):
+    """The 2D cylindrical sweep: FP64 and FP32 throughput, the level kernel
+    against the HBM roofline, and the RZ oracle timed on this host's cores."""
+    import umt_amd as U
+
+    p = U.RzProblem(nr, nz, npolar, nazimuthal, r0=0.0, r1=1.0, z0=0.0, z1=1.0)
+    q, g, m = p.quadrature(), p.geometry(), p.mesh()
+    swept = q["finishing"] == 0
+    unknowns = p.ncorner * int(swept.sum()) * G
+    start_frac = float(q["starting"].sum()) / float(swept.sum())
+    # incoming interior FP faces per (corner, swept direction)
+    inner = m["cFP"] <= p.ncorner
+    cnt = 0
+    for a in np.where(swept)[0]:
+        d = np.einsum("cfk,k->cf", g["A_fp"], q["omega"][a])
+        cnt += int(np.count_nonzero((d < 0) & inner))
+    n_in = cnt / (p.ncorner * int(swept.sum()))
+    rng = np.random.default_rng(SEED)
+    sig = np.exp(rng.uniform(np.log(1e-2), np.log(1e3), (p.nzones, G)))
+    S = rng.uniform(0.0, 1.0, (p.ncorner, G)) / (2 * np.pi)
+    out = {"config": {"workload": (f"UCB RZ sweep, {nr}x{nz} quad zones on [0,1]x[0,1] "
+                                   f"({p.nzones} zones, "
+                                   f"{p.ncorner} corners), RZ product quadrature {npolar}x"
+                                   f"{nazimuthal} ({p.nangles} directions in {p.nlevels} xi "
+                                   f"levels, {int(swept.sum())} swept), {G} groups, sigma_t "
+                                   f"logU[1e-2,1e3], vacuum"),
+                      "unknowns_per_sweep": unknowns, "level_launches": p.sweep_levels},
+           "unit": "unknowns/s (corners x swept directions x groups per second)"}
+    for prec in (64, 32):
+        ctx = U.RzContext(p, G, precision=prec)
+        ctx.set_graphs(GRAPHS[0])
+        ctx.set_sigt(sig)
+        ctx.set_source(S)
+        for _ in range(warmup):
+            ctx.sweep()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.sweep()
+        dt = time.perf_counter() - t0
+        ms_ev, ms_lv = ctx.time_sweeps(1, 3)
+        ctx.close()
+        B = rz_bytes_per_unknown(prec, n_in, G, start_frac)
+        achieved = B * unknowns / (ms_lv * 1e-3) / 1e9
+        key = "fp64" if prec == 64 else "fp32"
+        kname = f"void umt::rz_sweep_level<{'double' if prec == 64 else 'float'}, 4>(umt::RzLevelArgs<{'double' if prec == 64 else 'float'}>)"
+        prof = load_traffic("rz_sweep_level_512x512_traffic.json")
+        traffic = None
+        if prof and (nr, nz, G) == (512, 512, 16) and kname in prof.get("kernels", {}):
+            traffic = prof["kernels"][kname]["hbm_bytes_per_launch"]
+        out[key] = {"value": unknowns * steps / dt, "ms_per_step": dt * 1e3 / steps,
+                    "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                 "traffic": traffic,
+                                 "kernel": f"rz_sweep_level<{'double' if prec == 64 else 'float'},K>",
+                                 "level_kernels_ms_per_sweep": ms_lv,
+                                 "algorithmic_bytes_per_unknown": B, "n_in": n_in}}
+        log(f"RZ FP{prec} sweep: {unknowns * steps / dt:.3e} unknowns/s, level kernels "
+            f"{ms_lv:.2f} ms ({achieved:.0f} GB/s)")
+    if cpu_seconds > 0:
+        from oracle import rz as R
+
+        Gc = 4
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        threads = max(1, min(threads, 16, p.nlevels))
+        s = p.schedule()
+        Sc, sc = np.ascontiguousarray(S[:, :Gc]), np.ascontiguousarray(sig[:, :Gc])
+        R.lib(fast=True)
+        reps, t_total = 0, 0.0
+        while t_total < cpu_seconds or reps == 0:
+            t0 = time.perf_counter()
+            R.sweep_all(m, g, q, s, Sc, sc, nthreads=threads, fast=True)
+            t_total += time.perf_counter() - t0
+            reps += 1
+        out["cpu_baseline"] = {
+            "value": p.ncorner * int(swept.sum()) * Gc * reps / t_total, "unit": "unknowns/s",
+            "cores": threads, "kind": "port",
+            "sample": (f"oracle C RZ sweep (liboracle_fast.so, OpenMP over xi levels) of the "
+                       f"same mesh and directions with {Gc} groups, {reps} sweeps in "
+                       f"{t_total:.1f} s on {threads} threads of '{cpu_model()}'")}
+    return out
 
 
 APB = {64: 0, 32: 0}  # angles per block of the FP64 / FP32 contexts (0: all)
@@ -371,6 +466,10 @@ def main():
                          "iteration without GTA does not converge within --max-iters)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every sweep eagerly instead of replaying captured graphs")
+    ap.add_argument("--rz", type=int, nargs=2, default=[512, 512],
+                    help="RZ sweep section: nr nz quad zones (0 0: skip it)")
+    ap.add_argument("--only-rz", action="store_true",
+                    help="run only the RZ sweep section and print its JSON object")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the HBM copy probe (for rocprofv3 --pmc calibration)")
     args = ap.parse_args()
@@ -387,6 +486,13 @@ def main():
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     lib = _lib.load()
+    if args.only_rz:
+        GRAPHS[0] = not args.no_graphs
+        print(json.dumps({"rz_sweep": rz_sweep(args.rz[0], args.rz[1], 4, 4, args.groups,
+                                               args.steps, args.warmup,
+                                               0 if args.no_cpu else 8.0, log)}))
+        return
+
     if args.calibrate:
         n = 1 << 27
         ms = U.sweep.hbm_copy_probe(n, 5)
@@ -478,6 +584,10 @@ def main():
 
     if args.batch_domains:
         out["batched_domains"] = batched_domains(args.batch_domains, 20261205, log)
+
+    if args.rz[0] > 0:
+        out["rz_sweep"] = rz_sweep(args.rz[0], args.rz[1], 4, 4, G, args.steps, args.warmup,
+                                   0 if args.no_cpu else 8.0, log)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, G, st, S, args.cpu_seconds)

@@ -100,8 +100,13 @@ int umt_rz_ctx_source_iteration(umt_rz_ctx *ctx, double tol, int maxIters, int *
                                 double *residual, double *residual_history);
 /* Groups carried per lane (1, 2 or 4, dividing G); default the largest that divides G. */
 int umt_rz_ctx_set_groups_per_lane(umt_rz_ctx *ctx, int K);
-/* Mean wall time of one sweep (ms) and of its level kernels (ms) over reps
- * sweeps after warmup, HIP events on the context's stream. */
+/* Replay each sweep from a captured hipGraph (default on; kernel nodes
+ * only, rebuilt per groups-per-lane choice) or launch it eagerly. */
+int umt_rz_ctx_set_graphs(umt_rz_ctx *ctx, int on);
+/* Mean time of one sweep (ms) and of its level kernels alone (ms) over reps
+ * sweeps after warmup, HIP events on the context's stream (with graphs the
+ * level launches are replayed back to back from a graph of their own;
+ * without, each level is bracketed by events and the durations summed). */
 int umt_rz_ctx_time_sweeps(umt_rz_ctx *ctx, int warmup, int reps, double *msSweep,
                            double *msLevelKernels);
 

@@ -120,3 +120,28 @@ def test_rz_level_pipeline_is_shorter_than_serial_directions(prob):
     assert d["levels"] < serial / 2
     per_level = 2 * p._quad_orders[1] + 1  # swept directions of one xi level
     assert d["levels"] <= 2 * int(s["nHyperPlanes"].max()) + per_level
+
+
+def test_rz_graph_replay_equals_eager(prob):
+    """Replayed sweeps (with the program's own eager work in between) give
+    the eager sweep's bits, several times in a row."""
+    p, (m, g, q, s) = prob
+    G = 4
+    rng = np.random.default_rng(31)
+    sig = np.exp(rng.uniform(np.log(0.1), np.log(50.0), (p.nzones, G)))
+    sigs = 0.5 * sig
+    qq = rng.uniform(0.0, 1.0, (p.nzones, G))
+    runs = []
+    for graphs in (False, True):
+        ctx = U.RzContext(p, G)
+        ctx.set_graphs(graphs)
+        ctx.set_sigt(sig)
+        ctx.set_scattering(sigs, qq)
+        phis = []
+        for _ in range(3):
+            ctx.source_iteration(tol=0.0, max_iters=2)
+            phis.append(ctx.phi())
+        runs.append(phis)
+        ctx.close()
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)

@@ -133,6 +133,7 @@ SIGNATURES = {
     "umt_rz_ctx_source_iteration": (_i, [_vp, _d, _i, _pi, _pd, _pd]),
     "umt_rz_ctx_set_groups_per_lane": (_i, [_vp, _i]),
     "umt_rz_ctx_time_sweeps": (_i, [_vp, _i, _i, _pd, _pd]),
+    "umt_rz_ctx_set_graphs": (_i, [_vp, _i]),
 }
 
 _lib = None

@@ -44,6 +44,16 @@ struct umt_rz_ctx {
   void* psiM = nullptr;  // (G, C, xi levels)
   std::vector<hipEvent_t> evLevel;
   bool haveSigt = false, haveScattering = false;
+  // Captured sweeps: the level launches and the phi reduction (sweep), or
+  // the level launches alone (levels, for timing), per groups-per-lane K.
+  bool useGraphs = true;
+  struct Graph {
+    int K;
+    bool levelsOnly;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
   size_t rsize() const { return precision == 32 ? sizeof(float) : sizeof(double); }
   long stride() const { return (long)(C + nb) * G; }
   int levels() const { return (int)levelOffset.size() - 1; }
@@ -84,7 +94,7 @@ int upload(umt_rz_ctx* x, void* dst, const double* src, size_t n) {
 }
 
 template <typename Real>
-int sweep_impl(umt_rz_ctx* x, bool timeLevels) {
+int sweep_impl(umt_rz_ctx* x, bool timeLevels, bool levelsOnly = false) {
   umt::RzLevelArgs<Real> a{};
   a.G = x->G;
   a.C = x->C;
@@ -108,14 +118,58 @@ int sweep_impl(umt_rz_ctx* x, bool timeLevels) {
     HIP_TRY(umt::launch_rz_sweep_level<Real>(a, x->K, x->stream));
     if (timeLevels) HIP_TRY(hipEventRecord(x->evLevel[2 * h + 1], x->stream));
   }
-  HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)x->psi, x->weight, x->A, x->stride(),
-                                             (long)x->C * x->G, x->phi, false, x->stream));
+  if (!levelsOnly)
+    HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)x->psi, x->weight, x->A, x->stride(),
+                                               (long)x->C * x->G, x->phi, false, x->stream));
   return 0;
 }
 
-int sweep(umt_rz_ctx* x, bool timeLevels = false) {
+int launch_eager(umt_rz_ctx* x, bool timeLevels, bool levelsOnly) {
+  return x->precision == 32 ? sweep_impl<float>(x, timeLevels, levelsOnly)
+                            : sweep_impl<double>(x, timeLevels, levelsOnly);
+}
+
+void drop_graphs(umt_rz_ctx* x) {
+  for (auto& g : x->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  x->graphs.clear();
+}
+
+// The sweep's launches as a graph (kernel nodes only), built on first use
+// for the current groups per lane and replayed afterwards; the arrays it
+// reads and writes stay at fixed addresses for the context's lifetime.
+int sweep_graph(umt_rz_ctx* x, bool levelsOnly, hipGraphExec_t* out) {
+  for (auto& g : x->graphs)
+    if (g.K == x->K && g.levelsOnly == levelsOnly) {
+      *out = g.exec;
+      return 0;
+    }
+  umt_rz_ctx::Graph g{x->K, levelsOnly, nullptr, nullptr};
+  HIP_TRY(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  if (int rc = launch_eager(x, false, levelsOnly)) {
+    hipGraph_t dummy = nullptr;
+    (void)hipStreamEndCapture(x->stream, &dummy);
+    if (dummy) (void)hipGraphDestroy(dummy);
+    return rc;
+  }
+  HIP_TRY(hipStreamEndCapture(x->stream, &g.graph));
+  HIP_TRY(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  x->graphs.push_back(g);
+  *out = g.exec;
+  return 0;
+}
+
+int sweep(umt_rz_ctx* x, bool timeLevels = false, bool levelsOnly = false) {
   if (!x->haveSigt) return fail(30, "umt_rz_ctx_set_sigt first");
-  return x->precision == 32 ? sweep_impl<float>(x, timeLevels) : sweep_impl<double>(x, timeLevels);
+  if (x->useGraphs && !timeLevels) {
+    hipGraphExec_t exec = nullptr;
+    if (int rc = sweep_graph(x, levelsOnly, &exec)) return rc;
+    HIP_TRY(hipGraphLaunch(exec, x->stream));
+    return 0;
+  }
+  return launch_eager(x, timeLevels, levelsOnly);
 }
 
 int create_impl(const umt_rz_problem* p, int G, int precision, umt_rz_ctx* x) {
@@ -328,6 +382,7 @@ int umt_rz_problem_schedule(const umt_rz_problem* p, int* nHyperPlanes, int* zon
 void umt_rz_ctx_destroy(umt_rz_ctx* x) {
   if (!x) return;
   if (x->stream) (void)hipStreamSynchronize(x->stream);
+  drop_graphs(x);
   dfree(x->Afp); dfree(x->Aez); dfree(x->Rfp); dfree(x->Rez); dfree(x->area); dfree(x->vol);
   dfree(x->cFP); dfree(x->vol64); dfree(x->volZ); dfree(x->S); dfree(x->sigt); dfree(x->sigs);
   dfree(x->q); dfree(x->phi); dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
@@ -467,31 +522,50 @@ int umt_rz_ctx_set_groups_per_lane(umt_rz_ctx* x, int K) {
   return 0;
 }
 
+int umt_rz_ctx_set_graphs(umt_rz_ctx* x, int on) {
+  if (!x) return fail(1, "null context");
+  x->useGraphs = on != 0;
+  return 0;
+}
+
 int umt_rz_ctx_time_sweeps(umt_rz_ctx* x, int warmup, int reps, double* msSweep,
                            double* msLevelKernels) {
   if (!x || warmup < 0 || reps < 1) return fail(1, "bad argument");
   for (int i = 0; i < warmup; ++i)
     if (int rc = sweep(x)) return rc;
   HIP_TRY(hipStreamSynchronize(x->stream));
-  double kernelMs = 0.0;
+  float t = 0.f;
   HIP_TRY(hipEventRecord(x->ev0, x->stream));
   for (int r = 0; r < reps; ++r)
     if (int rc = sweep(x)) return rc;
   HIP_TRY(hipEventRecord(x->ev1, x->stream));
   HIP_TRY(hipEventSynchronize(x->ev1));
-  float t = 0.f;
   HIP_TRY(hipEventElapsedTime(&t, x->ev0, x->ev1));
+  if (msSweep) *msSweep = t / reps;
   if (msLevelKernels) {
-    if (int rc = sweep(x, true)) return rc;
-    HIP_TRY(hipStreamSynchronize(x->stream));
-    for (int h = 0; h < x->levels(); ++h) {
+    double kernelMs = 0.0;
+    if (x->useGraphs) {
+      // The level launches alone, replayed back to back from a graph.
+      if (int rc = sweep(x, false, true)) return rc;
+      HIP_TRY(hipEventRecord(x->ev0, x->stream));
+      for (int r = 0; r < reps; ++r)
+        if (int rc = sweep(x, false, true)) return rc;
+      HIP_TRY(hipEventRecord(x->ev1, x->stream));
+      HIP_TRY(hipEventSynchronize(x->ev1));
       float lt = 0.f;
-      HIP_TRY(hipEventElapsedTime(&lt, x->evLevel[2 * h], x->evLevel[2 * h + 1]));
-      kernelMs += lt;
+      HIP_TRY(hipEventElapsedTime(&lt, x->ev0, x->ev1));
+      kernelMs = lt / reps;
+    } else {
+      if (int rc = sweep(x, true)) return rc;
+      HIP_TRY(hipStreamSynchronize(x->stream));
+      for (int h = 0; h < x->levels(); ++h) {
+        float lt = 0.f;
+        HIP_TRY(hipEventElapsedTime(&lt, x->evLevel[2 * h], x->evLevel[2 * h + 1]));
+        kernelMs += lt;
+      }
     }
     *msLevelKernels = kernelMs;
   }
-  if (msSweep) *msSweep = t / reps;
   return 0;
 }
 

@@ -196,6 +196,10 @@ class RzContext:
         check("umt_rz_ctx_get_psi", _lib.load().umt_rz_ctx_get_psi(self._h, _dp(out)))
         return out
 
+    def set_graphs(self, on):
+        """Replay sweeps from a captured graph (default) or launch them eagerly."""
+        check("umt_rz_ctx_set_graphs", _lib.load().umt_rz_ctx_set_graphs(self._h, int(bool(on))))
+
     def set_groups_per_lane(self, K):
         check("umt_rz_ctx_set_groups_per_lane", _lib.load().umt_rz_ctx_set_groups_per_lane(self._h, K))
 
