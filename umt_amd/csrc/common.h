@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 namespace umt {
 
@@ -23,6 +24,16 @@ int dalloc(T** p, size_t n) {
   if (e != hipSuccess)
     return fail(100 + (int)e, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
   return 0;
+}
+
+// Component-major copy of an array of n records of k components each:
+// out[j * n + i] = in[i * k + j] (Teton's corner arrays are record-major).
+template <typename T>
+std::vector<T> component_major(const T* in, int k, size_t n) {
+  std::vector<T> out((size_t)k * n);
+  for (size_t i = 0; i < n; ++i)
+    for (int j = 0; j < k; ++j) out[(size_t)j * n + i] = in[i * k + j];
+  return out;
 }
 
 template <typename T>

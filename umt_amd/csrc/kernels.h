@@ -12,10 +12,10 @@ struct SweepLevelArgs {
   int G;                    // groups
   int groupsPerLane;        // K: groups one lane carries (1, 2 or 4; divides G)
   int C;                    // corners; cFP values above C are boundary elements
-  const Real* Afp;          // (3, 3, C)
-  const Real* Aez;          // (3, 3, C)
+  const Real* Afp;          // component-major: A_fp(d, f, c) at (3 f + d) C + c
+  const Real* Aez;          // component-major like Afp
   const Real* vol;          // (C)
-  const int* cFP;           // (3, C), 1-based contents
+  const int* cFP;           // face-major: cFP(f, c) at f C + c, 1-based contents
   const Real* S;            // (G, C) source per steradian
   const Real* sigt;         // (G, Z)
   const double* omega;      // (3, A)
@@ -37,10 +37,10 @@ hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, h
 // Many domains in one level launch (batches of uneven domains).
 template <typename Real>
 struct BatchDomain {
-  const Real* Afp;          // (3, 3, C) of this domain
+  const Real* Afp;          // component-major (as SweepLevelArgs) of this domain
   const Real* Aez;
   const Real* vol;
-  const int* cFP;           // (3, C), 1-based within the domain
+  const int* cFP;           // face-major, 1-based within the domain
   const Real* S;            // (G, C)
   const Real* sigt;         // (G, Z)
   const double* omega;      // (3, A)

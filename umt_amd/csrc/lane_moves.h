@@ -6,10 +6,12 @@ namespace umt {
 
 // DPP moves of 32-bit lanes (64-bit values move as two halves). kCtrl is
 // the dpp_ctrl encoding: quad_perm patterns move values inside groups of
-// four lanes, row shifts across a row of sixteen.
+// four lanes, row shifts and mirrors within a row of sixteen. A source
+// outside the row reads 0 (bound_ctrl), so no old value has to be set up.
+// The sweep kernels call these with every lane of the wave active.
 template <int kCtrl>
 __device__ __forceinline__ int dpp_i32(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, true);
 }
 template <int kCtrl>
 __device__ __forceinline__ double dpp_move(double x) {
@@ -29,6 +31,7 @@ constexpr int kQuadPrev = 0x93;  // quad_perm [3,0,1,2]: value of lane (i - 1) m
 constexpr int kQuadNext = 0x39;  // quad_perm [1,2,3,0]: value of lane (i + 1) mod 4
 constexpr int kRowShl4 = 0x104;
 constexpr int kRowShr4 = 0x114;
+constexpr int kRowHalfMirror = 0x141;  // row_half_mirror: lane i <-> 7 - i within each 8 lanes
 
 // K consecutive groups of one corner as one aligned vector access.
 template <typename Real, int K>

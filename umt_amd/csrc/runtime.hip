@@ -359,7 +359,9 @@ int sweep(umt_ctx* x, bool timeLevels) {
 int upload_real(umt_ctx* x, void* dst, const double* src, size_t n) {
   if (n == 0) return 0;
   if (x->precision == 64) {
+    // Synchronous: callers pass temporaries (transposed copies).
     HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
   } else {
     std::vector<float> tmp(n);
     for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
@@ -841,10 +843,13 @@ static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, co
       (rc = dalloc(&x->phibarOld, (size_t)G * Z)) || (rc = dalloc(&x->bits, 2)))
     return rc;
 
-  if ((rc = upload_real(x, x->Afp, A_fp, 9 * (size_t)C)) || (rc = upload_real(x, x->Aez, A_ez, 9 * (size_t)C)) ||
+  if ((rc = upload_real(x, x->Afp, umt::component_major(A_fp, 9, (size_t)C).data(), 9 * (size_t)C)) ||
+      (rc = upload_real(x, x->Aez, umt::component_major(A_ez, 9, (size_t)C).data(), 9 * (size_t)C)) ||
       (rc = upload_real(x, x->vol, Volume, (size_t)C)))
     return rc;
-  HIP_TRY(hipMemcpyAsync(x->cFP, cFP, 3 * (size_t)C * sizeof(int), hipMemcpyHostToDevice, x->stream));
+  const std::vector<int> cFPdev = umt::component_major(cFP, 3, (size_t)C);
+  HIP_TRY(hipMemcpyAsync(x->cFP, cFPdev.data(), 3 * (size_t)C * sizeof(int), hipMemcpyHostToDevice, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
   x->cFPHost.assign(cFP, cFP + 3 * (size_t)C);
   HIP_TRY(hipMemcpyAsync(x->vol64, Volume, (size_t)C * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipMemcpyAsync(x->volZ, VolumeZone, (size_t)Z * sizeof(double), hipMemcpyHostToDevice, x->stream));
@@ -2236,10 +2241,13 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   if (int rc = dropin_alloc(d, C, Z, G, nb)) return rc;
   hipStream_t s = d.stream;
   const size_t GC = (size_t)G * C;
-  HIP_TRY(hipMemcpyAsync(d.Afp, A_fp, 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(d.Aez, A_ez, 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
+  const std::vector<double> afpDev = umt::component_major(A_fp, 9, (size_t)C);
+  const std::vector<double> aezDev = umt::component_major(A_ez, 9, (size_t)C);
+  const std::vector<int> cfpDev = umt::component_major(cFP, 3, (size_t)C);
+  HIP_TRY(hipMemcpyAsync(d.Afp, afpDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.Aez, aezDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.vol, Volume, (size_t)C * 8, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(d.cFP, cFP, 3 * (size_t)C * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d.cFP, cfpDev.data(), 3 * (size_t)C * 4, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.S, STotal, GC * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.sigt, Sigt, (size_t)G * Z * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.omega, omega, 3 * 8, hipMemcpyHostToDevice, s));

@@ -32,30 +32,27 @@ namespace umt {
 //   {3,1,4} {0,2,5} {1,3,6} {2,0,7} {5,7,0} {6,4,1} {7,5,2} {4,6,3}
 // for c = 0..7 (corner-face chaining of aux/setTetonZone.F90:172-189). The
 // runtime checks every zone against this table before launching them.
+// Omega . A for corner-face vector components stored stride apart
+// (the device keeps geometry component-major: entry (k, cc) at k * C + cc).
 template <typename Real>
-__device__ __forceinline__ Real dot3(Real ox, Real oy, Real oz, const Real* a) {
-  return ox * a[0] + oy * a[1] + oz * a[2];
+__device__ __forceinline__ Real dot3(Real ox, Real oy, Real oz, const Real* a, long stride) {
+  return ox * a[0] + oy * a[stride] + oz * a[2 * stride];
 }
 
-// Cross-lane moves inside the 8-lane group of one (item, group) pair, lane
-// c = corner c (lane_moves.h). kEZ[c][0] is the previous corner of the
-// bottom quad (quad_perm [3,0,1,2]) but the next corner of the top quad
-// (quad_perm [1,2,3,0]); kEZ[c][1] is the inverse; kEZ[c][2] = c ^ 4
-// (row_shl:4 for the bottom quad, row_shr:4 for the top quad).
+// Cross-lane moves inside the 8-lane group of one (item, group) pair. Lane
+// l holds corner l for l < 4 and corner 11 - l for l >= 4 (the top quad in
+// reverse), so every EZ neighbour is one uniform DPP move (lane_moves.h):
+// slot 0 (kEZ[c][0]) is the previous lane of the quad (quad_perm
+// [3,0,1,2]), slot 1 the next lane (quad_perm [1,2,3,0]), slot 2 (c ^ 4)
+// the mirror lane of the half row (row_half_mirror: lane l <-> 7 - l).
+__device__ __forceinline__ int corner_of_lane(int l) { return l < 4 ? l : 11 - l; }
 
 // Value held by this lane's EZ neighbour in corner-face slot F.
 template <int F, typename T>
-__device__ __forceinline__ T from_ez(T x, bool bottom) {
-  if constexpr (F == 0) {
-    const T prev = dpp_move<kQuadPrev>(x), next = dpp_move<kQuadNext>(x);
-    return bottom ? prev : next;
-  } else if constexpr (F == 1) {
-    const T prev = dpp_move<kQuadPrev>(x), next = dpp_move<kQuadNext>(x);
-    return bottom ? next : prev;
-  } else {
-    const T up = dpp_move<kRowShl4>(x), down = dpp_move<kRowShr4>(x);
-    return bottom ? up : down;
-  }
+__device__ __forceinline__ T from_ez(T x) {
+  if constexpr (F == 0) return dpp_move<kQuadPrev>(x);
+  else if constexpr (F == 1) return dpp_move<kQuadNext>(x);
+  else return dpp_move<kRowHalfMirror>(x);
 }
 
 template <typename Real>
@@ -82,8 +79,7 @@ __device__ __forceinline__ Real sez_term(Real A, Real sig, Real v, Real halfSigI
 template <typename Real, int K, bool kSingleAngle>
 __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int item, int g0,
                                              bool valid) {
-  const int c = threadIdx.x & 7;
-  const bool bottom = c < 4;
+  const int c = corner_of_lane(threadIdx.x & 7);
   const int G = p.G;
   const int2 it = p.items[item];
   // A zone whose corners depend on each other in a cycle comes as ~zone.
@@ -97,13 +93,17 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   Real* __restrict__ psiA = p.psi + angleBase;
 
   const Real v = p.vol[cc];
+  // Geometry is component-major on the device (component k of corner cc at
+  // k * C + cc, k = 3 * face + dim), so the eight corners of a zone read
+  // eight consecutive values per load.
+  const long C = p.C;
   Real afp[3], aez[3];
   int cfp[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    afp[f] = dot3(ox, oy, oz, p.Afp + 3 * (3 * cc + f));
-    aez[f] = dot3(ox, oy, oz, p.Aez + 3 * (3 * cc + f));
-    cfp[f] = p.cFP[3 * cc + f] - 1;
+    afp[f] = dot3(ox, oy, oz, p.Afp + 3 * f * C + cc, C);
+    aez[f] = dot3(ox, oy, oz, p.Aez + 3 * f * C + cc, C);
+    cfp[f] = p.cFP[f * C + cc] - 1;
   }
   if (p.parTol > 0) {
     // Directions lying in a mesh face leave rounding noise of either sign in
@@ -111,10 +111,10 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
     // read across them and they cannot select the opposite-face EZ branch.
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      const Real* af = p.Afp + 3 * (3 * cc + f);
-      const Real* ae = p.Aez + 3 * (3 * cc + f);
-      if (fabs(afp[f]) <= p.parTol * (fabs(af[0]) + fabs(af[1]) + fabs(af[2]))) afp[f] = 0;
-      if (fabs(aez[f]) <= p.parTol * (fabs(ae[0]) + fabs(ae[1]) + fabs(ae[2]))) aez[f] = 0;
+      const Real* af = p.Afp + 3 * f * C + cc;
+      const Real* ae = p.Aez + 3 * f * C + cc;
+      if (fabs(afp[f]) <= p.parTol * (fabs(af[0]) + fabs(af[C]) + fabs(af[2 * C]))) afp[f] = 0;
+      if (fabs(aez[f]) <= p.parTol * (fabs(ae[0]) + fabs(ae[C]) + fabs(ae[2 * C]))) aez[f] = 0;
     }
   }
   Real outArea = 0;
@@ -165,7 +165,7 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
     const Real sig = sigv_.v[k];
     const Real halfSigInv = (Real)0.5 * ((Real)1 / sig);
     const Real q = Q.v[k];
-    const Real Qn[3] = {from_ez<0>(q, bottom), from_ez<1>(q, bottom), from_ez<2>(q, bottom)};
+    const Real Qn[3] = {from_ez<0>(q), from_ez<1>(q), from_ez<2>(q)};
     Real sezOut[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
@@ -179,9 +179,9 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
     s += sezOut[1];
     s += sezOut[2];
     // The neighbour in slot f sees this edge in slot 1, 0, 2 for f = 0, 1, 2.
-    s -= from_ez<0>(sezOut[1], bottom);
-    s -= from_ez<1>(sezOut[0], bottom);
-    s -= from_ez<2>(sezOut[2], bottom);
+    s -= from_ez<0>(sezOut[1]);
+    s -= from_ez<1>(sezOut[0]);
+    s -= from_ez<2>(sezOut[2]);
     src[k] = s;
   }
 
@@ -204,8 +204,8 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const Real x = cyclic ? old.v[k] : (Real)0;
-      const Real n0 = from_ez<0>(x, bottom), n1 = from_ez<1>(x, bottom),
-                 n2 = from_ez<2>(x, bottom);
+      const Real n0 = from_ez<0>(x), n1 = from_ez<1>(x),
+                 n2 = from_ez<2>(x);
       const Real once = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
       psi.v[k] = cyclic ? once : psi.v[k];
     }
@@ -213,13 +213,13 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   }
   for (int iter = 0; iter < 8; ++iter) {
     if (__all(fin)) break;
-    const int f0 = from_ez<0>(fin, bottom), f1 = from_ez<1>(fin, bottom),
-              f2 = from_ez<2>(fin, bottom);
+    const int f0 = from_ez<0>(fin), f1 = from_ez<1>(fin),
+              f2 = from_ez<2>(fin);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const Real x = psi.v[k];
-      const Real n0 = from_ez<0>(x, bottom), n1 = from_ez<1>(x, bottom),
-                 n2 = from_ez<2>(x, bottom);
+      const Real n0 = from_ez<0>(x), n1 = from_ez<1>(x),
+                 n2 = from_ez<2>(x);
       const Real upd = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
       psi.v[k] = cyclic ? psi.v[k] : upd;  // a cyclic zone keeps its single solve
     }
