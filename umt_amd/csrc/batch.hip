@@ -327,11 +327,11 @@ int create_domain(umt_batch* b, const umt_problem* p, int G, Domain& d) {
       (rc = dalloc(&d.phibar, Gp * Z)) || (rc = dalloc(&d.phibarOld, Gp * Z)) ||
       (rc = dalloc(&d.omega, 3 * (size_t)d.A)) || (rc = dalloc(&d.weight, (size_t)d.A)))
     return rc;
-  if ((rc = upload_real(b, d.Afp, umt::component_major(g.A_fp.data(), 9, C).data(), 9 * C)) ||
-      (rc = upload_real(b, d.Aez, umt::component_major(g.A_ez.data(), 9, C).data(), 9 * C)) ||
+  if ((rc = upload_real(b, d.Afp, umt::zone_blocked(g.A_fp.data(), 9, C).data(), 9 * C)) ||
+      (rc = upload_real(b, d.Aez, umt::zone_blocked(g.A_ez.data(), 9, C).data(), 9 * C)) ||
       (rc = upload_real(b, d.vol, g.Volume.data(), C)))
     return rc;
-  const std::vector<int> cfpDev = umt::component_major(m.cFP.data(), 3, C);
+  const std::vector<int> cfpDev = umt::zone_blocked(m.cFP.data(), 3, C);
   HIP_TRY(hipMemcpyAsync(d.cFP, cfpDev.data(), 3 * C * sizeof(int), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipMemcpyAsync(d.vol64, g.Volume.data(), C * sizeof(double), hipMemcpyHostToDevice, b->stream));

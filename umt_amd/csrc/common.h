@@ -26,13 +26,17 @@ int dalloc(T** p, size_t n) {
   return 0;
 }
 
-// Component-major copy of an array of n records of k components each:
-// out[j * n + i] = in[i * k + j] (Teton's corner arrays are record-major).
+// Zone-blocked copy of a per-corner array of k components (hex zones of 8
+// corners, corner-major as Teton stores it): component j of corner 8 z + c
+// goes to (z k + j) 8 + c, so each zone's block is contiguous and holds
+// each component for its eight corners side by side.
 template <typename T>
-std::vector<T> component_major(const T* in, int k, size_t n) {
-  std::vector<T> out((size_t)k * n);
-  for (size_t i = 0; i < n; ++i)
-    for (int j = 0; j < k; ++j) out[(size_t)j * n + i] = in[i * k + j];
+std::vector<T> zone_blocked(const T* in, int k, size_t ncorner) {
+  std::vector<T> out((size_t)k * ncorner);
+  for (size_t cc = 0; cc < ncorner; ++cc) {
+    const size_t z = cc / 8, c = cc % 8;
+    for (int j = 0; j < k; ++j) out[(z * k + j) * 8 + c] = in[cc * k + j];
+  }
   return out;
 }
 

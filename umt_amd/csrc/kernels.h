@@ -12,10 +12,10 @@ struct SweepLevelArgs {
   int G;                    // groups
   int groupsPerLane;        // K: groups one lane carries (1, 2 or 4; divides G)
   int C;                    // corners; cFP values above C are boundary elements
-  const Real* Afp;          // component-major: A_fp(d, f, c) at (3 f + d) C + c
-  const Real* Aez;          // component-major like Afp
+  const Real* Afp;          // zone-blocked: A_fp(d, f, 8 z + c) at (9 z + 3 f + d) 8 + c
+  const Real* Aez;          // zone-blocked like Afp
   const Real* vol;          // (C)
-  const int* cFP;           // face-major: cFP(f, c) at f C + c, 1-based contents
+  const int* cFP;           // zone-blocked: cFP(f, 8 z + c) at (3 z + f) 8 + c, 1-based
   const Real* S;            // (G, C) source per steradian
   const Real* sigt;         // (G, Z)
   const double* omega;      // (3, A)
@@ -37,10 +37,10 @@ hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, h
 // Many domains in one level launch (batches of uneven domains).
 template <typename Real>
 struct BatchDomain {
-  const Real* Afp;          // component-major (as SweepLevelArgs) of this domain
+  const Real* Afp;          // zone-blocked (as SweepLevelArgs) of this domain
   const Real* Aez;
   const Real* vol;
-  const int* cFP;           // face-major, 1-based within the domain
+  const int* cFP;           // zone-blocked, 1-based within the domain
   const Real* S;            // (G, C)
   const Real* sigt;         // (G, Z)
   const double* omega;      // (3, A)

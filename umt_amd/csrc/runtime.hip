@@ -843,11 +843,11 @@ static int ctx_create_impl(int Z, int C, int nb, int G, int A, int precision, co
       (rc = dalloc(&x->phibarOld, (size_t)G * Z)) || (rc = dalloc(&x->bits, 2)))
     return rc;
 
-  if ((rc = upload_real(x, x->Afp, umt::component_major(A_fp, 9, (size_t)C).data(), 9 * (size_t)C)) ||
-      (rc = upload_real(x, x->Aez, umt::component_major(A_ez, 9, (size_t)C).data(), 9 * (size_t)C)) ||
+  if ((rc = upload_real(x, x->Afp, umt::zone_blocked(A_fp, 9, (size_t)C).data(), 9 * (size_t)C)) ||
+      (rc = upload_real(x, x->Aez, umt::zone_blocked(A_ez, 9, (size_t)C).data(), 9 * (size_t)C)) ||
       (rc = upload_real(x, x->vol, Volume, (size_t)C)))
     return rc;
-  const std::vector<int> cFPdev = umt::component_major(cFP, 3, (size_t)C);
+  const std::vector<int> cFPdev = umt::zone_blocked(cFP, 3, (size_t)C);
   HIP_TRY(hipMemcpyAsync(x->cFP, cFPdev.data(), 3 * (size_t)C * sizeof(int), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
   x->cFPHost.assign(cFP, cFP + 3 * (size_t)C);
@@ -2241,9 +2241,9 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   if (int rc = dropin_alloc(d, C, Z, G, nb)) return rc;
   hipStream_t s = d.stream;
   const size_t GC = (size_t)G * C;
-  const std::vector<double> afpDev = umt::component_major(A_fp, 9, (size_t)C);
-  const std::vector<double> aezDev = umt::component_major(A_ez, 9, (size_t)C);
-  const std::vector<int> cfpDev = umt::component_major(cFP, 3, (size_t)C);
+  const std::vector<double> afpDev = umt::zone_blocked(A_fp, 9, (size_t)C);
+  const std::vector<double> aezDev = umt::zone_blocked(A_ez, 9, (size_t)C);
+  const std::vector<int> cfpDev = umt::zone_blocked(cFP, 3, (size_t)C);
   HIP_TRY(hipMemcpyAsync(d.Afp, afpDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.Aez, aezDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.vol, Volume, (size_t)C * 8, hipMemcpyHostToDevice, s));

@@ -32,8 +32,9 @@ namespace umt {
 //   {3,1,4} {0,2,5} {1,3,6} {2,0,7} {5,7,0} {6,4,1} {7,5,2} {4,6,3}
 // for c = 0..7 (corner-face chaining of aux/setTetonZone.F90:172-189). The
 // runtime checks every zone against this table before launching them.
-// Omega . A for corner-face vector components stored stride apart
-// (the device keeps geometry component-major: entry (k, cc) at k * C + cc).
+// Omega . A for corner-face vector components stored stride apart (the
+// device keeps geometry zone-blocked: component k of corner 8 z + c at
+// (9 z + k) 8 + c).
 template <typename Real>
 __device__ __forceinline__ Real dot3(Real ox, Real oy, Real oz, const Real* a, long stride) {
   return ox * a[0] + oy * a[stride] + oz * a[2 * stride];
@@ -93,17 +94,19 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   Real* __restrict__ psiA = p.psi + angleBase;
 
   const Real v = p.vol[cc];
-  // Geometry is component-major on the device (component k of corner cc at
-  // k * C + cc, k = 3 * face + dim), so the eight corners of a zone read
-  // eight consecutive values per load.
-  const long C = p.C;
+  // Geometry is zone-blocked on the device (component k = 3 face + dim of
+  // corner c at (9 z + k) 8 + c, cFP face f at (3 z + f) 8 + c): each load
+  // reads the same component of the zone's eight corners side by side, and
+  // a zone's block is contiguous.
+  const Real* Afz = p.Afp + 72L * z + c;
+  const Real* Aez = p.Aez + 72L * z + c;
   Real afp[3], aez[3];
   int cfp[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    afp[f] = dot3(ox, oy, oz, p.Afp + 3 * f * C + cc, C);
-    aez[f] = dot3(ox, oy, oz, p.Aez + 3 * f * C + cc, C);
-    cfp[f] = p.cFP[f * C + cc] - 1;
+    afp[f] = dot3(ox, oy, oz, Afz + 24 * f, 8);
+    aez[f] = dot3(ox, oy, oz, Aez + 24 * f, 8);
+    cfp[f] = p.cFP[24L * z + 8 * f + c] - 1;
   }
   if (p.parTol > 0) {
     // Directions lying in a mesh face leave rounding noise of either sign in
@@ -111,10 +114,10 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
     // read across them and they cannot select the opposite-face EZ branch.
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      const Real* af = p.Afp + 3 * f * C + cc;
-      const Real* ae = p.Aez + 3 * f * C + cc;
-      if (fabs(afp[f]) <= p.parTol * (fabs(af[0]) + fabs(af[C]) + fabs(af[2 * C]))) afp[f] = 0;
-      if (fabs(aez[f]) <= p.parTol * (fabs(ae[0]) + fabs(ae[C]) + fabs(ae[2 * C]))) aez[f] = 0;
+      const Real* af = Afz + 24 * f;
+      const Real* ae = Aez + 24 * f;
+      if (fabs(afp[f]) <= p.parTol * (fabs(af[0]) + fabs(af[8]) + fabs(af[16]))) afp[f] = 0;
+      if (fabs(aez[f]) <= p.parTol * (fabs(ae[0]) + fabs(ae[8]) + fabs(ae[16]))) aez[f] = 0;
     }
   }
   Real outArea = 0;
