@@ -98,6 +98,19 @@ int umt_rz_ctx_get_psi(const umt_rz_ctx *ctx, double *psi);
  * converged, 1 when maxIters was reached. */
 int umt_rz_ctx_source_iteration(umt_rz_ctx *ctx, double tol, int maxIters, int *iters,
                                 double *residual, double *residual_history);
+/* Time-dependent sweeps. tau = 1/(c dt) (control/initializeSets.F90:81):
+ * the sweep sees Sigt + tau (control/setTotalOpacity.F90) and the source
+ * STotal + tau Psi(g, c, Angle) (snac/SweepUCBrz.F90:136), Psi being the
+ * previous cycle's angular flux held on the device for every direction.
+ * tau = 0 (default) is the steady problem. umt_rz_ctx_save_psi is the
+ * cycle's final sweep with savePsi (rt/rtmainsn.F90:238-246,
+ * SweepUCBrz.F90:268-276): source from the current phi, one sweep, Psi = its
+ * psi (finishing directions: the last half-angle flux), phi = its phi.
+ * Psi arrays on the host are Teton's Psi(G, C, A). */
+int umt_rz_ctx_set_tau(umt_rz_ctx *ctx, double tau);
+int umt_rz_ctx_set_psi_prev(umt_rz_ctx *ctx, const double *Psi);
+int umt_rz_ctx_get_psi_prev(const umt_rz_ctx *ctx, double *Psi);
+int umt_rz_ctx_save_psi(umt_rz_ctx *ctx);
 /* Groups carried per lane (1, 2 or 4, dividing G); default the largest that divides G. */
 int umt_rz_ctx_set_groups_per_lane(umt_rz_ctx *ctx, int K);
 /* Replay each sweep from a captured hipGraph (default on; kernel nodes

@@ -464,7 +464,9 @@ void oracle_rz_sweep_angle(int nzones, int ncorner, int nbelem, int Groups, cons
 /* snac/SetSweep.F90:90-170 for one RZ domain with vacuum boundaries: PsiM
  * and Phi start at zero, the angles of each xi level are swept in order
  * (finishing directions are not swept), xi levels shared among threads
- * with a Phi per thread summed in thread order. Psi (G, C, A) receives every
+ * with a Phi per thread summed in thread order. With PsiPrev (G, C, A) not
+ * NULL, each direction's source is STotal + tau PsiPrev(:, :, Angle)
+ * (SweepUCBrz.F90:136; Sigt is then the caller's sigma_t + tau). Psi (G, C, A) receives every
  * angle's psi when not NULL (finishing directions: the last PsiM). */
 void oracle_rz_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, const int *numCorner,
                          const int *cOffSet, const int *cFP, const int *cEZ, const double *A_fp,
@@ -474,7 +476,8 @@ void oracle_rz_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, cons
                          const int *starting, const int *finishing, const double *angDerivFac,
                          const double *quadTauW1, const double *quadTauW2,
                          const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ,
-                         const int *nextC, int nthreads, double *PsiB, double *Phi, double *Psi) {
+                         const int *nextC, double tau, const double *PsiPrev, int nthreads,
+                         double *PsiB, double *Phi, double *Psi) {
   const size_t nGC = (size_t)G * ncorner, nPsi1 = (size_t)G * (ncorner + nbelem);
   int *planeStart = malloc(sizeof(int) * (A + 1));
   planeStart[0] = 0;
@@ -507,7 +510,8 @@ void oracle_rz_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, cons
         if (finishing[a]) continue;
         const int finNext = a + 1 < A && finishing[a + 1];
         oracle_rz_sweep_angle(nzones, ncorner, nbelem, G, numCorner, cOffSet, cFP, cEZ, A_fp,
-                              A_ez, RadiusFP, RadiusEZ, Area, Volume, STotal, Sigt, 0.0, NULL,
+                              A_ez, RadiusFP, RadiusEZ, Area, Volume, STotal, Sigt,
+                              PsiPrev ? tau : 0.0, PsiPrev ? &PsiPrev[(size_t)a * nGC] : NULL,
                               &omega[2 * (size_t)a], weight[a], angDerivFac[a], quadTauW1[a],
                               quadTauW2[a], starting[a], nHyperPlanes[a],
                               &zonesInPlane[planeStart[a]], &nextZ[(size_t)a * nzones],
@@ -571,7 +575,7 @@ int oracle_rz_source_iteration(int nzones, int ncorner, int nbelem, int G, int A
     oracle_rz_sweep_all(nzones, ncorner, nbelem, G, A, numCorner, cOffSet, cFP, cEZ, A_fp, A_ez,
                         RadiusFP, RadiusEZ, Area, Volume, STotal, Sigt, omega, weight, starting,
                         finishing, angDerivFac, quadTauW1, quadTauW2, nHyperPlanes, zonesInPlane,
-                        nextZ, nextC, nthreads, PsiB, Phi, NULL);
+                        nextZ, nextC, 0.0, NULL, nthreads, PsiB, Phi, NULL);
     double mx = 0.0;
     for (int z = 1; z <= nzones; ++z)
       for (int g = 1; g <= G; ++g) {

@@ -94,7 +94,8 @@ void oracle_rz_sweep_all(int nzones, int ncorner, int nbelem, int G, int A, cons
                          const int *starting, const int *finishing, const double *angDerivFac,
                          const double *quadTauW1, const double *quadTauW2,
                          const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ,
-                         const int *nextC, int nthreads, double *PsiB, double *Phi, double *Psi);
+                         const int *nextC, double tau, const double *PsiPrev, int nthreads,
+                         double *PsiB, double *Phi, double *Psi);
 
 /* Steady source iteration in RZ (STotal = (sigs phi + q) / (2 pi)), residual
  * as oracle_source_iteration over four-corner zone averages. */

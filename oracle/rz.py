@@ -28,7 +28,7 @@ _SIG = {
                                       _pi, _pi, _pi, _pi]),
     "oracle_rz_sweep_all": (None, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd,
                                    _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pi, _pd, _pd, _pd, _pi,
-                                   _pi, _pi, _pi, _i, _pd, _pd, _pd]),
+                                   _pi, _pi, _pi, _d, _pd, _i, _pd, _pd, _pd]),
     "oracle_rz_source_iteration": (_i, [_i, _i, _i, _i, _i, _pi, _pi, _pi, _pi, _pd, _pd, _pd,
                                         _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pi,
                                         _pd, _pd, _pd, _pi, _pi, _pi, _pi, _i, _d, _i, _pd, _pd,
@@ -204,9 +204,10 @@ def schedule(mesh, geom, quad):
 
 
 def sweep_all(mesh, geom, quad, sched, STotal, Sigt, nthreads=1, PsiB=None, fast=False,
-              want_psi=False):
-    """All directions (SetSweep semantics). Returns (Phi (C,G), PsiB (A,nb,G),
-    Psi (A,C,G) or None)."""
+              want_psi=False, tau=0.0, psi_prev=None):
+    """All directions (SetSweep semantics). With psi_prev (A, C, G) each
+    direction's source is STotal + tau psi_prev[a] (Sigt is then sigma_t +
+    tau). Returns (Phi (C,G), PsiB (A,nb,G), Psi (A,C,G) or None)."""
     Z = mesh["numCorner"].shape[0]
     Cn = mesh["cFP"].shape[0]
     G = STotal.shape[1]
@@ -216,6 +217,9 @@ def sweep_all(mesh, geom, quad, sched, STotal, Sigt, nthreads=1, PsiB=None, fast
         PsiB = np.zeros((A, nb, G))
     Phi = np.zeros((Cn, G))
     Psi = np.zeros((A, Cn, G)) if want_psi else None
+    prev = None if psi_prev is None else np.ascontiguousarray(psi_prev, dtype=np.float64)
+    if prev is not None:
+        assert prev.shape == (A, Cn, G)
     lib(fast).oracle_rz_sweep_all(
         Z, Cn, nb, G, A, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["cFP"]),
         _ip(mesh["cEZ"]), _dp(geom["A_fp"]), _dp(geom["A_ez"]), _dp(geom["RadiusFP"]),
@@ -224,7 +228,7 @@ def sweep_all(mesh, geom, quad, sched, STotal, Sigt, nthreads=1, PsiB=None, fast
         _dp(quad["weight"]), _ip(quad["starting"]), _ip(quad["finishing"]),
         _dp(quad["angDerivFac"]), _dp(quad["quadTauW1"]), _dp(quad["quadTauW2"]),
         _ip(sched["nHyperPlanes"]), _ip(sched["zonesInPlane"]), _ip(sched["nextZ"]),
-        _ip(sched["nextC"]), nthreads, _dp(PsiB), _dp(Phi), _dp(Psi))
+        _ip(sched["nextC"]), float(tau), _dp(prev), nthreads, _dp(PsiB), _dp(Phi), _dp(Psi))
     return Phi, PsiB, Psi
 
 
@@ -249,6 +253,47 @@ def source_iteration(mesh, geom, quad, sched, Sigt, sigs, q, tol, max_iters, phi
         _ip(sched["zonesInPlane"]), _ip(sched["nextZ"]), _ip(sched["nextC"]), nthreads, tol,
         max_iters, _dp(Phi), C.byref(res), _dp(hist))
     return Phi, it, res.value, hist[:it].copy()
+
+
+def time_step(mesh, geom, quad, sched, sigt, sigs, q, tau, psi_prev, tol, max_iters, phi0=None):
+    """One implicit time step of the RZ sweep: source iteration with the
+    sweep of snac/SweepUCBrz.F90:130-137 for tau > 0 (total Sigt + tau,
+    control/setTotalOpacity.F90; source STotal + tau Psi(:,:,Angle) with Psi
+    the previous cycle's psi_prev (A, C, G)), residual as in
+    source_iteration, then the cycle's final sweep with savePsi
+    (rt/rtmainsn.F90:238-246, SweepUCBrz.F90:268-276: every swept direction's
+    psi, finishing directions the last half-angle flux).
+    Returns (phi (C, G), psi_new (A, C, G), iterations, residual, history)."""
+    Cn = mesh["cFP"].shape[0]
+    G = sigt.shape[1]
+    V, VZ = geom["Volume"], geom["VolumeZone"]
+    st = np.ascontiguousarray(sigt + tau)
+    sigs_c, q_c = np.repeat(sigs, 4, axis=0), np.repeat(q, 4, axis=0)
+
+    def sweep(phi, save):
+        S = (sigs_c * phi + q_c) / (2.0 * np.pi)
+        Phi, _, Psi = sweep_all(mesh, geom, quad, sched, S, st, want_psi=save, tau=tau,
+                                psi_prev=psi_prev)
+        return Phi, Psi
+
+    def zavg(x):
+        return (x * V[:, None]).reshape(-1, 4, G).sum(1) / VZ[:, None]
+
+    phi = np.zeros((Cn, G)) if phi0 is None else np.array(phi0, dtype=np.float64)
+    old = zavg(phi)
+    hist, res, it = [], 1.0, 0
+    while it < max_iters:
+        phi, _ = sweep(phi, False)
+        cur = zavg(phi)
+        mask = cur > 1e-6 * cur.max()
+        res = float(np.max(np.abs((cur[mask] - old[mask]) / cur[mask]))) if mask.any() else 0.0
+        old = cur
+        hist.append(res)
+        it += 1
+        if res < tol:
+            break
+    phi, new = sweep(phi, True)
+    return phi, new, it, res, np.array(hist)
 
 
 def problem(nr, nz, npolar, nazimuthal, **kw):

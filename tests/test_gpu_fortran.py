@@ -1,7 +1,9 @@
 """The Fortran host (umt_amd/fortran, ISO_C_BINDING over the C-ABI) on the
 GPU against the oracle: source iteration through the device-resident
-context built from Fortran-held Teton-layout arrays, and one angle through
-the drop-in gpu_sweepucbxyz called with the reference's argument list."""
+context built from Fortran-held Teton-layout arrays, one angle through
+the drop-in gpu_sweepucbxyz called with the reference's argument list, and
+the RZ context driven from Fortran (source iteration and time-dependent
+cycles on a quad mesh the host hands over)."""
 import os
 import subprocess
 
@@ -10,17 +12,19 @@ import pytest
 
 import umt_amd as U
 from oracle import oracle as O
+from oracle import rz as R
 
 from conftest import ROOT, rel_err
 
 pytestmark = pytest.mark.gpu
 
 DRIVER = os.path.join(ROOT, "umt_amd", "fortran", "umt_si_driver")
+RZ_DRIVER = os.path.join(ROOT, "umt_amd", "fortran", "umt_rz_driver")
 
 
-def _run(args, out, tmp_path):
+def _run(args, out, tmp_path, driver=DRIVER):
     path = str(tmp_path / out)
-    cmd = [DRIVER] + [str(a) for a in args] + [path]
+    cmd = [driver] + [str(a) for a in args] + [path]
     print("run:", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
@@ -79,3 +83,55 @@ def test_fortran_gta_source_iteration_matches_python_host(tmp_path):
     assert rres < 1e-13
     assert rel_err(phi.reshape(p.ncorner, G), ref) < 1e-7
     ctx.close()
+
+
+def _rz_mesh(tmp_path, npolar, naz):
+    """A jittered quad mesh written for the Fortran host with counterclockwise
+    zones (the library turns them clockwise), and the oracle's view of the
+    problem the library builds from it."""
+    pts, quads = R.structured(6, 5, r0=0.25, r1=1.5, z0=-1.0, z1=1.0, jitter=0.3, seed=9)
+    ccw = np.ascontiguousarray(quads[:, ::-1], dtype=np.int32)
+    path = tmp_path / "rz_mesh.bin"
+    with open(path, "wb") as f:
+        np.array([pts.shape[0], ccw.shape[0]], np.int32).tofile(f)
+        np.ascontiguousarray(pts, np.float64).tofile(f)
+        ccw.tofile(f)
+    p = U.RzProblem.from_quad_mesh(pts, ccw, npolar, naz)
+    pts2, quads2 = p.points()
+    m = R.quad_mesh(pts2, quads2)
+    g = R.geometry(m)
+    q = R.quadrature(npolar, naz)
+    return str(path), p, (m, g, q, R.schedule(m, g, q))
+
+
+def test_fortran_rz_source_iteration_matches_oracle(tmp_path):
+    seed, G = 20261208, 2
+    mesh, p, (m, g, q, s) = _rz_mesh(tmp_path, 2, 3)
+    phi, out = _run(["si", mesh, 2, 3, G, 64, 8, 0.0, seed, 0.1, 20.0, 0.0, 0.9, 0.5, 1.0, 0.0, 0],
+                    "phi_rz_si.bin", tmp_path, RZ_DRIVER)
+    assert "rz si iterations=     8" in out
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (0.1, 20.0), (0.0, 0.9), (0.5, 1.0))
+    ref, it, _, _ = R.source_iteration(m, g, q, s, st, ss, qq, 0.0, 8)
+    assert rel_err(phi.reshape(p.ncorner, G), ref) < 1e-12
+
+
+def test_fortran_rz_time_cycles_match_oracle(tmp_path):
+    seed, G, tau = 20261209, 2, 2.0
+    mesh, p, (m, g, q, s) = _rz_mesh(tmp_path, 2, 2)
+    out_all, out = _run(["time", mesh, 2, 2, G, 64, 5, 0.0, seed, 0.1, 20.0, 0.0, 0.9, 0.5, 1.0,
+                         tau, 2], "phi_rz_time.bin", tmp_path, RZ_DRIVER)
+    assert "rz cycle   2" in out
+    A, Cn = p.nangles, p.ncorner
+    phi = out_all[:Cn * G].reshape(Cn, G)
+    psi = out_all[Cn * G:].reshape(A, Cn, G)
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (0.1, 20.0), (0.0, 0.9), (0.5, 1.0))
+    # the host's starting Psi(g, c, a) = (1 + 0.25 mod(a, 3) + 0.1 g) / (2 pi), 1-based a and g
+    a1 = np.arange(1, A + 1)[:, None, None]
+    g1 = np.arange(1, G + 1)[None, None, :]
+    psi_ref = np.broadcast_to((1 + 0.25 * (a1 % 3) + 0.1 * g1) / (2 * np.pi), (A, Cn, G))
+    phi_ref = None
+    for _ in range(2):
+        phi_ref, psi_ref, _, _, _ = R.time_step(m, g, q, s, st, ss, qq, tau, psi_ref, 0.0, 5,
+                                                phi0=phi_ref)
+    assert rel_err(phi, phi_ref) < 1e-12
+    assert np.abs(psi - psi_ref).max() <= 1e-12 * np.abs(psi_ref).max()

@@ -145,3 +145,38 @@ def test_rz_graph_replay_equals_eager(prob):
         ctx.close()
     for a, b in zip(*runs):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_rz_two_cycles_match_oracle(prob, precision):
+    """Time-dependent cycles (sigma_t + tau, the tau psi_prev source of each
+    direction, the savePsi sweep) against the oracle's time step, with a
+    psi_prev that differs between directions."""
+    p, (m, g, q, s) = prob
+    print(f"case RZ time step {p.nzones} zones, precision={precision}", flush=True)
+    G, tau = 2, 3.0
+    rng = np.random.default_rng(17)
+    sig = np.exp(rng.uniform(np.log(0.1), np.log(20.0), (p.nzones, G)))
+    sigs = sig * rng.uniform(0.0, 0.9, (p.nzones, G))
+    qq = rng.uniform(0.0, 1.0, (p.nzones, G))
+    psi0 = rng.uniform(0.0, 1.0, (p.nangles, p.ncorner, G))
+    ctx = U.RzContext(p, G, precision=precision)
+    with pytest.raises(RuntimeError):
+        ctx.set_psi_prev(psi0)  # needs tau > 0 first
+    ctx.set_sigt(sig)
+    ctx.set_scattering(sigs, qq)
+    ctx.set_tau(tau)
+    ctx.set_psi_prev(psi0)
+    tol = 1e-12 if precision == 64 else 2e-5
+    psi_ref, phi_ref = psi0, None
+    for cycle in range(2):
+        ctx.source_iteration(tol=0.0, max_iters=5)
+        ctx.save_psi()
+        phi_ref, psi_ref, _, _, _ = R.time_step(m, g, q, s, sig, sigs, qq, tau, psi_ref, 0.0, 5,
+                                                phi0=phi_ref)
+        e_phi = rel_err(ctx.phi(), phi_ref)
+        got = ctx.psi_prev()
+        e_psi = np.max(np.abs(got - psi_ref)) / np.max(np.abs(psi_ref))
+        print(f"cycle {cycle}: phi {e_phi:.2e}, psi {e_psi:.2e}", flush=True)
+        assert e_phi < tol and e_psi < tol, cycle
+    ctx.close()

@@ -18,7 +18,13 @@ here. Pins:
     snac/SweepGreyUCBrz.F90:289-297 (an independent transcription); every
     zone conserves particles over the directions (the angular terms
     telescope to zero);
-  - source iteration: global balance at convergence.
+  - source iteration: global balance at convergence;
+  - time-dependent sweep (tau psi_prev source): the constant solution
+    (sigma + tau) psi = S + tau psi0 is exact, every zone conserves particles
+    with an angle-dependent psi_prev, psi_prev of one direction reaches only
+    that direction and the later ones of its xi level; the time step reduces
+    to source iteration at tau = 0 and satisfies the global balance with the
+    tau terms at convergence.
 """
 import itertools
 import json
@@ -311,6 +317,133 @@ def test_source_iteration_global_balance(jittered):
     absorb = ((np.repeat(sig - sigs, 4, 0) * phi)[:, 0] * g["Volume"]).sum()
     source = (np.repeat(qq, 4, 0)[:, 0] * g["Volume"]).sum()
     assert abs(absorb + leak - source) / source < 1e-10
+
+
+def _zone_balance_lhs(m, g, q, psi, sig):
+    """sum_m w_m [net radius-weighted FP outflow + sigma V psi] per zone."""
+    Cn = m["cFP"].shape[0]
+    Z = Cn // 4
+    lhs = np.zeros(Z)
+    for a in range(len(q["weight"])):
+        w = q["weight"][a]
+        if w == 0:
+            continue
+        om = q["omega"][a]
+        for c in range(Cn):
+            z = c // 4
+            lhs[z] += w * sig[z, 0] * g["Volume"][c] * psi[a, c, 0]
+            for f in range(2):
+                d = g["A_fp"][c, f] @ om
+                up = m["cFP"][c, f] - 1
+                flux = psi[a, c, 0] if d > 0 else (psi[a, up, 0] if up < Cn else 0.0)
+                lhs[z] += w * g["RadiusFP"][c, f] * d * flux
+    return lhs
+
+
+def test_time_dependent_constant_solution_is_exact(jittered):
+    """(sigma + tau) psi = S + tau psi0 with psi0 isotropic and the same
+    incoming boundary flux: psi is constant through every direction."""
+    pts, quads, m, g, q, s = jittered
+    Z, nb, A = quads.shape[0], m["nbelem"], len(q["weight"])
+    rng = np.random.default_rng(11)
+    sig = rng.uniform(0.1, 30.0, (Z, 2))
+    tau, psi0, psic = 7.5, 0.4, 1.5
+    S = np.repeat(sig + tau, 4, 0) * psic - tau * psi0
+    prev = np.full((A, 4 * Z, 2), psi0)
+    _, _, psi = R.sweep_all(m, g, q, s, S, sig + tau, PsiB=np.full((A, nb, 2), psic),
+                            want_psi=True, tau=tau, psi_prev=prev)
+    assert np.abs(psi / psic - 1).max() < 1e-13
+    # without the tau psi0 source the same data is not that solution
+    _, _, psi_no = R.sweep_all(m, g, q, s, S, sig + tau, PsiB=np.full((A, nb, 2), psic),
+                               want_psi=True)
+    assert np.abs(psi_no / psic - 1).max() > 0.1
+
+
+def test_time_dependent_zones_conserve_particles(jittered):
+    """With an angle-dependent psi_prev the tau source enters each direction's
+    balance: sum_m w_m [outflow + (sigma + tau) V psi] = sum_m w_m V (S + tau
+    psi_prev[m]) for every zone."""
+    pts, quads, m, g, q, s = jittered
+    rng = np.random.default_rng(12)
+    Z, A = quads.shape[0], len(q["weight"])
+    Cn = 4 * Z
+    sig = rng.uniform(0.5, 40.0, (Z, 1))
+    tau = 3.0
+    S = rng.uniform(0.1, 1.0, (Cn, 1))
+    prev = rng.uniform(0.0, 2.0, (A, Cn, 1))
+    _, _, psi = R.sweep_all(m, g, q, s, S, sig + tau, want_psi=True, tau=tau, psi_prev=prev)
+    lhs = _zone_balance_lhs(m, g, q, psi, sig + tau)
+    src = S[None, :, 0] + tau * prev[:, :, 0]
+    rhs = ((q["weight"][:, None] * src).sum(0) * g["Volume"]).reshape(Z, 4).sum(1)
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-12)
+
+
+def test_tau_source_of_one_direction_reaches_only_its_successors(jittered):
+    """psi_prev nonzero in one direction changes that direction and the later
+    directions of its xi level (through the half-angle flux), nothing else."""
+    pts, quads, m, g, q, s = jittered
+    rng = np.random.default_rng(13)
+    Z, A = quads.shape[0], len(q["weight"])
+    sig = rng.uniform(0.5, 10.0, (Z, 1))
+    S = rng.uniform(0.1, 1.0, (4 * Z, 1))
+    starts = np.flatnonzero(q["starting"])
+    lev = np.searchsorted(starts, np.arange(A), side="right") - 1
+    target = starts[1] + 2  # inside the second xi level
+    assert not q["starting"][target] and not q["finishing"][target]
+    prev = np.zeros((A, 4 * Z, 1))
+    prev[target] = rng.uniform(0.5, 1.5, (4 * Z, 1))
+    _, _, base = R.sweep_all(m, g, q, s, S, sig + 2.0, want_psi=True, tau=2.0,
+                             psi_prev=np.zeros_like(prev))
+    _, _, psi = R.sweep_all(m, g, q, s, S, sig + 2.0, want_psi=True, tau=2.0, psi_prev=prev)
+    changed = np.abs(psi - base).max(axis=(1, 2)) > 0
+    expect = (lev == lev[target]) & (np.arange(A) >= target)
+    np.testing.assert_array_equal(changed, expect)
+
+
+def test_time_step_reduces_to_source_iteration_at_tau_zero(jittered):
+    pts, quads, m, g, q, s = jittered
+    rng = np.random.default_rng(14)
+    Z, A = quads.shape[0], len(q["weight"])
+    sig = rng.uniform(1.0, 20.0, (Z, 2))
+    sigs, qq = 0.5 * sig, rng.uniform(0.1, 1.0, (Z, 2))
+    phi, new, it, res, hist = R.time_step(m, g, q, s, sig, sigs, qq, 0.0,
+                                          rng.uniform(0, 1, (A, 4 * Z, 2)), 0.0, 5)
+    ref, it_ref, _, hist_ref = R.source_iteration(m, g, q, s, sig, sigs, qq, 0.0, 5)
+    assert it == it_ref == 5
+    np.testing.assert_allclose(hist, hist_ref, rtol=1e-10)
+    # phi after the final (savePsi) sweep is one more iteration; new holds its psi
+    S = (np.repeat(sigs, 4, 0) * ref + np.repeat(qq, 4, 0)) / (2 * np.pi)
+    ref6, _, psi6 = R.sweep_all(m, g, q, s, S, sig, want_psi=True)
+    np.testing.assert_allclose(phi, ref6, rtol=1e-13)
+    np.testing.assert_allclose(new, psi6, rtol=1e-13, atol=1e-300)
+
+
+def test_time_step_global_balance(jittered):
+    """At convergence: leakage + sum V (sigma - sigs + tau) phi = sum V q +
+    tau sum_m w_m sum V psi_prev[m]."""
+    pts, quads, m, g, q, s = jittered
+    rng = np.random.default_rng(15)
+    Z, A = quads.shape[0], len(q["weight"])
+    Cn = 4 * Z
+    sig = rng.uniform(1.0, 20.0, (Z, 1))
+    sigs, qq = 0.7 * sig, rng.uniform(0.1, 1.0, (Z, 1))
+    tau = 5.0
+    prev = rng.uniform(0.0, 1.0, (A, Cn, 1))
+    phi, new, it, res, _ = R.time_step(m, g, q, s, sig, sigs, qq, tau, prev, 1e-14, 300)
+    assert res < 1e-14
+    leak = 0.0
+    for a in range(A):
+        for c in range(Cn):
+            for f in range(2):
+                d = g["A_fp"][c, f] @ q["omega"][a]
+                if m["cFP"][c, f] > Cn and d > 0:
+                    leak += q["weight"][a] * g["RadiusFP"][c, f] * d * new[a, c, 0]
+    V = g["Volume"]
+    absorb = ((np.repeat(sig - sigs + tau, 4, 0) * phi)[:, 0] * V).sum()
+    source = (np.repeat(qq, 4, 0)[:, 0] * V).sum() + tau * (q["weight"] @ (prev[:, :, 0] @ V))
+    assert abs(absorb + leak - source) / source < 1e-10
+    # phi is the weighted sum of the saved psi
+    np.testing.assert_allclose(np.tensordot(q["weight"], new, 1), phi, rtol=1e-13)
 
 
 def test_product_host_arrays_equal_oracle(jittered):

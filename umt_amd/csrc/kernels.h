@@ -139,6 +139,8 @@ struct RzLevelArgs {
   Real* psi;            // (G, C + nbelem, A)
   long psiAngleStride;  // (C + nbelem) * G
   Real* psiM;           // (G, C, xi levels)
+  const Real* tauPsi;   // null, or the previous cycle's psi (G, C, A): source += tau * it
+  Real tau;
 };
 template <typename Real>
 hipError_t launch_rz_sweep_level(const RzLevelArgs<Real>& a, int K, hipStream_t s);

@@ -86,7 +86,13 @@ __global__ __launch_bounds__(256) void rz_sweep_level(RzLevelArgs<Real> p) {
     if (aez[f] < 0) inArea -= rez[f] * aez[f];
   }
   const GroupVec<Real, K> sg = load_groups<Real, K>(p.sigt + (long)z * G + g0);
-  const GroupVec<Real, K> Q = load_groups<Real, K>(p.S + cc * G + g0);
+  GroupVec<Real, K> Q = load_groups<Real, K>(p.S + cc * G + g0);
+  if (p.tauPsi) {  // time-dependent source STotal + tau Psi (SweepUCBrz.F90:136)
+    const GroupVec<Real, K> prev =
+        load_groups<Real, K>(p.tauPsi + ((long)a * p.C + cc) * G + g0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) Q.v[k] += p.tau * prev.v[k];
+  }
   // Incoming FP fluxes; boundaries are vacuum in these sweeps.
   GroupVec<Real, K> psf[2];
 #pragma unroll

@@ -44,6 +44,9 @@ struct umt_rz_ctx {
   void* psiM = nullptr;  // (G, C, xi levels)
   std::vector<hipEvent_t> evLevel;
   bool haveSigt = false, haveScattering = false;
+  std::vector<double> sigtHost;  // sigma_t as set; the sweep sees sigma_t + tau
+  double tau = 0.0;              // 1/(c dt): time-dependent sweeps when > 0
+  void* psiPrev = nullptr;       // previous cycle's psi (G, C, A), sweep precision
   // Captured sweeps: the level launches and the phi reduction (sweep), or
   // the level launches alone (levels, for timing), per groups-per-lane K.
   bool useGraphs = true;
@@ -93,6 +96,13 @@ int upload(umt_rz_ctx* x, void* dst, const double* src, size_t n) {
   return 0;
 }
 
+// The sweep's total cross section: sigma_t + tau (control/setTotalOpacity.F90).
+int upload_sigt(umt_rz_ctx* x) {
+  std::vector<double> st(x->sigtHost);
+  for (double& v : st) v += x->tau;
+  return upload(x, x->sigt, st.data(), st.size());
+}
+
 template <typename Real>
 int sweep_impl(umt_rz_ctx* x, bool timeLevels, bool levelsOnly = false) {
   umt::RzLevelArgs<Real> a{};
@@ -111,6 +121,8 @@ int sweep_impl(umt_rz_ctx* x, bool timeLevels, bool levelsOnly = false) {
   a.psi = (Real*)x->psi;
   a.psiAngleStride = x->stride();
   a.psiM = (Real*)x->psiM;
+  a.tauPsi = x->tau > 0.0 ? (const Real*)x->psiPrev : nullptr;
+  a.tau = (Real)x->tau;
   for (int h = 0; h < x->levels(); ++h) {
     a.items = x->items + x->levelOffset[h];
     a.nItems = x->levelOffset[h + 1] - x->levelOffset[h];
@@ -387,6 +399,7 @@ void umt_rz_ctx_destroy(umt_rz_ctx* x) {
   dfree(x->cFP); dfree(x->vol64); dfree(x->volZ); dfree(x->S); dfree(x->sigt); dfree(x->sigs);
   dfree(x->q); dfree(x->phi); dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
   dfree(x->angles); dfree(x->weight); dfree(x->items); dfree(x->psi); dfree(x->psiM);
+  dfree(x->psiPrev);
   for (auto e : x->evLevel) (void)hipEventDestroy(e);
   if (x->ev0) (void)hipEventDestroy(x->ev0);
   if (x->ev1) (void)hipEventDestroy(x->ev1);
@@ -419,7 +432,8 @@ int umt_rz_ctx_set_sigt(umt_rz_ctx* x, const double* sigt) {
   const long n = (long)x->G * x->Z;
   for (long i = 0; i < n; ++i)
     if (!(sigt[i] > 0.0) || !std::isfinite(sigt[i])) return fail(30, "Sigt entry %ld is not positive and finite", i + 1);
-  if (int rc = upload(x, x->sigt, sigt, n)) return rc;
+  x->sigtHost.assign(sigt, sigt + n);
+  if (int rc = upload_sigt(x)) return rc;
   x->haveSigt = true;
   return 0;
 }
@@ -512,6 +526,68 @@ int umt_rz_ctx_source_iteration(umt_rz_ctx* x, double tol, int maxIters, int* it
   if (iters) *iters = it;
   if (residual) *residual = res;
   return res < tol ? 0 : 1;
+}
+
+int umt_rz_ctx_set_tau(umt_rz_ctx* x, double tau) {
+  if (!x) return fail(1, "null context");
+  if (!(tau >= 0.0) || !std::isfinite(tau)) return fail(48, "tau must be finite and >= 0");
+  const long n = (long)x->G * x->C * x->A;
+  if (tau > 0.0 && !x->psiPrev) {
+    if (int rc = dalloc((char**)&x->psiPrev, (size_t)n * x->rsize())) return rc;
+    if (x->precision == 32) HIP_TRY(umt::launch_fill_zero<float>((float*)x->psiPrev, n, x->stream));
+    else HIP_TRY(umt::launch_fill_zero<double>((double*)x->psiPrev, n, x->stream));
+  }
+  x->tau = tau;
+  drop_graphs(x);  // the level launches carry tau and the Psi pointer
+  if (x->haveSigt)
+    if (int rc = upload_sigt(x)) return rc;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_rz_ctx_set_psi_prev(umt_rz_ctx* x, const double* Psi) {
+  if (!x || !Psi) return fail(1, "null argument");
+  if (!x->psiPrev) return fail(49, "umt_rz_ctx_set_tau with tau > 0 first");
+  const long n = (long)x->G * x->C * x->A;
+  for (long i = 0; i < n; ++i)
+    if (!std::isfinite(Psi[i])) return fail(31, "Psi entry %ld is not finite", i + 1);
+  return upload(x, x->psiPrev, Psi, n);
+}
+
+int umt_rz_ctx_get_psi_prev(const umt_rz_ctx* x, double* Psi) {
+  if (!x || !Psi) return fail(1, "null argument");
+  if (!x->psiPrev) return fail(49, "umt_rz_ctx_set_tau with tau > 0 first");
+  const size_t n = (size_t)x->G * x->C * x->A;
+  if (x->precision == 64) {
+    HIP_TRY(hipMemcpyAsync(Psi, x->psiPrev, n * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  } else {
+    std::vector<float> f(n);
+    HIP_TRY(hipMemcpyAsync(f.data(), x->psiPrev, n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+    for (size_t i = 0; i < n; ++i) Psi[i] = f[i];
+  }
+  return 0;
+}
+
+int umt_rz_ctx_save_psi(umt_rz_ctx* x) {
+  if (!x) return fail(1, "null context");
+  if (!x->psiPrev) return fail(49, "umt_rz_ctx_set_tau with tau > 0 first");
+  if (!x->haveScattering) return fail(33, "umt_rz_ctx_save_psi needs umt_rz_ctx_set_scattering first");
+  // The cycle's final sweep (rt/rtmainsn.F90:238-246): source from the
+  // current phi, one sweep reading the previous cycle's Psi, then Psi = the
+  // psi of every direction (SweepUCBrz.F90:268-276).
+  const long nGC = (long)x->G * x->C;
+  if (x->precision == 32)
+    HIP_TRY(umt::launch_rz_source_update<float>(x->phi, x->sigs, x->q, x->G, nGC, (float*)x->S, x->stream));
+  else
+    HIP_TRY(umt::launch_rz_source_update<double>(x->phi, x->sigs, x->q, x->G, nGC, (double*)x->S, x->stream));
+  if (int rc = sweep(x)) return rc;
+  const size_t rs = x->rsize();
+  HIP_TRY(hipMemcpy2DAsync(x->psiPrev, nGC * rs, x->psi, (size_t)x->stride() * rs, nGC * rs,
+                           (size_t)x->A, hipMemcpyDeviceToDevice, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
 }
 
 int umt_rz_ctx_set_groups_per_lane(umt_rz_ctx* x, int K) {

@@ -196,6 +196,37 @@ class RzContext:
         check("umt_rz_ctx_get_psi", _lib.load().umt_rz_ctx_get_psi(self._h, _dp(out)))
         return out
 
+    def set_tau(self, tau):
+        """Time-dependent sweeps with tau = 1/(c dt): the sweep sees sigma_t +
+        tau and the source S + tau psi_prev[direction]."""
+        check("umt_rz_ctx_set_tau", _lib.load().umt_rz_ctx_set_tau(self._h, float(tau)))
+
+    def set_psi_prev(self, psi):
+        """The previous cycle's psi, (A, C, G)."""
+        p = self.problem
+        a = np.ascontiguousarray(psi, dtype=np.float64)
+        if a.shape != (p.nangles, p.ncorner, self.G):
+            raise ValueError(f"psi_prev must be {(p.nangles, p.ncorner, self.G)}, got {a.shape}")
+        check("umt_rz_ctx_set_psi_prev", _lib.load().umt_rz_ctx_set_psi_prev(self._h, _dp(a)))
+
+    def psi_prev(self):
+        p = self.problem
+        out = np.zeros((p.nangles, p.ncorner, self.G))
+        check("umt_rz_ctx_get_psi_prev", _lib.load().umt_rz_ctx_get_psi_prev(self._h, _dp(out)))
+        return out
+
+    def save_psi(self):
+        """The cycle's final sweep: source from the current phi, one sweep,
+        psi_prev = its psi of every direction."""
+        check("umt_rz_ctx_save_psi", _lib.load().umt_rz_ctx_save_psi(self._h))
+
+    def time_step(self, tol=1e-8, max_iters=1000):
+        """Source iteration with the current tau and psi_prev, then the
+        savePsi sweep. Returns (converged, iterations, residual, history)."""
+        out = self.source_iteration(tol, max_iters)
+        self.save_psi()
+        return out
+
     def set_graphs(self, on):
         """Replay sweeps from a captured graph (default) or launch them eagerly."""
         check("umt_rz_ctx_set_graphs", _lib.load().umt_rz_ctx_set_graphs(self._h, int(bool(on))))
