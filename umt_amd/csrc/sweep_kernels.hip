@@ -151,7 +151,13 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
       for (int k = 0; k < K; ++k) psf[f].v[k] = 0;
     }
   }
-  Real src[K], rden[K];
+  // FP64 shares divisions: 1/sigma and 1/(outArea + sigma V) come from one
+  // reciprocal of their product, and the three EZ faces' denominators from
+  // another (a few ulps against correctly rounded quotients; the factors are
+  // far inside FP64's exponent range). FP32 divides each one: the product of
+  // three face denominators can leave FP32's range on small zones.
+  constexpr bool kShareDiv = sizeof(Real) == 8;
+  Real src[K], rden[K], halfSigInv[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const Real sig = sigv_.v[k];
@@ -159,23 +165,58 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
 #pragma unroll
     for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
     src[k] = s;
-    rden[k] = (Real)1 / (outArea + sig * v);
+    const Real D = outArea + sig * v;
+    if constexpr (kShareDiv) {
+      const Real r = (Real)1 / (sig * D);
+      rden[k] = r * sig;
+      halfSigInv[k] = (Real)0.5 * (r * D);
+    } else {
+      rden[k] = (Real)1 / D;
+      halfSigInv[k] = (Real)0.5 * ((Real)1 / sig);
+    }
   }
   // EZ faces: coupling terms computed by the upwind corner
   // (SweepUCBxyz.F90:182-252), pulled with the opposite sign by the downwind one.
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const Real sig = sigv_.v[k];
-    const Real halfSigInv = (Real)0.5 * ((Real)1 / sig);
     const Real q = Q.v[k];
     const Real Qn[3] = {from_ez<0>(q), from_ez<1>(q), from_ez<2>(q)};
     Real sezOut[3];
+    if constexpr (kShareDiv) {
+      const Real sigv = sig * v;
+      const Real sigv2 = sigv * sigv;
+      Real num[3], den[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const int fo = (f + 1) % 3;
-      const Real areaOpp = afp[fo] < 0 ? -afp[fo] : (Real)0;
-      const Real t = sez_term<Real>(aez[f], sig, v, halfSigInv, q, Qn[f], areaOpp, psf[fo].v[k]);
-      sezOut[f] = aez[f] > 0 ? t : (Real)0;
+      for (int f = 0; f < 3; ++f) {
+        const int fo = (f + 1) % 3;
+        const Real A = aez[f];
+        const Real gnum = A * A * ((Real)1.82 * sigv2 + A * ((Real)4 * sigv + (Real)3 * A));
+        const Real gden =
+            v * ((Real)4 * sigv * sigv2 + A * ((Real)6 * sigv2 + (Real)2 * A * ((Real)2 * sigv + A)));
+        num[f] = v * gnum * (sig * psf[fo].v[k] - q) + (Real)0.5 * A * gden * (q - Qn[f]);
+        // Faces that are not outgoing take no part (their gnum + gden sigma
+        // can vanish); outgoing ones have gnum, gden > 0.
+        den[f] = A > 0 ? gnum + gden * sig : (Real)1;
+      }
+      const Real d01 = den[0] * den[1];
+      const Real r = (Real)1 / (d01 * den[2]);
+      const Real inv[3] = {r * (den[1] * den[2]), r * (den[0] * den[2]), r * d01};
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        const int fo = (f + 1) % 3;
+        const Real t = afp[fo] < 0 ? num[f] * inv[f] : aez[f] * halfSigInv[k] * (q - Qn[f]);
+        sezOut[f] = aez[f] > 0 ? t : (Real)0;
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        const int fo = (f + 1) % 3;
+        const Real areaOpp = afp[fo] < 0 ? -afp[fo] : (Real)0;
+        const Real t =
+            sez_term<Real>(aez[f], sig, v, halfSigInv[k], q, Qn[f], areaOpp, psf[fo].v[k]);
+        sezOut[f] = aez[f] > 0 ? t : (Real)0;
+      }
     }
     Real s = src[k];
     s += sezOut[0];
