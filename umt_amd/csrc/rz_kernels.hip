@@ -18,6 +18,7 @@
 // values, as the xyz kernel does.
 #include <hip/hip_runtime.h>
 
+#include "device_math.h"
 #include "kernels.h"
 #include "lane_moves.h"
 
@@ -41,10 +42,10 @@ __device__ __forceinline__ Real rz_sez(Real R, Real A, Real sig, Real area, Real
     const Real gnum = A * A * ((Real)1.82 * sigA2 + A * ((Real)4 * sigA + (Real)3 * A));
     const Real gden =
         area * ((Real)4 * sigA * sigA2 + A * ((Real)6 * sigA2 + (Real)2 * A * ((Real)2 * sigA + A)));
-    return R * (area * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd)) /
-           (gnum + gden * sig);
+    return quot<Real>(R * (area * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd)),
+                      gnum + gden * sig);
   }
-  return (Real)0.5 * R * A * (Qu - Qd) / sig;
+  return quot<Real>((Real)0.5 * R * A * (Qu - Qd), sig);
 }
 
 }  // namespace
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void rz_sweep_level(RzLevelArgs<Real> p) {
     s -= rz_from_ez<1>(sez0);
     s += area * fac * pm.v[k];
     src[k] = s;
-    rden[k] = (Real)1 / (inArea + sig * v);
+    rden[k] = quot<Real>(1, inArea + sig * v);
   }
   // Corner solve: psi_c = (src_c + sum over upwind EZ neighbours of R |aez| psi_n) / den.
   const Real k0 = aez[0] < 0 ? -rez[0] * aez[0] : (Real)0;

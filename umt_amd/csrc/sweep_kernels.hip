@@ -22,6 +22,7 @@
 
 #include <algorithm>
 
+#include "device_math.h"
 #include "kernels.h"
 #include "lane_moves.h"
 
@@ -65,8 +66,8 @@ __device__ __forceinline__ Real sez_term(Real A, Real sig, Real v, Real halfSigI
   const Real gnum = A2 * ((Real)1.82 * sigv2 + A * ((Real)4 * sigv + (Real)3 * A));
   const Real gden =
       v * ((Real)4 * sigv * sigv2 + A * ((Real)6 * sigv2 + (Real)2 * A * ((Real)2 * sigv + A)));
-  const Real withOpp =
-      (v * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd)) / (gnum + gden * sig);
+  const Real withOpp = quot<Real>(v * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd),
+                                  gnum + gden * sig);
   const Real noOpp = A * halfSigInv * (Qu - Qd);
   return areaOpp > 0 ? withOpp : noOpp;
 }
@@ -151,12 +152,6 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
       for (int k = 0; k < K; ++k) psf[f].v[k] = 0;
     }
   }
-  // FP64 shares divisions: 1/sigma and 1/(outArea + sigma V) come from one
-  // reciprocal of their product, and the three EZ faces' denominators from
-  // another (a few ulps against correctly rounded quotients; the factors are
-  // far inside FP64's exponent range). FP32 divides each one: the product of
-  // three face denominators can leave FP32's range on small zones.
-  constexpr bool kShareDiv = sizeof(Real) == 8;
   Real src[K], rden[K], halfSigInv[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -165,15 +160,8 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
 #pragma unroll
     for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
     src[k] = s;
-    const Real D = outArea + sig * v;
-    if constexpr (kShareDiv) {
-      const Real r = (Real)1 / (sig * D);
-      rden[k] = r * sig;
-      halfSigInv[k] = (Real)0.5 * (r * D);
-    } else {
-      rden[k] = (Real)1 / D;
-      halfSigInv[k] = (Real)0.5 * ((Real)1 / sig);
-    }
+    rden[k] = quot<Real>(1, outArea + sig * v);
+    halfSigInv[k] = quot<Real>((Real)0.5, sig);
   }
   // EZ faces: coupling terms computed by the upwind corner
   // (SweepUCBxyz.F90:182-252), pulled with the opposite sign by the downwind one.
@@ -183,40 +171,13 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
     const Real q = Q.v[k];
     const Real Qn[3] = {from_ez<0>(q), from_ez<1>(q), from_ez<2>(q)};
     Real sezOut[3];
-    if constexpr (kShareDiv) {
-      const Real sigv = sig * v;
-      const Real sigv2 = sigv * sigv;
-      Real num[3], den[3];
 #pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        const int fo = (f + 1) % 3;
-        const Real A = aez[f];
-        const Real gnum = A * A * ((Real)1.82 * sigv2 + A * ((Real)4 * sigv + (Real)3 * A));
-        const Real gden =
-            v * ((Real)4 * sigv * sigv2 + A * ((Real)6 * sigv2 + (Real)2 * A * ((Real)2 * sigv + A)));
-        num[f] = v * gnum * (sig * psf[fo].v[k] - q) + (Real)0.5 * A * gden * (q - Qn[f]);
-        // Faces that are not outgoing take no part (their gnum + gden sigma
-        // can vanish); outgoing ones have gnum, gden > 0.
-        den[f] = A > 0 ? gnum + gden * sig : (Real)1;
-      }
-      const Real d01 = den[0] * den[1];
-      const Real r = (Real)1 / (d01 * den[2]);
-      const Real inv[3] = {r * (den[1] * den[2]), r * (den[0] * den[2]), r * d01};
-#pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        const int fo = (f + 1) % 3;
-        const Real t = afp[fo] < 0 ? num[f] * inv[f] : aez[f] * halfSigInv[k] * (q - Qn[f]);
-        sezOut[f] = aez[f] > 0 ? t : (Real)0;
-      }
-    } else {
-#pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        const int fo = (f + 1) % 3;
-        const Real areaOpp = afp[fo] < 0 ? -afp[fo] : (Real)0;
-        const Real t =
-            sez_term<Real>(aez[f], sig, v, halfSigInv[k], q, Qn[f], areaOpp, psf[fo].v[k]);
-        sezOut[f] = aez[f] > 0 ? t : (Real)0;
-      }
+    for (int f = 0; f < 3; ++f) {
+      const int fo = (f + 1) % 3;
+      const Real areaOpp = afp[fo] < 0 ? -afp[fo] : (Real)0;
+      const Real t =
+          sez_term<Real>(aez[f], sig, v, halfSigInv[k], q, Qn[f], areaOpp, psf[fo].v[k]);
+      sezOut[f] = aez[f] > 0 ? t : (Real)0;
     }
     Real s = src[k];
     s += sezOut[0];
