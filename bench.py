@@ -167,6 +167,7 @@ def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds, log):
         ctx.set_graphs(GRAPHS[0])
         ctx.set_sigt(sig)
         ctx.set_source(S)
+        K, ms_k = ctx.autotune(2)
         for _ in range(warmup):
             ctx.sweep()
         t0 = time.perf_counter()
@@ -178,7 +179,7 @@ def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds, log):
         B = rz_bytes_per_unknown(prec, n_in, G, start_frac)
         achieved = B * unknowns / (ms_lv * 1e-3) / 1e9
         key = "fp64" if prec == 64 else "fp32"
-        kname = f"void umt::rz_sweep_level<{'double' if prec == 64 else 'float'}, 4>(umt::RzLevelArgs<{'double' if prec == 64 else 'float'}>)"
+        kname = f"void umt::rz_sweep_level<{'double' if prec == 64 else 'float'}, {K}>(umt::RzLevelArgs<{'double' if prec == 64 else 'float'}>)"
         prof = load_traffic("rz_sweep_level_512x512_traffic.json")
         traffic = None
         if prof and (nr, nz, G) == (512, 512, 16) and kname in prof.get("kernels", {}):
@@ -189,9 +190,11 @@ def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds, log):
                                  "traffic": traffic,
                                  "kernel": f"rz_sweep_level<{'double' if prec == 64 else 'float'},K>",
                                  "level_kernels_ms_per_sweep": ms_lv,
-                                 "algorithmic_bytes_per_unknown": B, "n_in": n_in}}
+                                 "algorithmic_bytes_per_unknown": B, "n_in": n_in,
+                                 "groups_per_lane": K,
+                                 "autotune_ms_per_sweep": {str(k): v for k, v in ms_k.items()}}}
         log(f"RZ FP{prec} sweep: {unknowns * steps / dt:.3e} unknowns/s, level kernels "
-            f"{ms_lv:.2f} ms ({achieved:.0f} GB/s)")
+            f"{ms_lv:.2f} ms ({achieved:.0f} GB/s), groups per lane {K} (autotune {ms_k})")
     if cpu_seconds > 0:
         from oracle import rz as R
 

@@ -113,6 +113,11 @@ int umt_rz_ctx_get_psi_prev(const umt_rz_ctx *ctx, double *Psi);
 int umt_rz_ctx_save_psi(umt_rz_ctx *ctx);
 /* Groups carried per lane (1, 2 or 4, dividing G); default the largest that divides G. */
 int umt_rz_ctx_set_groups_per_lane(umt_rz_ctx *ctx, int K);
+/* Autotuner: times reps sweeps of the current source with each supported
+ * groups-per-lane value and keeps the fastest. msPerK[0..2] receives the
+ * mean ms per sweep for K = 1, 2, 4 (-1 when unsupported); may be NULL.
+ * Overwrites psi and phi. No counterpart in the reference. */
+int umt_rz_ctx_autotune(umt_rz_ctx *ctx, int reps, int *bestK, double *msPerK);
 /* Replay each sweep from a captured hipGraph (default on; kernel nodes
  * only, rebuilt per groups-per-lane choice) or launch it eagerly. */
 int umt_rz_ctx_set_graphs(umt_rz_ctx *ctx, int on);

@@ -180,3 +180,22 @@ def test_rz_two_cycles_match_oracle(prob, precision):
         print(f"cycle {cycle}: phi {e_phi:.2e}, psi {e_psi:.2e}", flush=True)
         assert e_phi < tol and e_psi < tol, cycle
     ctx.close()
+
+
+def test_rz_autotune_keeps_a_supported_layout(prob):
+    """The autotuner times every groups-per-lane value that divides G, keeps
+    the fastest, and the sweep afterwards still gives the oracle's phi."""
+    p, (m, g, q, s) = prob
+    G = 4
+    sig, S = _xs(p, G, 41)
+    ctx = U.RzContext(p, G)
+    ctx.set_sigt(sig)
+    ctx.set_source(S)
+    K, ms = ctx.autotune(reps=2)
+    print(f"RZ autotune: K={K}, ms per sweep {ms}", flush=True)
+    assert set(ms) == {1, 2, 4} and all(t > 0 for t in ms.values())
+    assert ms[K] == min(ms.values())
+    ctx.sweep()
+    ref, _, _ = R.sweep_all(m, g, q, s, S, sig)
+    assert rel_err(ctx.phi(), ref) < 1e-12
+    ctx.close()

@@ -598,6 +598,36 @@ int umt_rz_ctx_set_groups_per_lane(umt_rz_ctx* x, int K) {
   return 0;
 }
 
+int umt_rz_ctx_autotune(umt_rz_ctx* x, int reps, int* bestK, double* msPerK) {
+  if (!x || reps < 1) return fail(1, "bad argument");
+  if (!x->haveSigt) return fail(30, "umt_rz_ctx_set_sigt first");
+  static const int cand[3] = {1, 2, 4};
+  double best = 1e300;
+  int bk = x->K;
+  for (int i = 0; i < 3; ++i) {
+    if (msPerK) msPerK[i] = -1.0;
+    if (!umt::groups_per_lane_supported(x->G, cand[i])) continue;
+    x->K = cand[i];
+    if (int rc = sweep(x)) return rc;  // warm-up (and graph capture)
+    HIP_TRY(hipEventRecord(x->ev0, x->stream));
+    for (int r = 0; r < reps; ++r)
+      if (int rc = sweep(x)) return rc;
+    HIP_TRY(hipEventRecord(x->ev1, x->stream));
+    HIP_TRY(hipEventSynchronize(x->ev1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, x->ev0, x->ev1));
+    const double ms = t / reps;
+    if (msPerK) msPerK[i] = ms;
+    if (ms < best) {
+      best = ms;
+      bk = cand[i];
+    }
+  }
+  x->K = bk;
+  if (bestK) *bestK = bk;
+  return 0;
+}
+
 int umt_rz_ctx_set_graphs(umt_rz_ctx* x, int on) {
   if (!x) return fail(1, "null context");
   x->useGraphs = on != 0;

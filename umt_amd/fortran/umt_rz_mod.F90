@@ -28,7 +28,7 @@ module umt_rz_mod
             umt_rz_ctx_get_phi, umt_rz_ctx_set_phi, umt_rz_ctx_get_psi,                       &
             umt_rz_ctx_source_iteration, umt_rz_ctx_set_tau, umt_rz_ctx_set_psi_prev,         &
             umt_rz_ctx_get_psi_prev, umt_rz_ctx_save_psi, umt_rz_ctx_set_groups_per_lane,     &
-            umt_rz_ctx_set_graphs, umt_rz_ctx_time_sweeps
+            umt_rz_ctx_set_graphs, umt_rz_ctx_time_sweeps, umt_rz_ctx_autotune
 
   interface
 
@@ -211,6 +211,16 @@ module umt_rz_mod
       import :: c_int, c_ptr
       type(c_ptr), value    :: ctx
       integer(c_int), value :: K
+    end function
+
+!   msPerK(3): mean ms per sweep for groups per lane 1, 2, 4 (-1 when unsupported)
+    integer(c_int) function umt_rz_ctx_autotune(ctx, reps, bestK, msPerK) &
+        bind(c, name="umt_rz_ctx_autotune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value    :: ctx
+      integer(c_int), value :: reps
+      integer(c_int)        :: bestK
+      real(c_double)        :: msPerK(*)
     end function
 
     integer(c_int) function umt_rz_ctx_set_graphs(ctx, on) bind(c, name="umt_rz_ctx_set_graphs")

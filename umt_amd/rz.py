@@ -234,6 +234,14 @@ class RzContext:
     def set_groups_per_lane(self, K):
         check("umt_rz_ctx_set_groups_per_lane", _lib.load().umt_rz_ctx_set_groups_per_lane(self._h, K))
 
+    def autotune(self, reps=3):
+        """Times groups per lane K = 1, 2, 4 on the current source and keeps
+        the fastest. Returns (best K, {K: ms per sweep})."""
+        k = C.c_int(0)
+        ms = np.zeros(3)
+        check("umt_rz_ctx_autotune", _lib.load().umt_rz_ctx_autotune(self._h, reps, C.byref(k), _dp(ms)))
+        return k.value, {K: float(t) for K, t in zip((1, 2, 4), ms) if t >= 0}
+
     def source_iteration(self, tol=1e-8, max_iters=1000):
         """Returns (converged, iterations, residual, residual_history)."""
         it = C.c_int(0)
