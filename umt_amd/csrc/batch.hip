@@ -200,6 +200,7 @@ int layout(umt_batch* b, int n, int K) {
         sg.taskStart = tasks;
         sg.domain = di;
         sg.itemStart = (int)(before - itemBase);
+        sg.perItem = umt::make_fast_div((unsigned)(d.Gp / K));
         segs.push_back(sg);
         tasks += cnt * (d.Gp / K);
       }

@@ -4,6 +4,29 @@
 
 namespace umt {
 
+// Quotient of n < 2^31 by an invariant divisor d >= 1 as one multiply and a
+// shift: q = (n m) >> s with s = 31 + ceil(log2 d) and m = floor(2^s / d) + 1,
+// exact because 2^s < m d <= 2^s + 2^(s - 31) (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994, Thm 4.2).
+// Kernels use it to split a flat thread index into (item, group block).
+struct FastDiv {
+  unsigned long long m;
+  int shift;
+};
+inline FastDiv make_fast_div(unsigned d) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  FastDiv f;
+  f.shift = 31 + l;
+  f.m = (1ull << f.shift) / d + 1;
+  return f;
+}
+__device__ __forceinline__ unsigned fast_div(unsigned n, FastDiv f) {
+  return (unsigned)(((unsigned long long)n * f.m) >> f.shift);
+}
+// Level launches index threads with 32 bits.
+constexpr long kMaxLevelThreads = 0x7fffffffL;
+
 template <typename Real>
 struct SweepLevelArgs {
   const int2* items;        // (zone, angle) pairs of this level, 0-based; a zone
@@ -29,6 +52,7 @@ struct SweepLevelArgs {
   const Real* tauPsi;       // null, or source += tau * Psi: (G, C) of this angle (single
                             // angle) or the previous cycle's (G, C + nbelem, A) (all angles)
   Real tau;
+  FastDiv perItem;          // division by G / groupsPerLane (set by launch_sweep_level)
 };
 
 template <typename Real>
@@ -53,6 +77,7 @@ struct BatchSeg {
   long taskStart;           // first task (item, K groups) of the segment
   int domain;
   int itemStart;            // first item of the segment within the level
+  FastDiv perItem;          // division by the domain's G / K
 };
 template <typename Real>
 struct BatchLevelArgs {
@@ -141,6 +166,7 @@ struct RzLevelArgs {
   Real* psiM;           // (G, C, xi levels)
   const Real* tauPsi;   // null, or the previous cycle's psi (G, C, A): source += tau * it
   Real tau;
+  FastDiv perItem;      // division by G / K (set by launch_rz_sweep_level)
 };
 template <typename Real>
 hipError_t launch_rz_sweep_level(const RzLevelArgs<Real>& a, int K, hipStream_t s);

@@ -52,12 +52,12 @@ __device__ __forceinline__ Real rz_sez(Real R, Real A, Real sig, Real area, Real
 
 template <typename Real, int K>
 __global__ __launch_bounds__(256) void rz_sweep_level(RzLevelArgs<Real> p) {
-  const long task = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  const unsigned task = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
   const int c = threadIdx.x & 3;
   const int G = p.G;
-  const int perItem = G / K;
-  int item = (int)(task / perItem);
-  const int g0 = K * (int)(task - (long)item * perItem);
+  const unsigned perItem = (unsigned)(G / K);
+  int item = (int)fast_div(task, p.perItem);
+  const int g0 = K * (int)(task - (unsigned)item * perItem);
   const bool valid = item < p.nItems;
   if (!valid) item = 0;
   const int2 it = p.items[item];
@@ -243,11 +243,14 @@ template <typename Real>
 hipError_t launch_rz_sweep_level(const RzLevelArgs<Real>& a, int K, hipStream_t s) {
   const long threads = 4L * a.nItems * (a.G / K);
   if (threads == 0) return hipSuccess;
+  if (threads > kMaxLevelThreads) return hipErrorInvalidValue;
+  RzLevelArgs<Real> b = a;
+  b.perItem = make_fast_div((unsigned)(a.G / K));
   const dim3 grid(rz_blocks(threads, 256)), block(256);
   switch (K) {
-    case 4: hipLaunchKernelGGL((rz_sweep_level<Real, 4>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((rz_sweep_level<Real, 2>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((rz_sweep_level<Real, 1>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((rz_sweep_level<Real, 4>), grid, block, 0, s, b); break;
+    case 2: hipLaunchKernelGGL((rz_sweep_level<Real, 2>), grid, block, 0, s, b); break;
+    default: hipLaunchKernelGGL((rz_sweep_level<Real, 1>), grid, block, 0, s, b); break;
   }
   return hipGetLastError();
 }

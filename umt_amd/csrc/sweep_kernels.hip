@@ -244,10 +244,10 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
 
 template <typename Real, int K, bool kSingleAngle>
 __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
-  const long pair = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
-  const int blocksPerItem = p.G / K;
-  int item = (int)(pair / blocksPerItem);
-  const int g0 = K * (int)(pair - (long)item * blocksPerItem);
+  const unsigned pair = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const unsigned blocksPerItem = (unsigned)(p.G / K);
+  int item = (int)fast_div(pair, p.perItem);
+  const int g0 = K * (int)(pair - (unsigned)item * blocksPerItem);
   const bool valid = item < p.nItems;
   if (!valid) item = 0;
   sweep_corner<Real, K, kSingleAngle>(p, item, g0, valid);
@@ -286,10 +286,11 @@ __global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real
   p.psiAngleStride = d.psiAngleStride;
   p.angle0 = b.angle0;
   p.parTol = 0;
-  const int blocksPerItem = d.G / K;
-  const long local = t - sg.taskStart;
-  const int item = sg.itemStart + (int)(local / blocksPerItem);
-  const int g0 = K * (int)(local - (local / blocksPerItem) * blocksPerItem);
+  const unsigned blocksPerItem = (unsigned)(d.G / K);
+  const unsigned local = (unsigned)(t - sg.taskStart);
+  const unsigned q = fast_div(local, sg.perItem);
+  const int item = sg.itemStart + (int)q;
+  const int g0 = K * (int)(local - q * blocksPerItem);
   sweep_corner<Real, K, false>(p, item, g0, valid);
 }
 
@@ -427,12 +428,15 @@ template <typename Real, int K>
 static hipError_t launch_k(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
   const long threads = 8L * a.nItems * (a.G / K);  // one lane per corner of each (item, K groups)
   if (threads == 0) return hipSuccess;
+  if (threads > kMaxLevelThreads) return hipErrorInvalidValue;
+  SweepLevelArgs<Real> b = a;
+  b.perItem = make_fast_div((unsigned)(a.G / K));
   if (singleAngle)
     hipLaunchKernelGGL((ucb_sweep_level<Real, K, true>), dim3(blocks_for(threads, 256)), dim3(256),
-                       0, s, a);
+                       0, s, b);
   else
     hipLaunchKernelGGL((ucb_sweep_level<Real, K, false>), dim3(blocks_for(threads, 256)),
-                       dim3(256), 0, s, a);
+                       dim3(256), 0, s, b);
   return hipGetLastError();
 }
 
@@ -442,6 +446,7 @@ template <typename Real>
 hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStream_t s) {
   const long threads = 8L * b.nTasks;
   if (threads == 0) return hipSuccess;
+  if (threads > kMaxLevelThreads) return hipErrorInvalidValue;
   const dim3 grid(blocks_for(threads, 256)), block(256);
   switch (K) {
     case 4: hipLaunchKernelGGL((ucb_sweep_batch_level<Real, 4>), grid, block, 0, s, b); break;
