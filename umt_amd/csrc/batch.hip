@@ -201,8 +201,10 @@ int layout(umt_batch* b, int n, int K) {
         sg.domain = di;
         sg.itemStart = (int)(before - itemBase);
         sg.perItem = umt::make_fast_div((unsigned)(d.Gp / K));
+        sg.nTasks = (int)(cnt * (d.Gp / K));
         segs.push_back(sg);
-        tasks += cnt * (d.Gp / K);
+        // pad to whole waves so no wave spans two segments
+        tasks += (sg.nTasks + umt::kBatchWaveTasks - 1) / umt::kBatchWaveTasks * umt::kBatchWaveTasks;
       }
       if (tasks == 0) continue;  // no domain has this level in this block
       levelItem.push_back((long)items.size());

@@ -73,11 +73,15 @@ struct BatchDomain {
   int G;                    // groups as stored (a multiple of the groups per lane)
   int C;
 };
+// Tasks of one wave (64 lanes, 8 per task) stay in one segment: segments
+// start at multiples of kBatchWaveTasks and are padded to them.
+constexpr int kBatchWaveTasks = 8;
 struct BatchSeg {
-  long taskStart;           // first task (item, K groups) of the segment
+  long taskStart;           // first task (item, K groups) of the segment, a multiple of 8
   int domain;
   int itemStart;            // first item of the segment within the level
   FastDiv perItem;          // division by the domain's G / K
+  int nTasks;               // tasks of the segment before padding
 };
 template <typename Real>
 struct BatchLevelArgs {
@@ -85,7 +89,7 @@ struct BatchLevelArgs {
   const int2* items;        // (zone, angle) of this level, grouped by domain
   const BatchSeg* segs;     // segments of this level
   int nSegs;
-  long nTasks;
+  long nTasks;              // padded tasks of the level
   int angle0;               // first angle of the block being swept
 };
 template <typename Real>

@@ -255,22 +255,25 @@ __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
 
 // One level of many domains in one launch. The level's items are grouped by
 // domain into segments; segment s starts at task segs[s].taskStart, where a
-// task is one (item, K groups) pair of 8 lanes. Each 8-lane group finds its
-// segment by binary search, takes its domain's arrays, and runs the same
-// corner code as the one-domain kernel.
+// task is one (item, K groups) pair of 8 lanes. Segments are padded to whole
+// waves, so a wave finds its segment once, by a binary search on its first
+// task with wave-uniform (scalar) loads, takes its domain's arrays the same
+// way, and runs the same corner code as the one-domain kernel; padding tasks
+// compute the segment's first item and store nothing.
 template <typename Real, int K>
 __global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real> b) {
-  const long task = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
-  const bool valid = task < b.nTasks;
-  const long t = valid ? task : 0;
+  const unsigned task = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const unsigned waveTask = __builtin_amdgcn_readfirstlane(task);
   int lo = 0, hi = b.nSegs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (b.segs[mid].taskStart <= t) lo = mid;
+    if (b.segs[mid].taskStart <= (long)waveTask) lo = mid;
     else hi = mid - 1;
   }
   const BatchSeg sg = b.segs[lo];
   const BatchDomain<Real>& d = b.doms[sg.domain];
+  const unsigned inSeg = (unsigned)((long)task - sg.taskStart);
+  const bool valid = (long)task < b.nTasks && inSeg < (unsigned)sg.nTasks;
   SweepLevelArgs<Real> p{};
   p.items = b.items;
   p.G = d.G;
@@ -287,7 +290,7 @@ __global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real
   p.angle0 = b.angle0;
   p.parTol = 0;
   const unsigned blocksPerItem = (unsigned)(d.G / K);
-  const unsigned local = (unsigned)(t - sg.taskStart);
+  const unsigned local = valid ? inSeg : 0u;
   const unsigned q = fast_div(local, sg.perItem);
   const int item = sg.itemStart + (int)q;
   const int g0 = K * (int)(local - q * blocksPerItem);
