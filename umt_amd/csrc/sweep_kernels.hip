@@ -153,34 +153,28 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
       for (int k = 0; k < K; ++k) psf[f].v[k] = 0;
     }
   }
-  Real src[K], rden[K], halfSigInv[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const Real sig = sigv_.v[k];
-    Real s = v * Q.v[k];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
-    src[k] = s;
-    rden[k] = quot<Real>(1, outArea + sig * v);
-    halfSigInv[k] = quot<Real>((Real)0.5, sig);
-  }
-  // EZ faces: coupling terms computed by the upwind corner
-  // (SweepUCBxyz.F90:182-252), pulled with the opposite sign by the downwind one.
+  // Per group: volume and incoming FP terms (SweepUCBxyz.F90:119-161), then
+  // the EZ coupling terms computed by the upwind corner (:182-252), pulled
+  // with the opposite sign by the downwind one.
+  Real src[K], rden[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const Real sig = sigv_.v[k];
     const Real q = Q.v[k];
+    Real s = v * q;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
+    rden[k] = quot<Real>(1, outArea + sig * v);
+    const Real halfSigInv = quot<Real>((Real)0.5, sig);
     const Real Qn[3] = {from_ez<0>(q), from_ez<1>(q), from_ez<2>(q)};
     Real sezOut[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
       const int fo = (f + 1) % 3;
       const Real areaOpp = afp[fo] < 0 ? -afp[fo] : (Real)0;
-      const Real t =
-          sez_term<Real>(aez[f], sig, v, halfSigInv[k], q, Qn[f], areaOpp, psf[fo].v[k]);
+      const Real t = sez_term<Real>(aez[f], sig, v, halfSigInv, q, Qn[f], areaOpp, psf[fo].v[k]);
       sezOut[f] = aez[f] > 0 ? t : (Real)0;
     }
-    Real s = src[k];
     s += sezOut[0];
     s += sezOut[1];
     s += sezOut[2];
