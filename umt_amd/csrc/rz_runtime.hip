@@ -531,6 +531,7 @@ int umt_rz_ctx_source_iteration(umt_rz_ctx* x, double tol, int maxIters, int* it
 int umt_rz_ctx_set_tau(umt_rz_ctx* x, double tau) {
   if (!x) return fail(1, "null context");
   if (!(tau >= 0.0) || !std::isfinite(tau)) return fail(48, "tau must be finite and >= 0");
+  HIP_TRY(hipStreamSynchronize(x->stream));  // no replay in flight when the graphs go
   const long n = (long)x->G * x->C * x->A;
   if (tau > 0.0 && !x->psiPrev) {
     if (int rc = dalloc((char**)&x->psiPrev, (size_t)n * x->rsize())) return rc;
@@ -541,7 +542,6 @@ int umt_rz_ctx_set_tau(umt_rz_ctx* x, double tau) {
   drop_graphs(x);  // the level launches carry tau and the Psi pointer
   if (x->haveSigt)
     if (int rc = upload_sigt(x)) return rc;
-  HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
 }
 
