@@ -1,9 +1,10 @@
 """The Fortran host (umt_amd/fortran, ISO_C_BINDING over the C-ABI) on the
 GPU against the oracle: source iteration through the device-resident
 context built from Fortran-held Teton-layout arrays, one angle through
-the drop-in gpu_sweepucbxyz called with the reference's argument list, and
-the RZ context driven from Fortran (source iteration and time-dependent
-cycles on a quad mesh the host hands over)."""
+the drop-in gpu_sweepucbxyz called with the reference's argument list, a
+hex mesh handed over by the host (mesh ingestion), time-dependent cycles,
+a batch of uneven domains, and the RZ context driven from Fortran (source
+iteration and time-dependent cycles on a quad mesh the host hands over)."""
 import os
 import subprocess
 
@@ -14,7 +15,7 @@ import umt_amd as U
 from oracle import oracle as O
 from oracle import rz as R
 
-from conftest import ROOT, rel_err
+from conftest import ROOT, hex_grid, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -22,9 +23,9 @@ DRIVER = os.path.join(ROOT, "umt_amd", "fortran", "umt_si_driver")
 RZ_DRIVER = os.path.join(ROOT, "umt_amd", "fortran", "umt_rz_driver")
 
 
-def _run(args, out, tmp_path, driver=DRIVER):
+def _run(args, out, tmp_path, driver=DRIVER, extra=None):
     path = str(tmp_path / out)
-    cmd = [driver] + [str(a) for a in args] + [path]
+    cmd = [driver] + [str(a) for a in args] + [path] + ([str(extra)] if extra is not None else [])
     print("run:", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
@@ -135,3 +136,71 @@ def test_fortran_rz_time_cycles_match_oracle(tmp_path):
                                                 phi0=phi_ref)
     assert rel_err(phi, phi_ref) < 1e-12
     assert np.abs(psi - psi_ref).max() <= 1e-12 * np.abs(psi_ref).max()
+
+
+def test_fortran_mesh_ingestion_source_iteration_matches_oracle(tmp_path):
+    """A distorted hex mesh handed over from Fortran as points and
+    connectivity (umt_problem_from_hex_mesh) iterates like the oracle."""
+    seed, G = 20261210, 2
+    pts, hexes = hex_grid(3, 0.12, 5)
+    p = U.TiledProblem.from_hex_mesh(pts, hexes, 2, 2)
+    s = p.schedule()
+    assert int(np.sum(s.get("numCycles", [0]))) == 0  # the array-built context takes no cycle lists
+    mesh = tmp_path / "hex_mesh.bin"
+    with open(mesh, "wb") as f:
+        np.array([pts.shape[0], hexes.shape[0]], np.int32).tofile(f)
+        np.ascontiguousarray(pts, np.float64).tofile(f)
+        np.ascontiguousarray(hexes, np.int32).tofile(f)
+    phi, out = _run(["mesh", 0, 0, 0, 2, 2, G, 64, 6, 0.0, seed, 0.1, 10.0, 0.0, 0.9, 0.5, 1.0, 1],
+                    "phi_mesh.bin", tmp_path, extra=mesh)
+    assert "si iterations=     6" in out
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (0.1, 10.0), (0.0, 0.9), (0.5, 1.0))
+    ref, it, _, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), s, st, ss, qq,
+                                       0.0, 6)
+    assert rel_err(phi.reshape(p.ncorner, G), ref) < 1e-12
+
+
+def test_fortran_time_cycles_match_oracle(tmp_path):
+    """Two time-dependent cycles (tau Psi source, savePsi) from a Psi the
+    Fortran host holds, against the oracle's time step."""
+    seed, G, tau, cycles = 20261211, 2, 3.0, 2
+    phi_all, out = _run(["time", 2, 2, 1, 2, 2, G, 64, 5, 0.0, seed, 0.1, 10.0, 0.0, 0.9, 0.5, 1.0,
+                         cycles], "phi_time.bin", tmp_path, extra=tau)
+    assert "time cycle   2" in out
+    p = U.TiledProblem(2, 2, 1, 2, 2)
+    A, Cn = p.nangles, p.ncorner
+    phi = phi_all[:Cn * G].reshape(Cn, G)
+    psi = phi_all[Cn * G:].reshape(A, Cn, G)
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (0.1, 10.0), (0.0, 0.9), (0.5, 1.0))
+    # the host's starting Psi(g, c, a) = (1 + 0.25 mod(a, 3) + 0.1 g) / (4 pi), 1-based a and g
+    a1 = np.arange(1, A + 1)[:, None, None]
+    g1 = np.arange(1, G + 1)[None, None, :]
+    psi_ref = np.broadcast_to((1 + 0.25 * (a1 % 3) + 0.1 * g1) / (4 * np.pi), (A, Cn, G)).copy()
+    phi_ref = None
+    m, g, q, s = p.mesh(), p.geometry(), p.quadrature(), p.schedule()
+    for _ in range(cycles):
+        phi_ref, psi_ref, _, _, _ = O.time_step(m, g, q, s, st, ss, qq, tau, psi_ref, 0.0, 5,
+                                                phi0=phi_ref)
+    assert rel_err(phi, phi_ref) < 1e-12
+    assert np.abs(psi - psi_ref).max() <= 1e-12 * np.abs(psi_ref).max()
+
+
+def test_fortran_batch_matches_oracle(tmp_path):
+    """Three uneven domains (different sizes and group counts) iterated in
+    one batch from Fortran: each domain's phi is the oracle's."""
+    seed, G, nd, iters = 20261212, 2, 3, 5
+    phi_all, out = _run(["batch", 2, 2, 1, 2, 2, G, 64, iters, 0.0, seed, 0.1, 10.0, 0.0, 0.9, 0.5,
+                         1.0, nd], "phi_batch.bin", tmp_path)
+    assert f"iterations=     {iters}" in out
+    off = 0
+    for d in range(1, nd + 1):
+        p = U.TiledProblem(2 + d - 1, 2, 1, 2, 2)
+        Gd = G + d - 1
+        st, ss, qq = U.draw_cross_sections(seed + d, p.nzones, Gd, (0.1, 10.0), (0.0, 0.9),
+                                           (0.5, 1.0))
+        ref, _, _, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), p.schedule(),
+                                          st, ss, qq, 0.0, iters)
+        phi = phi_all[off:off + p.ncorner * Gd].reshape(p.ncorner, Gd)
+        off += p.ncorner * Gd
+        assert rel_err(phi, ref) < 1e-12, d
+    assert off == phi_all.size

@@ -9,13 +9,25 @@
 !    mode "gta":    the same with grey-transport acceleration, the context
 !                   built from the problem handle (S2 schedule included);
 !    mode "dropin": sweeps one angle through gpu_sweepucbxyz with the
-!                   argument list of src/teton/gpu/SweepUCBxyzToGPU.F90:198-240.
+!                   argument list of src/teton/gpu/SweepUCBxyzToGPU.F90:198-240;
+!    mode "mesh":   as "si", on a hex mesh read from the file named by the
+!                   20th argument (int32 npoints, int32 nzones, real64
+!                   coords(3, npoints), int32 connectivity(8, nzones), 0-based
+!                   Blueprint / VTK hexahedron order; mesh ingestion);
+!    mode "time":   `angle` time-dependent cycles with tau (20th argument):
+!                   source iteration with the tau Psi source of
+!                   snac/SweepUCBxyz.F90:121, then the savePsi sweep, from a
+!                   Psi(G, C, A) the host holds; appends the final Psi;
+!    mode "batch":  `angle` uneven domains (tiles nx + d - 1, ny, nz and
+!                   groups + d - 1 for domain d, cross sections drawn with
+!                   seed + d) iterated together in one batch; writes every
+!                   domain's Phi in turn.
 !  Writes the resulting Phi(G,C) as raw FP64 to the output file and prints a
 !  one-line summary.
 !
 !  usage: umt_si_driver mode nx ny nz npolar nazimuthal groups precision
 !                       maxIters tol seed sigt_lo sigt_hi c_lo c_hi q_lo q_hi
-!                       angle outfile
+!                       angle outfile [tau | meshfile]
 !***********************************************************************
 program umt_si_driver
 
@@ -24,13 +36,13 @@ program umt_si_driver
 
   implicit none
 
-  character(len=256) :: arg, mode, outfile
+  character(len=256) :: arg, mode, outfile, extra
   integer(c_int)     :: nx, ny, nz, npolar, nazimu, G, precision, maxIters, angle
   real(c_double)     :: tol, sigt_lo, sigt_hi, c_lo, c_hi, q_lo, q_hi
   integer(c_int64_t) :: seed
   integer(c_int64_t) :: dims(8)
   type(c_ptr)        :: prob, ctx
-  integer            :: Z, C, nb, A, totalPlanes, rc, i, u, start
+  integer            :: Z, C, nb, A, totalPlanes, rc, i, k, u, start
   integer(c_int)     :: iters, bestK, status
 
   integer(c_int), allocatable :: numCorner(:), cOffSet(:), nCFaces(:), cFP(:,:), cEZ(:,:)
@@ -39,7 +51,7 @@ program umt_si_driver
   real(c_double), allocatable :: px(:,:), A_fp(:,:,:), A_ez(:,:,:), Volume(:), VolumeZone(:)
   real(c_double), allocatable :: omega(:,:), weight(:)
   real(c_double), allocatable :: Sigt(:,:), sigs(:,:), q(:,:), Phi(:,:), STotal(:,:)
-  real(c_double), allocatable :: Psi1(:,:), PsiB(:,:), Psi(:,:), hist(:)
+  real(c_double), allocatable :: Psi1(:,:), PsiB(:,:), Psi(:,:), hist(:), Psi3(:,:,:)
   real(c_double)              :: residual, msPerK(3), tau, quadwt
   real(c_double)              :: dummy(1)
   integer(c_int64_t)          :: greyIts, greySweeps
@@ -57,9 +69,25 @@ program umt_si_driver
   call real_arg(16, q_lo); call real_arg(17, q_hi)
   call int_arg(18, angle)
   call get_command_argument(19, outfile)
+  call get_command_argument(20, extra)
 
-  rc = umt_problem_tiled(nx, ny, nz, npolar, nazimu, 1, prob)
-  call check(rc, "umt_problem_tiled")
+  if (trim(mode) == "mesh") then
+    block
+      integer(c_int) :: npts, nzs
+      real(c_double), allocatable :: coords(:,:)
+      integer(c_int), allocatable :: conn(:,:)
+      open(newunit=u, file=trim(extra), access="stream", form="unformatted", status="old")
+      read(u) npts, nzs
+      allocate(coords(3, npts), conn(8, nzs))
+      read(u) coords, conn
+      close(u)
+      rc = umt_problem_from_hex_mesh(npts, coords, nzs, conn, npolar, nazimu, 1, prob)
+      call check(rc, "umt_problem_from_hex_mesh")
+    end block
+  else
+    rc = umt_problem_tiled(nx, ny, nz, npolar, nazimu, 1, prob)
+    call check(rc, "umt_problem_tiled")
+  endif
   rc = umt_problem_dims(prob, dims)
   Z = int(dims(1)); C = int(dims(2)); nb = int(dims(3)); A = int(dims(4))
   totalPlanes = int(dims(7))
@@ -80,7 +108,7 @@ program umt_si_driver
   rc = umt_draw_cross_sections(seed, Z, G, sigt_lo, sigt_hi, c_lo, c_hi, q_lo, q_hi, Sigt, sigs, q)
   call check(rc, "umt_draw_cross_sections")
 
-  if (trim(mode) == "si") then
+  if (trim(mode) == "si" .or. trim(mode) == "mesh") then
 
     rc = umt_ctx_create(Z, C, nb, G, A, precision, numCorner, cOffSet, nCFaces, cFP, cEZ, &
                         A_fp, A_ez, Volume, VolumeZone, omega, weight, nHyperPlanes,       &
@@ -141,6 +169,82 @@ program umt_si_driver
     if (status /= 0) call check(status, "gpu_sweepucbxyz")
     write(*, '(a,i4,a,es22.15)') "dropin angle=", angle, " sumPhi=", sum(Phi)
 
+  elseif (trim(mode) == "time") then
+
+    ! The previous cycle's angular flux held by the host, different per angle.
+    read(extra, *) tau
+    allocate(Psi3(G, C, A), hist(max(maxIters, 1)))
+    do i = 1, A
+      do k = 1, G
+        Psi3(k, :, i) = (1.0d0 + 0.25d0 * mod(i, 3) + 0.1d0 * k) / (16.0d0 * atan(1.0d0))
+      enddo
+    enddo
+    rc = umt_ctx_create_from_problem(prob, G, precision, ctx)
+    call check(rc, "umt_ctx_create_from_problem")
+    call check(umt_ctx_set_sigt(ctx, Sigt), "umt_ctx_set_sigt")
+    call check(umt_ctx_set_scattering(ctx, sigs, q), "umt_ctx_set_scattering")
+    call check(umt_ctx_set_tau(ctx, tau), "umt_ctx_set_tau")
+    call check(umt_ctx_set_psi_prev(ctx, Psi3), "umt_ctx_set_psi_prev")
+    do i = 1, angle
+      rc = umt_ctx_source_iteration(ctx, tol, maxIters, iters, residual, hist)
+      if (rc /= 0 .and. rc /= 1) call check(rc, "umt_ctx_source_iteration")
+      call check(umt_ctx_save_psi(ctx), "umt_ctx_save_psi")
+      write(*, '(a,i3,a,i6,a,es12.5)') "time cycle ", i, " iterations=", iters, &
+           " residual=", residual
+    enddo
+    call check(umt_ctx_get_phi(ctx, Phi), "umt_ctx_get_phi")
+    call check(umt_ctx_get_psi_prev(ctx, Psi3), "umt_ctx_get_psi_prev")
+    call umt_ctx_destroy(ctx)
+
+  elseif (trim(mode) == "batch") then
+
+    block
+      type(c_ptr), allocatable :: probs(:)
+      integer(c_int), allocatable :: gs(:), its(:)
+      real(c_double), allocatable :: res(:), sT(:,:), sS(:,:), sQ(:,:), ph(:,:)
+      type(c_ptr) :: batch
+      integer(c_int) :: d, total, bestK, bestApb
+      integer(c_int64_t) :: dd(8)
+      real(c_double) :: ms
+      allocate(probs(angle), gs(angle), its(angle), res(angle))
+      do d = 1, angle
+        call check(umt_problem_tiled(nx + d - 1, ny, nz, npolar, nazimu, 1, probs(d)), &
+                   "umt_problem_tiled")
+        gs(d) = G + d - 1
+      enddo
+      call check(umt_batch_create(angle, probs, gs, precision, batch), "umt_batch_create")
+      do d = 1, angle
+        call check(umt_problem_dims(probs(d), dd), "umt_problem_dims")
+        allocate(sT(gs(d), dd(1)), sS(gs(d), dd(1)), sQ(gs(d), dd(1)))
+        call check(umt_draw_cross_sections(seed + d, int(dd(1), c_int), gs(d), sigt_lo, sigt_hi, &
+                                           c_lo, c_hi, q_lo, q_hi, sT, sS, sQ),            &
+                   "umt_draw_cross_sections")
+        call check(umt_batch_set_cross_sections(batch, d - 1, sT, sS, sQ), &
+                   "umt_batch_set_cross_sections")
+        deallocate(sT, sS, sQ)
+      enddo
+      call check(umt_batch_autotune(batch, 1, bestK, bestApb, ms), "umt_batch_autotune")
+      rc = umt_batch_source_iteration(batch, tol, maxIters, its, res, total)
+      if (rc /= 0 .and. rc /= 1) call check(rc, "umt_batch_source_iteration")
+      write(*, '(a,i3,a,i6,a,i2,a,i4)') "batch domains=", angle, " iterations=", total, &
+           " groupsPerLane=", bestK, " anglesPerBlock=", bestApb
+      open(newunit=u, file=trim(outfile), access="stream", form="unformatted", status="replace")
+      do d = 1, angle
+        call check(umt_problem_dims(probs(d), dd), "umt_problem_dims")
+        allocate(ph(gs(d), dd(2)))
+        call check(umt_batch_get_phi(batch, d - 1, ph), "umt_batch_get_phi")
+        write(u) ph
+        deallocate(ph)
+      enddo
+      close(u)
+      call umt_batch_destroy(batch)
+      do d = 1, angle
+        call umt_problem_destroy(probs(d))
+      enddo
+    end block
+    call umt_problem_destroy(prob)
+    stop
+
   else
     write(*, *) "unknown mode ", trim(mode)
     stop 2
@@ -148,6 +252,7 @@ program umt_si_driver
 
   open(newunit=u, file=trim(outfile), access="stream", form="unformatted", status="replace")
   write(u) Phi
+  if (allocated(Psi3)) write(u) Psi3
   close(u)
   call umt_problem_destroy(prob)
 

@@ -31,7 +31,17 @@ module umt_sweep_mod
             umt_ctx_set_stall_window, umt_ctx_set_zone_neighbors,             &
             umt_learner_create, umt_learner_destroy, umt_learner_train,       &
             umt_ctx_set_learner, umt_ctx_learn_collect, umt_ctx_learn_commit, &
-            umt_ctx_learn_stats, umt_problem_cycles, umt_ctx_set_cycles
+            umt_ctx_learn_stats, umt_problem_cycles, umt_ctx_set_cycles,      &
+            umt_problem_from_hex_mesh, umt_problem_gta, umt_ctx_enable_gta,   &
+            umt_ctx_dims, umt_ctx_get_psi, umt_ctx_set_groups_per_lane,       &
+            umt_ctx_get_groups_per_lane, umt_ctx_get_angles_per_block,        &
+            umt_ctx_sweep_n, umt_ctx_time_sweeps, umt_ctx_set_tau,            &
+            umt_ctx_set_psi_prev, umt_ctx_get_psi_prev, umt_ctx_save_psi,     &
+            umt_batch_create, umt_batch_destroy, umt_batch_dims,              &
+            umt_batch_set_cross_sections, umt_batch_reset, umt_batch_sweep,   &
+            umt_batch_get_phi, umt_batch_source_iteration,                    &
+            umt_batch_set_graphs, umt_batch_set_layout,                       &
+            umt_batch_time_iterations, umt_batch_autotune
 
   interface
 
@@ -339,6 +349,197 @@ module umt_sweep_mod
       integer(c_int64_t) :: corrections
       integer(c_int) :: fellBackAt
     end function umt_ctx_learn_stats
+
+!   Any hex mesh: coords(3, npoints), connectivity(8, nzones) 0-based in the
+!   Blueprint / VTK hexahedron node order (mesh ingestion, mesh.cpp).
+    integer(c_int) function umt_problem_from_hex_mesh(npoints, coords, nzones, connectivity, &
+                                                      npolar, nazimuthal, kdtree, prob)      &
+        bind(c, name="umt_problem_from_hex_mesh")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: npoints, nzones, npolar, nazimuthal, kdtree
+      real(c_double) :: coords(*)
+      integer(c_int) :: connectivity(*)
+      type(c_ptr) :: prob
+    end function umt_problem_from_hex_mesh
+
+!   The S2 grey-acceleration directions and their schedule.
+    integer(c_int) function umt_problem_gta(prob, nangles, omega, weight, nHyperPlanes, &
+                                            zonesInPlane, nextZ) bind(c, name="umt_problem_gta")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: prob
+      integer(c_int) :: nangles
+      real(c_double) :: omega(*), weight(*)
+      integer(c_int) :: nHyperPlanes(*), zonesInPlane(*), nextZ(*)
+    end function umt_problem_gta
+
+    integer(c_int) function umt_ctx_enable_gta(ctx, nangles, omega, weight, nHyperPlanes, &
+                                               zonesInPlane, nextZ) bind(c, name="umt_ctx_enable_gta")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nangles
+      real(c_double) :: omega(*), weight(*)
+      integer(c_int) :: nHyperPlanes(*), zonesInPlane(*), nextZ(*)
+    end function umt_ctx_enable_gta
+
+    integer(c_int) function umt_ctx_dims(ctx, dims) bind(c, name="umt_ctx_dims")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t) :: dims(*)
+    end function umt_ctx_dims
+
+!   psi(G, C + nbelem, A)
+    integer(c_int) function umt_ctx_get_psi(ctx, psi) bind(c, name="umt_ctx_get_psi")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: psi(*)
+    end function umt_ctx_get_psi
+
+    integer(c_int) function umt_ctx_set_groups_per_lane(ctx, K) &
+        bind(c, name="umt_ctx_set_groups_per_lane")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: K
+    end function umt_ctx_set_groups_per_lane
+
+    integer(c_int) function umt_ctx_get_groups_per_lane(ctx, K) &
+        bind(c, name="umt_ctx_get_groups_per_lane")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: K
+    end function umt_ctx_get_groups_per_lane
+
+    integer(c_int) function umt_ctx_get_angles_per_block(ctx, n, blocks) &
+        bind(c, name="umt_ctx_get_angles_per_block")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: n, blocks
+    end function umt_ctx_get_angles_per_block
+
+    integer(c_int) function umt_ctx_sweep_n(ctx, n) bind(c, name="umt_ctx_sweep_n")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+    end function umt_ctx_sweep_n
+
+    integer(c_int) function umt_ctx_time_sweeps(ctx, warmup, reps, msSweep, msLevelKernels, &
+                                                launches) bind(c, name="umt_ctx_time_sweeps")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: warmup, reps
+      real(c_double) :: msSweep, msLevelKernels
+      integer(c_int) :: launches
+    end function umt_ctx_time_sweeps
+
+!   Time-dependent cycles: tau = 1/(c dt); Psi(G, C, A) is Teton's Psi.
+    integer(c_int) function umt_ctx_set_tau(ctx, tau) bind(c, name="umt_ctx_set_tau")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tau
+    end function umt_ctx_set_tau
+
+    integer(c_int) function umt_ctx_set_psi_prev(ctx, Psi) bind(c, name="umt_ctx_set_psi_prev")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: Psi(*)
+    end function umt_ctx_set_psi_prev
+
+    integer(c_int) function umt_ctx_get_psi_prev(ctx, Psi) bind(c, name="umt_ctx_get_psi_prev")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: Psi(*)
+    end function umt_ctx_get_psi_prev
+
+    integer(c_int) function umt_ctx_save_psi(ctx) bind(c, name="umt_ctx_save_psi")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function umt_ctx_save_psi
+
+!   Batches of uneven domains: problems(nDomains) handles, groups(nDomains).
+    integer(c_int) function umt_batch_create(nDomains, problems, groups, precision, batch) &
+        bind(c, name="umt_batch_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nDomains, precision
+      type(c_ptr) :: problems(*)
+      integer(c_int) :: groups(*)
+      type(c_ptr) :: batch
+    end function umt_batch_create
+
+    subroutine umt_batch_destroy(batch) bind(c, name="umt_batch_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: batch
+    end subroutine umt_batch_destroy
+
+    integer(c_int) function umt_batch_dims(batch, dims) bind(c, name="umt_batch_dims")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: batch
+      integer(c_int64_t) :: dims(*)
+    end function umt_batch_dims
+
+    integer(c_int) function umt_batch_set_cross_sections(batch, domain, sigt, sigs, q) &
+        bind(c, name="umt_batch_set_cross_sections")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: domain
+      real(c_double) :: sigt(*), sigs(*), q(*)
+    end function umt_batch_set_cross_sections
+
+    integer(c_int) function umt_batch_reset(batch) bind(c, name="umt_batch_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function umt_batch_reset
+
+    integer(c_int) function umt_batch_sweep(batch) bind(c, name="umt_batch_sweep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function umt_batch_sweep
+
+    integer(c_int) function umt_batch_get_phi(batch, domain, phi) bind(c, name="umt_batch_get_phi")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: domain
+      real(c_double) :: phi(*)
+    end function umt_batch_get_phi
+
+    integer(c_int) function umt_batch_source_iteration(batch, tol, maxIters, iters, residuals, &
+                                                       total) bind(c, name="umt_batch_source_iteration")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double), value :: tol
+      integer(c_int), value :: maxIters
+      integer(c_int) :: iters(*)
+      real(c_double) :: residuals(*)
+      integer(c_int) :: total
+    end function umt_batch_source_iteration
+
+    integer(c_int) function umt_batch_set_graphs(batch, on) bind(c, name="umt_batch_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: on
+    end function umt_batch_set_graphs
+
+    integer(c_int) function umt_batch_set_layout(batch, groupsPerLane, anglesPerBlock) &
+        bind(c, name="umt_batch_set_layout")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: groupsPerLane, anglesPerBlock
+    end function umt_batch_set_layout
+
+    integer(c_int) function umt_batch_time_iterations(batch, warmup, reps, msPerIteration) &
+        bind(c, name="umt_batch_time_iterations")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: warmup, reps
+      real(c_double) :: msPerIteration
+    end function umt_batch_time_iterations
+
+    integer(c_int) function umt_batch_autotune(batch, reps, bestK, bestAnglesPerBlock, &
+                                               msPerIteration) bind(c, name="umt_batch_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: reps
+      integer(c_int) :: bestK, bestAnglesPerBlock
+      real(c_double) :: msPerIteration
+    end function umt_batch_autotune
 
     integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
       import :: c_ptr, c_size_t
