@@ -78,8 +78,8 @@ typedef struct umt_rz_ctx umt_rz_ctx;
  * the residual stay FP64). */
 int umt_rz_ctx_create(const umt_rz_problem *p, int groups, int precision, umt_rz_ctx **out);
 void umt_rz_ctx_destroy(umt_rz_ctx *ctx);
-/* dims: nzones, ncorner, nbelem, groups, nangles, level launches per sweep,
- * (zone, direction) items per sweep. */
+/* dims[0..7]: nzones, ncorner, nbelem, groups, nangles, level launches per
+ * sweep, (zone, direction) items per sweep, 1 when sweeping in tile levels. */
 int umt_rz_ctx_dims(const umt_rz_ctx *ctx, int64_t *dims);
 int umt_rz_ctx_set_sigt(umt_rz_ctx *ctx, const double *sigt);      /* (G, Z) */
 int umt_rz_ctx_set_source(umt_rz_ctx *ctx, const double *STotal);  /* (G, C) per steradian */
@@ -118,6 +118,13 @@ int umt_rz_ctx_set_groups_per_lane(umt_rz_ctx *ctx, int K);
  * mean ms per sweep for K = 1, 2, 4 (-1 when unsupported); may be NULL.
  * Overwrites psi and phi. No counterpart in the reference. */
 int umt_rz_ctx_autotune(umt_rz_ctx *ctx, int reps, int *bestK, double *msPerK);
+/* on != 0: sweep in tile levels, one wave walking the zone levels of a tile
+ * of about 4 x 4 zones for one direction between barriers, so a launch
+ * covers several zone levels; 0 (default): one launch per zone level, which
+ * is faster on MI355X today. Both give the same bits. Fails when the problem
+ * has no tile schedule (some direction's tile graph has a cycle).
+ * No counterpart in the reference. */
+int umt_rz_ctx_set_tiles(umt_rz_ctx *ctx, int on);
 /* Replay each sweep from a captured hipGraph (default on; kernel nodes
  * only, rebuilt per groups-per-lane choice) or launch it eagerly. */
 int umt_rz_ctx_set_graphs(umt_rz_ctx *ctx, int on);

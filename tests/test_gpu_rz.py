@@ -199,3 +199,38 @@ def test_rz_autotune_keeps_a_supported_layout(prob):
     ref, _, _ = R.sweep_all(m, g, q, s, S, sig)
     assert rel_err(ctx.phi(), ref) < 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("nr,nz,npol,naz", [(16, 12, 2, 3), (9, 23, 3, 2), (64, 48, 4, 4)])
+def test_rz_tile_levels_give_the_zone_level_bits(nr, nz, npol, naz):
+    """On regular meshes the sweep runs in tile levels (a workgroup walking a
+    tile's zone levels between barriers): far fewer launches, the same bits
+    as one launch per zone level for every groups-per-lane choice, and the
+    oracle's phi."""
+    p = U.RzProblem(nr, nz, npol, naz, r0=0.0, r1=1.0, z0=0.0, z1=1.5)
+    G = 8
+    sig, S = _xs(p, G, 51)
+    print(f"case RZ tiles {nr}x{nz} q{npol}{naz}", flush=True)
+    out = {}
+    for tiles in (True, False):
+        ctx = U.RzContext(p, G)
+        ctx.set_tiles(tiles)
+        d = ctx.dims()
+        assert d["tiled"] == int(tiles)
+        ctx.set_sigt(sig)
+        ctx.set_source(S)
+        for K in (1, 2, 4):
+            ctx.set_groups_per_lane(K)
+            ctx.sweep()
+            out[(tiles, K)] = (ctx.phi(), ctx.psi(), d["levels"])
+        ctx.close()
+    for K in (1, 2, 4):
+        np.testing.assert_array_equal(out[(True, K)][0], out[(False, K)][0])
+        np.testing.assert_array_equal(out[(True, K)][1], out[(False, K)][1])
+    assert out[(True, 1)][2] < out[(False, 1)][2] / 2
+    pts, quads = p.points()
+    m = R.quad_mesh(pts, quads)
+    g = R.geometry(m)
+    q = R.quadrature(npol, naz)
+    ref, _, _ = R.sweep_all(m, g, q, R.schedule(m, g, q), S, sig)
+    assert rel_err(out[(True, 4)][0], ref) < 1e-12

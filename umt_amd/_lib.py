@@ -136,6 +136,7 @@ SIGNATURES = {
     "umt_rz_ctx_set_graphs": (_i, [_vp, _i]),
     "umt_rz_ctx_set_tau": (_i, [_vp, _d]),
     "umt_rz_ctx_autotune": (_i, [_vp, _i, _pi, _pd]),
+    "umt_rz_ctx_set_tiles": (_i, [_vp, _i]),
     "umt_rz_ctx_set_psi_prev": (_i, [_vp, _pd]),
     "umt_rz_ctx_get_psi_prev": (_i, [_vp, _pd]),
     "umt_rz_ctx_save_psi": (_i, [_vp]),

@@ -174,6 +174,22 @@ struct RzLevelArgs {
 };
 template <typename Real>
 hipError_t launch_rz_sweep_level(const RzLevelArgs<Real>& a, int K, hipStream_t s);
+// Tile levels: one 64-lane wave per (tile, direction) item and chunk of
+// gbPerChunk (1, 2 or 4) group blocks; items/order/stepOff/nsteps are this
+// level's.
+template <typename Real>
+struct RzTileArgs {
+  RzLevelArgs<Real> p;            // the per-corner arrays (items unused)
+  const int2* items;              // (tile, direction) of this level
+  const unsigned char* order;     // (items, maxZones) zone slots sorted by step
+  const unsigned char* stepOff;   // (items, maxZones + 1) first entry of each step
+  const int* nsteps;              // (items)
+  const int* tileZone;            // (tiles, maxZones) zone or -1
+  int maxZones;
+  int gbPerChunk;
+};
+template <typename Real>
+hipError_t launch_rz_sweep_tile(const RzTileArgs<Real>& a, int nItems, int K, hipStream_t s);
 template <typename Real>
 hipError_t launch_rz_source_update(const double* phi, const double* sigs, const double* q, int G,
                                    long n, Real* S, hipStream_t s);

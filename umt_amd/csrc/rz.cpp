@@ -513,4 +513,130 @@ void build_rz_levels(const RzMesh& m, const RzGeometry& g, const RzQuadrature& q
   }
 }
 
+void build_rz_tiles(const RzMesh& m, const RzGeometry& g, const RzQuadrature& q,
+                    const Schedule& s, int T, RzTiles& t) {
+  t = RzTiles();
+  t.T = T;
+  const int Z = m.nzones, A = q.nangles;
+  if (Z == 0 || T < 1) return;
+  // Tiles: zone centroids binned on a grid of about T x T zones per cell.
+  double rmin = 1e300, rmax = -1e300, zmin = 1e300, zmax = -1e300;
+  for (size_t i = 0; i + 1 < m.points.size(); i += 2) {
+    rmin = std::min(rmin, m.points[i]);
+    rmax = std::max(rmax, m.points[i]);
+    zmin = std::min(zmin, m.points[i + 1]);
+    zmax = std::max(zmax, m.points[i + 1]);
+  }
+  const double bin = T * std::sqrt((rmax - rmin) * (zmax - zmin) / Z);
+  if (!(bin > 0.0) || !std::isfinite(bin)) return;
+  const int nbr = std::max(1, (int)std::ceil((rmax - rmin) / bin));
+  const int nbz = std::max(1, (int)std::ceil((zmax - zmin) / bin));
+  std::vector<int> tileOfBin((size_t)nbr * nbz, -1), tileOf(Z);
+  std::vector<std::vector<int>> members;
+  for (int z = 0; z < Z; ++z) {
+    double cr = 0.0, cz = 0.0;
+    for (int c = 4 * z; c < 4 * z + 4; ++c) {
+      cr += 0.25 * m.px[2 * (size_t)c];
+      cz += 0.25 * m.px[2 * (size_t)c + 1];
+    }
+    const int br = std::min(nbr - 1, std::max(0, (int)std::floor((cr - rmin) / bin)));
+    const int bz = std::min(nbz - 1, std::max(0, (int)std::floor((cz - zmin) / bin)));
+    int& tid = tileOfBin[(size_t)bz * nbr + br];
+    if (tid < 0) {
+      tid = (int)members.size();
+      members.emplace_back();
+    }
+    members[tid].push_back(z);
+    tileOf[z] = tid;
+  }
+  const int NT = (int)members.size();
+  int maxZ = 0;
+  for (auto& mz : members) maxZ = std::max(maxZ, (int)mz.size());
+  if (maxZ > kRzMaxTileZones) return;
+  // Per swept direction: zone steps inside tiles, the tile graph (Kahn's
+  // order; a leftover tile means a cycle) and the tile levels.
+  std::vector<unsigned char> zstep((size_t)A * Z, 255);
+  std::vector<int> tileLev((size_t)A * NT, -1);
+  std::vector<std::vector<int>> succ(NT);
+  std::vector<int> indeg(NT), order;
+  int maxLev = -1;
+  for (int a = 0; a < A; ++a) {
+    if (q.finishing[a]) continue;
+    const double* om = &q.omega[2 * (size_t)a];
+    const int* nextZ = &s.nextZ[(size_t)a * Z];
+    unsigned char* st = &zstep[(size_t)a * Z];
+    for (auto& v : succ) v.clear();
+    std::fill(indeg.begin(), indeg.end(), 0);
+    for (int i = 0; i < Z; ++i) {
+      const int z = std::abs(nextZ[i]) - 1;
+      int k = 0;
+      for (int c = 4 * z; c < 4 * z + 4; ++c)
+        for (int f = 0; f < kRzCF; ++f) {
+          const int up = m.cFP[2 * (size_t)c + f] - 1;
+          if (up >= m.ncorner || dot2(om, &g.A_fp[4 * (size_t)c + 2 * f]) >= 0.0) continue;
+          const int uz = up / 4;
+          if (tileOf[uz] == tileOf[z]) k = std::max(k, st[uz] + 1);
+          else succ[tileOf[uz]].push_back(tileOf[z]);
+        }
+      if (k > 254) return;
+      st[z] = (unsigned char)k;
+    }
+    for (auto& v : succ) {
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      for (int w : v) ++indeg[w];
+    }
+    order.clear();
+    for (int tt = 0; tt < NT; ++tt)
+      if (indeg[tt] == 0) order.push_back(tt);
+    for (size_t k = 0; k < order.size(); ++k)
+      for (int w : succ[order[k]])
+        if (--indeg[w] == 0) order.push_back(w);
+    if ((int)order.size() != NT) return;
+    const bool chain = !q.starting[a];
+    int* L = &tileLev[(size_t)a * NT];
+    for (int tt : order) {
+      if (chain) L[tt] = std::max(L[tt], tileLev[(size_t)(a - 1) * NT + tt] + 1);
+      L[tt] = std::max(L[tt], 0);
+      for (int w : succ[tt]) L[w] = std::max(L[w], L[tt] + 1);
+      maxLev = std::max(maxLev, L[tt]);
+    }
+  }
+  t.ntiles = NT;
+  t.maxZones = maxZ;
+  t.tileZone.assign((size_t)NT * maxZ, -1);
+  for (int tt = 0; tt < NT; ++tt)
+    for (size_t i = 0; i < members[tt].size(); ++i)
+      t.tileZone[(size_t)tt * maxZ + i] = members[tt][i];
+  std::vector<std::vector<std::array<int, 2>>> byLevel(maxLev + 1);
+  for (int a = 0; a < A; ++a) {
+    if (q.finishing[a]) continue;
+    for (int tt = 0; tt < NT; ++tt) byLevel[tileLev[(size_t)a * NT + tt]].push_back({tt, a});
+  }
+  t.levelOffset.assign(1, 0);
+  std::vector<std::pair<int, int>> bySlot;
+  for (auto& l : byLevel) {
+    for (auto& it : l) {
+      const int tt = it[0], a = it[1];
+      bySlot.clear();
+      for (int i = 0; i < maxZ; ++i) {
+        const int z = t.tileZone[(size_t)tt * maxZ + i];
+        if (z >= 0) bySlot.push_back({zstep[(size_t)a * Z + z], i});
+      }
+      std::sort(bySlot.begin(), bySlot.end());
+      const int n = bySlot.empty() ? 0 : bySlot.back().first + 1;
+      for (int i = 0; i < maxZ; ++i)
+        t.order.push_back(i < (int)bySlot.size() ? (unsigned char)bySlot[i].second : (unsigned char)255);
+      for (int k = 0, j = 0; k <= maxZ; ++k) {
+        while (j < (int)bySlot.size() && bySlot[j].first < k) ++j;
+        t.stepOff.push_back((unsigned char)j);
+      }
+      t.nsteps.push_back(n);
+      t.items.push_back(it);
+    }
+    t.levelOffset.push_back((int)t.items.size());
+  }
+  t.ok = true;
+}
+
 }  // namespace umt

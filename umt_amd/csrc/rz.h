@@ -52,6 +52,26 @@ struct RzLevels {
   std::vector<int> levelOffset;           // (levels + 1)
 };
 
+// The same sweep in tiles: zones are binned by centroid into tiles of about
+// T x T zones; item (tile, angle) runs after the items of its upwind tiles
+// and after (tile, previous angle of its xi level). One wave sweeps an
+// item's zones in steps, step s holding the zones whose longest chain of
+// upwind zones inside the tile has s links, with a barrier between steps,
+// so one launch covers several zone levels. ok is false (and the zone
+// levels are used) when some direction's tile graph has a cycle or a tile
+// holds more than kRzMaxTileZones zones.
+constexpr int kRzMaxTileZones = 64;
+struct RzTiles {
+  bool ok = false;
+  int T = 0, maxZones = 0, ntiles = 0;
+  std::vector<int> tileZone;              // (ntiles * maxZones) zone or -1
+  std::vector<std::array<int, 2>> items;  // (tile, angle), level by level
+  std::vector<int> levelOffset;           // (levels + 1)
+  std::vector<unsigned char> order;       // (items * maxZones) zone slots by step, 255 = none
+  std::vector<unsigned char> stepOff;     // (items * (maxZones + 1)) first entry of each step
+  std::vector<int> nsteps;                // (items)
+};
+
 // Any quad mesh: points (2, npoints) as (r, z), four node ids per zone
 // (0-based). Zones listed counterclockwise are reversed to the clockwise
 // order the corner-face conventions need.
@@ -68,5 +88,7 @@ int build_rz_schedule(const RzMesh& m, const RzGeometry& g, const RzQuadrature& 
                       std::string& err);
 void build_rz_levels(const RzMesh& m, const RzGeometry& g, const RzQuadrature& q,
                      const Schedule& s, RzLevels& lv);
+void build_rz_tiles(const RzMesh& m, const RzGeometry& g, const RzQuadrature& q,
+                    const Schedule& s, int T, RzTiles& t);
 
 }  // namespace umt

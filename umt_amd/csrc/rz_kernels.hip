@@ -50,18 +50,13 @@ __device__ __forceinline__ Real rz_sez(Real R, Real A, Real sig, Real area, Real
 
 }  // namespace
 
+// One lane: corner c of zone z in direction a for groups g0 .. g0 + K - 1;
+// the zone's four corners are four consecutive lanes of a quad. Lanes with
+// valid == false compute but store nothing.
 template <typename Real, int K>
-__global__ __launch_bounds__(256) void rz_sweep_level(RzLevelArgs<Real> p) {
-  const unsigned task = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
-  const int c = threadIdx.x & 3;
+__device__ __forceinline__ void rz_corner(const RzLevelArgs<Real>& p, int z, int a, int c, int g0,
+                                          bool valid) {
   const int G = p.G;
-  const unsigned perItem = (unsigned)(G / K);
-  int item = (int)fast_div(task, p.perItem);
-  const int g0 = K * (int)(task - (unsigned)item * perItem);
-  const bool valid = item < p.nItems;
-  if (!valid) item = 0;
-  const int2 it = p.items[item];
-  const int z = it.x, a = it.y;
   const RzAngle an = p.angles[a];
   const bool starting = an.flags & 1, finishingNext = an.flags & 2;
   const long cc = 4L * z + c;
@@ -176,6 +171,45 @@ __global__ __launch_bounds__(256) void rz_sweep_level(RzLevelArgs<Real> p) {
   }
 }
 
+template <typename Real, int K>
+__global__ __launch_bounds__(256) void rz_sweep_level(RzLevelArgs<Real> p) {
+  const unsigned task = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  const unsigned perItem = (unsigned)(p.G / K);
+  int item = (int)fast_div(task, p.perItem);
+  const int g0 = K * (int)(task - (unsigned)item * perItem);
+  const bool valid = item < p.nItems;
+  if (!valid) item = 0;
+  const int2 it = p.items[item];
+  rz_corner<Real, K>(p, it.x, it.y, threadIdx.x & 3, g0, valid);
+}
+
+// One wave per (tile, direction) item of a tile level and chunk of group
+// blocks: the tile's zones are swept in the item's steps, the zones of step
+// s reading upwind psi written at earlier steps (or by earlier launches),
+// with a barrier between steps. The 16 quads of the wave are 16 / gbPerChunk
+// zone positions times gbPerChunk group blocks; a step with more zones than
+// positions takes several passes. Every lane runs every step's barrier.
+template <typename Real, int K>
+__global__ __launch_bounds__(64) void rz_sweep_tile(RzTileArgs<Real> q) {
+  const int item = blockIdx.x;
+  const int quad = threadIdx.x >> 2;
+  const int pos = quad / q.gbPerChunk;
+  const int positions = 16 / q.gbPerChunk;
+  const int gb = blockIdx.y * q.gbPerChunk + (quad - pos * q.gbPerChunk);
+  const bool live = gb < q.p.G / K;
+  const int2 ti = q.items[item];
+  const unsigned char* order = q.order + (long)item * q.maxZones;
+  const unsigned char* off = q.stepOff + (long)item * (q.maxZones + 1);
+  const int* zones = q.tileZone + (long)ti.x * q.maxZones;
+  const int nsteps = q.nsteps[item];
+  for (int s = 0; s < nsteps; ++s) {
+    const int hi = off[s + 1];
+    for (int j = off[s] + pos; j < hi; j += positions)
+      if (live) rz_corner<Real, K>(q.p, zones[order[j]], ti.y, threadIdx.x & 3, K * gb, true);
+    __syncthreads();
+  }
+}
+
 // S = wtiso (sigma_s phi + q) with wtiso = 1/(2 pi) for RZ.
 template <typename Real>
 __global__ __launch_bounds__(256) void rz_source_update(const double* __restrict__ phi,
@@ -256,6 +290,21 @@ hipError_t launch_rz_sweep_level(const RzLevelArgs<Real>& a, int K, hipStream_t 
 }
 
 template <typename Real>
+hipError_t launch_rz_sweep_tile(const RzTileArgs<Real>& a, int nItems, int K, hipStream_t s) {
+  if (nItems == 0) return hipSuccess;
+  const int blocksPerItem = a.p.G / K;
+  if (a.gbPerChunk != 1 && a.gbPerChunk != 2 && a.gbPerChunk != 4) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nItems, (unsigned)((blocksPerItem + a.gbPerChunk - 1) / a.gbPerChunk));
+  const dim3 block(64);
+  switch (K) {
+    case 4: hipLaunchKernelGGL((rz_sweep_tile<Real, 4>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((rz_sweep_tile<Real, 2>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((rz_sweep_tile<Real, 1>), grid, block, 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+template <typename Real>
 hipError_t launch_rz_source_update(const double* phi, const double* sigs, const double* q, int G,
                                    long n, Real* S, hipStream_t s) {
   hipLaunchKernelGGL((rz_source_update<Real>), dim3(rz_blocks(n, 256)), dim3(256), 0, s, phi,
@@ -275,6 +324,8 @@ hipError_t launch_rz_residual(const double* phi, const double* vol, const double
 }
 
 template hipError_t launch_rz_sweep_level<double>(const RzLevelArgs<double>&, int, hipStream_t);
+template hipError_t launch_rz_sweep_tile<double>(const RzTileArgs<double>&, int, int, hipStream_t);
+template hipError_t launch_rz_sweep_tile<float>(const RzTileArgs<float>&, int, int, hipStream_t);
 template hipError_t launch_rz_sweep_level<float>(const RzLevelArgs<float>&, int, hipStream_t);
 template hipError_t launch_rz_source_update<double>(const double*, const double*, const double*,
                                                     int, long, double*, hipStream_t);

@@ -23,6 +23,7 @@ struct umt_rz_problem {
   umt::RzQuadrature q;
   umt::Schedule s;
   umt::RzLevels lv;
+  umt::RzTiles tiles;
 };
 
 struct umt_rz_ctx {
@@ -40,6 +41,15 @@ struct umt_rz_ctx {
   double* weight = nullptr;
   int2* items = nullptr;
   std::vector<int> levelOffset;
+  // Tile levels (umt::RzTiles), used when tilesOk and useTiles (off by
+  // default: each step of a tile pays the whole dependent-load chain of one
+  // item, which the zone levels hide behind their many independent items).
+  bool tilesOk = false, useTiles = false;
+  int tileMaxZones = 0;
+  int2* tileItems = nullptr;
+  unsigned char *tileOrder = nullptr, *tileStepOff = nullptr;
+  int *tileNsteps = nullptr, *tileZone = nullptr;
+  std::vector<int> tileLevelOffset;
   void* psi = nullptr;   // (G, C + nb, A)
   void* psiM = nullptr;  // (G, C, xi levels)
   std::vector<hipEvent_t> evLevel;
@@ -59,10 +69,16 @@ struct umt_rz_ctx {
   std::vector<Graph> graphs;
   size_t rsize() const { return precision == 32 ? sizeof(float) : sizeof(double); }
   long stride() const { return (long)(C + nb) * G; }
-  int levels() const { return (int)levelOffset.size() - 1; }
+  bool tiled() const { return tilesOk && useTiles; }
+  int levels() const {
+    return (int)(tiled() ? tileLevelOffset.size() : levelOffset.size()) - 1;
+  }
 };
 
 namespace {
+
+// Tiles of about kRzTile x kRzTile zones per workgroup.
+constexpr int kRzTile = 4;
 
 int finish_rz_problem(umt_rz_problem* p, int npolar, int nazimuthal) {
   try {
@@ -72,6 +88,7 @@ int finish_rz_problem(umt_rz_problem* p, int npolar, int nazimuthal) {
     if (int rc = umt::build_rz_schedule(p->m, p->g, p->q, p->s, err))
       return fail(60 + rc, "RZ schedule: %s", err.c_str());
     umt::build_rz_levels(p->m, p->g, p->q, p->s, p->lv);
+    umt::build_rz_tiles(p->m, p->g, p->q, p->s, kRzTile, p->tiles);
   } catch (const std::exception& e) {
     return fail(60, "RZ problem: %s", e.what());
   }
@@ -123,6 +140,24 @@ int sweep_impl(umt_rz_ctx* x, bool timeLevels, bool levelsOnly = false) {
   a.psiM = (Real*)x->psiM;
   a.tauPsi = x->tau > 0.0 ? (const Real*)x->psiPrev : nullptr;
   a.tau = (Real)x->tau;
+  if (x->tiled()) {
+    umt::RzTileArgs<Real> t{};
+    t.p = a;
+    t.tileZone = x->tileZone;
+    t.maxZones = x->tileMaxZones;
+    const int blocks = x->G / x->K;
+    t.gbPerChunk = blocks >= 4 ? 4 : (blocks >= 2 ? 2 : 1);
+    for (int h = 0; h < x->levels(); ++h) {
+      const int i0 = x->tileLevelOffset[h], n = x->tileLevelOffset[h + 1] - i0;
+      t.items = x->tileItems + i0;
+      t.order = x->tileOrder + (size_t)i0 * x->tileMaxZones;
+      t.stepOff = x->tileStepOff + (size_t)i0 * (x->tileMaxZones + 1);
+      t.nsteps = x->tileNsteps + i0;
+      if (timeLevels) HIP_TRY(hipEventRecord(x->evLevel[2 * h], x->stream));
+      HIP_TRY(umt::launch_rz_sweep_tile<Real>(t, n, x->K, x->stream));
+      if (timeLevels) HIP_TRY(hipEventRecord(x->evLevel[2 * h + 1], x->stream));
+    }
+  } else
   for (int h = 0; h < x->levels(); ++h) {
     a.items = x->items + x->levelOffset[h];
     a.nItems = x->levelOffset[h + 1] - x->levelOffset[h];
@@ -241,6 +276,27 @@ int create_impl(const umt_rz_problem* p, int G, int precision, umt_rz_ctx* x) {
   for (size_t i = 0; i < nItems; ++i) items[i] = make_int2(p->lv.items[i][0], p->lv.items[i][1]);
   HIP_TRY(hipMemcpyAsync(x->items, items.data(), nItems * sizeof(int2), hipMemcpyHostToDevice, x->stream));
   x->levelOffset = p->lv.levelOffset;
+  const umt::RzTiles& tl = p->tiles;
+  x->tilesOk = tl.ok;
+  if (tl.ok) {
+    x->tileMaxZones = tl.maxZones;
+    x->tileLevelOffset = tl.levelOffset;
+    const size_t ni = tl.items.size();
+    if ((rc = dalloc(&x->tileItems, ni)) || (rc = dalloc(&x->tileOrder, tl.order.size())) ||
+        (rc = dalloc(&x->tileStepOff, tl.stepOff.size())) ||
+        (rc = dalloc(&x->tileNsteps, ni)) || (rc = dalloc(&x->tileZone, tl.tileZone.size())))
+      return rc;
+    std::vector<int2> ti(ni);
+    for (size_t i = 0; i < ni; ++i) ti[i] = make_int2(tl.items[i][0], tl.items[i][1]);
+    HIP_TRY(hipMemcpyAsync(x->tileItems, ti.data(), ni * sizeof(int2), hipMemcpyHostToDevice, x->stream));
+    HIP_TRY(hipMemcpyAsync(x->tileOrder, tl.order.data(), tl.order.size(), hipMemcpyHostToDevice, x->stream));
+    HIP_TRY(hipMemcpyAsync(x->tileStepOff, tl.stepOff.data(), tl.stepOff.size(), hipMemcpyHostToDevice,
+                           x->stream));
+    HIP_TRY(hipMemcpyAsync(x->tileNsteps, tl.nsteps.data(), ni * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIP_TRY(hipMemcpyAsync(x->tileZone, tl.tileZone.data(), tl.tileZone.size() * sizeof(int),
+                           hipMemcpyHostToDevice, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  }
   const long nPsi = x->stride() * (long)A, nPm = (long)Gs * C * x->nlev;
   if (precision == 32) {
     HIP_TRY(umt::launch_fill_zero<float>((float*)x->psi, nPsi, x->stream));
@@ -255,7 +311,7 @@ int create_impl(const umt_rz_problem* p, int G, int precision, umt_rz_ctx* x) {
   HIP_TRY(umt::launch_fill_zero<double>(x->phibarOld, (long)Gs * Z, x->stream));
   HIP_TRY(umt::launch_fill_zero<double>(x->sigs, (long)Gs * Z, x->stream));
   HIP_TRY(umt::launch_fill_zero<double>(x->q, (long)Gs * Z, x->stream));
-  x->evLevel.resize(2 * (size_t)x->levels());
+  x->evLevel.resize(2 * std::max(x->levelOffset.size(), x->tileLevelOffset.size()));
   for (auto& e : x->evLevel) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
@@ -400,6 +456,8 @@ void umt_rz_ctx_destroy(umt_rz_ctx* x) {
   dfree(x->q); dfree(x->phi); dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
   dfree(x->angles); dfree(x->weight); dfree(x->items); dfree(x->psi); dfree(x->psiM);
   dfree(x->psiPrev);
+  dfree(x->tileItems); dfree(x->tileOrder); dfree(x->tileStepOff); dfree(x->tileNsteps);
+  dfree(x->tileZone);
   for (auto e : x->evLevel) (void)hipEventDestroy(e);
   if (x->ev0) (void)hipEventDestroy(x->ev0);
   if (x->ev1) (void)hipEventDestroy(x->ev1);
@@ -424,6 +482,7 @@ int umt_rz_ctx_dims(const umt_rz_ctx* x, int64_t* d) {
   d[0] = x->Z; d[1] = x->C; d[2] = x->nb; d[3] = x->G; d[4] = x->A;
   d[5] = x->levels();
   d[6] = x->levelOffset.back();
+  d[7] = x->tiled() ? 1 : 0;
   return 0;
 }
 
@@ -625,6 +684,15 @@ int umt_rz_ctx_autotune(umt_rz_ctx* x, int reps, int* bestK, double* msPerK) {
   }
   x->K = bk;
   if (bestK) *bestK = bk;
+  return 0;
+}
+
+int umt_rz_ctx_set_tiles(umt_rz_ctx* x, int on) {
+  if (!x) return fail(1, "null context");
+  if (on && !x->tilesOk) return fail(62, "this RZ problem has no tile schedule (a tile graph with a cycle)");
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  x->useTiles = on != 0;
+  drop_graphs(x);
   return 0;
 }
 

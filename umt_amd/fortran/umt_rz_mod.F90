@@ -28,7 +28,8 @@ module umt_rz_mod
             umt_rz_ctx_get_phi, umt_rz_ctx_set_phi, umt_rz_ctx_get_psi,                       &
             umt_rz_ctx_source_iteration, umt_rz_ctx_set_tau, umt_rz_ctx_set_psi_prev,         &
             umt_rz_ctx_get_psi_prev, umt_rz_ctx_save_psi, umt_rz_ctx_set_groups_per_lane,     &
-            umt_rz_ctx_set_graphs, umt_rz_ctx_time_sweeps, umt_rz_ctx_autotune
+            umt_rz_ctx_set_graphs, umt_rz_ctx_time_sweeps, umt_rz_ctx_autotune,   &
+            umt_rz_ctx_set_tiles
 
   interface
 
@@ -221,6 +222,12 @@ module umt_rz_mod
       integer(c_int), value :: reps
       integer(c_int)        :: bestK
       real(c_double)        :: msPerK(*)
+    end function
+
+    integer(c_int) function umt_rz_ctx_set_tiles(ctx, on) bind(c, name="umt_rz_ctx_set_tiles")
+      import :: c_int, c_ptr
+      type(c_ptr), value    :: ctx
+      integer(c_int), value :: on
     end function
 
     integer(c_int) function umt_rz_ctx_set_graphs(ctx, on) bind(c, name="umt_rz_ctx_set_graphs")

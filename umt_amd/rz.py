@@ -158,11 +158,11 @@ class RzContext:
             pass
 
     def dims(self):
-        d = np.zeros(7, dtype=np.int64)
+        d = np.zeros(8, dtype=np.int64)
         check("umt_rz_ctx_dims",
               _lib.load().umt_rz_ctx_dims(self._h, d.ctypes.data_as(C.POINTER(C.c_int64))))
-        return dict(zip(("zones", "corners", "nbelem", "groups", "angles", "levels", "items"),
-                        (int(x) for x in d)))
+        return dict(zip(("zones", "corners", "nbelem", "groups", "angles", "levels", "items",
+                         "tiled"), (int(x) for x in d)))
 
     def set_sigt(self, sigt):
         a = np.ascontiguousarray(sigt, dtype=np.float64)
@@ -226,6 +226,11 @@ class RzContext:
         out = self.source_iteration(tol, max_iters)
         self.save_psi()
         return out
+
+    def set_tiles(self, on):
+        """Sweep in tile levels or one launch per zone level (default); both
+        give the same bits."""
+        check("umt_rz_ctx_set_tiles", _lib.load().umt_rz_ctx_set_tiles(self._h, int(bool(on))))
 
     def set_graphs(self, on):
         """Replay sweeps from a captured graph (default) or launch them eagerly."""
