@@ -281,6 +281,30 @@ def rel_error(phi, ref):
     return float(np.max(np.abs(phi - ref) / np.maximum(np.abs(ref), 1e-300)))
 
 
+def dropin_rate(p, G, st, S, n, log):
+    """The per-angle drop-in gpu_sweepucbxyz with host arrays: every call
+    copies its inputs to the device and its outputs back, as the reference
+    CUDA path does, so the rate is PCIe-bound. Not the benchmark's value."""
+    import umt_amd as U
+
+    C, nb = p.ncorner, p.nbelem
+    Psi1 = np.zeros((C + nb, G))
+    PsiB = np.zeros((nb, G))
+    Phi = np.zeros((C, G))
+    if U.sweep_ucbxyz(p, 1, S, st, Psi1, PsiB, Phi) != 0:  # first call allocates
+        raise RuntimeError("drop-in sweep failed")
+    t0 = time.perf_counter()
+    for a in range(1, n + 1):
+        if U.sweep_ucbxyz(p, a, S, st, Psi1, PsiB, Phi) != 0:
+            raise RuntimeError(f"drop-in sweep failed at angle {a}")
+    dt = time.perf_counter() - t0
+    r = {"unknowns_per_s": C * G * n / dt, "ms_per_angle": dt * 1e3 / n, "angles_timed": n,
+         "note": "host arrays in and out every call (PCIe-inclusive); not value"}
+    log(f"drop-in (PCIe-inclusive): {r['unknowns_per_s']:.3e} unknowns/s, "
+        f"{r['ms_per_angle']:.1f} ms per angle")
+    return r
+
+
 def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0.95),
                      gta_only=False):
     """Plain FP64 source iteration (the reference method on the GPU) against
@@ -457,6 +481,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-converge", action="store_true", help="skip time-to-convergence")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 sweep")
+    ap.add_argument("--dropin-angles", type=int, default=4,
+                    help="angles timed through the per-angle drop-in with host arrays (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iters", type=int, default=3000)
@@ -586,6 +612,9 @@ def main():
                          "algorithmic_bytes_per_unknown": B32,
                          "groups_per_lane": r32["groups_per_lane"]},
         }
+
+    if args.dropin_angles > 0:
+        out["dropin_pcie"] = dropin_rate(p, G, st, S, args.dropin_angles, log)
 
     if not args.no_converge:
         out["time_to_converge"] = time_to_converge(
