@@ -231,7 +231,7 @@ int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const dou
                        const int *nHyperPlanes, const int *zonesInPlane, const int *nextZ);
 /* mode 0: plain source iteration (default); 1: GTA-accelerated; 2: learned
  * correction (umt_ctx_set_learner); 3: both, the learned correction applied to
- * the GTA-corrected iterates (GTA is then the fallback and the judge). */
+ * the GTA-corrected iterates (GTA is then the fallback and the arbiter). */
 int umt_ctx_set_acceleration(umt_ctx *ctx, int mode);
 /* Autotuner of the S2 grey sweeps' groups per lane (their levels are much
  * smaller than the S_N sweep's, so the best K can differ): times K = 1, 2,
@@ -352,7 +352,7 @@ int umt_batch_autotune(umt_batch *batch, int reps, int *bestK, int *bestAnglesPe
  * A per-(zone, group) network (10 features in 16 slots -> 64 -> 64 -> 1,
  * tanh; BF16 MFMA with FP32 accumulation, FP32 Adam masters) predicts r, and
  * every `every` sweeps the state becomes phi_half + r (phi_half - phi_prev)
- * per corner. Plain source iteration stays the judge: the residual is always
+ * per corner. Plain source iteration stays the arbiter: the residual is always
  * that of a plain sweep, a fixed point of plain iteration is left unchanged
  * (the correction scales the sweep's change), and with guard != 0 the
  * corrections stop for the rest of the solve as soon as a corrected state's

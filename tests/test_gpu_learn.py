@@ -282,7 +282,7 @@ def test_fp32_learned_then_fp64_matches_plain_fp64(cfg4, trained):
 
 def test_learned_correction_on_top_of_gta(cfg4):
     """Mode 3: the network trained on GTA-accelerated iterates corrects them
-    further; GTA stays the fallback and the judge, the fixed point is plain
+    further; GTA stays the fallback and the arbiter, the fixed point is plain
     iteration's, and it takes no more outer iterations than GTA alone."""
     p = cfg4[0]
     net, info = U.train_on_iterates(p, 2, seeds=[1, 2, 3], rounds=3, epochs=40, base="gta",

@@ -2,8 +2,8 @@
 restatement of snac/SweepUCBxyz.F90 / snac/SetSweep.F90 on the same seeded
 meshes and cross sections.
 
-Tolerances: FP64 sweep 1e-12 relative in every scalar-flux entry (the
-north-star bar); FP32 sweep 2e-5 relative in phi against the FP64 oracle
+Tolerances: FP64 sweep 1e-12 relative in every scalar-flux entry; FP32
+sweep 2e-5 relative in phi against the FP64 oracle
 (single-precision arithmetic on a sweep that mixes thin and thick zones).
 """
 import numpy as np

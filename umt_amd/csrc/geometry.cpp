@@ -113,8 +113,8 @@ void compute_geometry(const Mesh& m, Geometry& g) {
 
 // Product quadrature from the Spence QRS45 polar/azimuthal tables
 // (mods/QuadratureData_mod.F90:1144-2857, N = 1..4 only), assembled as
-// rt/quadProduct.F90:73-182 does with the polar axis along x (the driver's
-// default, driver/test_driver.cc:1198), then normalised so that the weights
+// rt/quadProduct.F90:73-182 does with the polar axis along x (UMT's test
+// program's default, driver/test_driver.cc:1198), then normalised so that the weights
 // sum to 4*pi (rt/rtquad.F90:95-105).
 
 void product_quadrature(int npolar, int nazimuthal, Quadrature& q) {

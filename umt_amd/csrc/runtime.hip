@@ -1696,7 +1696,7 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
       break;
     }
     if (track) {
-      // Plain source iteration judges every correction: once a corrected
+      // Plain source iteration arbitrates every correction: once a corrected
       // state's sweep changes phi more than the sweep the correction
       // followed, corrections stop and the iteration continues plain.
       if (lastCorrectedRes >= 0.0) {

@@ -2,7 +2,7 @@
 umt_learner_* and umt_ctx_learn_* entry points).
 
 A per-(zone, group) network predicts r; every `every` sweeps the state
-becomes phi_half + r (phi_half - phi_prev). Plain source iteration judges
+becomes phi_half + r (phi_half - phi_prev). Plain source iteration decides
 convergence and is the fallback (guard). The network is trained on the
 method's own iterates: solves with collection on record their states, and
 after convergence each state's target is the correction that would have

@@ -462,7 +462,7 @@ def solve_mixed(fast, exact, tol=1e-8, switch_tol=1e-6, max_iters=1000, stall_wi
     """Source iteration with FP32 sweeps (context `fast`) until the residual
     falls below switch_tol (or stops improving at the FP32 rounding floor),
     then FP64 sweeps (context `exact`, same problem and cross sections) from
-    that phi to tol; the FP64 residual judges convergence. Both contexts keep
+    that phi to tol; the FP64 residual decides convergence. Both contexts keep
     their own acceleration. fast's phi is set to the result. Returns
     (converged, FP32 iterations, FP64 iterations, residual, history)."""
     fast.set_stall_window(stall_window)
