@@ -139,6 +139,10 @@ int domain_levels(const umt::Schedule& s, int Z, int A, Domain& d) {
   d.items.clear();
   d.levelOffset.assign(1, 0);
   for (auto& l : lv) {
+    // Zone order (then angle), as in the one-domain levels: a zone's
+    // directions of one level read its source and geometry side by side.
+    std::sort(l.begin(), l.end(),
+              [](int2 p, int2 q) { return p.x != q.x ? p.x < q.x : p.y < q.y; });
     d.items.insert(d.items.end(), l.begin(), l.end());
     d.levelOffset.push_back((int)d.items.size());
   }

@@ -693,6 +693,16 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
     }
     bl.push_back((int)lo.size() - 1);
   }
+  // A level's items in zone order (then angle): the directions that reach a
+  // zone at the same level (on hex tiles, classes of directions share their
+  // hyperplanes) sit side by side, so their reads of the zone's source,
+  // cross section and corner geometry hit the same cache lines. Items of a
+  // level are independent, so the order changes no result.
+  for (size_t h = 0; h + 1 < lo.size(); ++h)
+    std::sort(items.begin() + lo[h], items.begin() + lo[h + 1], [](int2 p, int2 q) {
+      const int zp = p.x < 0 ? ~p.x : p.x, zq = q.x < 0 ? ~q.x : q.x;
+      return zp != zq ? zp < zq : p.y < q.y;
+    });
   if (x->stream) HIP_TRY(hipStreamSynchronize(x->stream));
   for (auto& g : set.graphs) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
