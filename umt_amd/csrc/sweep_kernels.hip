@@ -156,14 +156,32 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   // Per group: volume and incoming FP terms (SweepUCBxyz.F90:119-161), then
   // the EZ coupling terms computed by the upwind corner (:182-252), pulled
   // with the opposite sign by the downwind one.
+  // Incoming FP areas, +0 on the other faces (whose psf is 0, so their term
+  // is s - (+0) = s, as the reference skips them).
+  Real afpIn[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) afpIn[f] = afp[f] < 0 ? afp[f] : (Real)0;
   Real src[K], rden[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const Real sig = sigv_.v[k];
     const Real q = Q.v[k];
-    Real s = v * q;
+    // Volume term and incoming FP terms (SweepUCBxyz.F90:139-161): each
+    // face product rounded on its own, the first taken off V q in one fused
+    // step. Both forms give the same bits; the masked one saves the FP64
+    // kernel 18 selects (1% faster), the FP32 kernel is 1% faster with the
+    // selects.
+    Real s;
+    if constexpr (sizeof(Real) == 8) {
+#pragma clang fp contract(off)
+      s = fma(v, q, -(afpIn[0] * psf[0].v[k]));
+      s = s - afpIn[1] * psf[1].v[k];
+      s = s - afpIn[2] * psf[2].v[k];
+    } else {
+      s = v * q;
 #pragma unroll
-    for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
+      for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
+    }
     rden[k] = quot<Real>(1, outArea + sig * v);
     const Real halfSigInv = quot<Real>((Real)0.5, sig);
     const Real Qn[3] = {from_ez<0>(q), from_ez<1>(q), from_ez<2>(q)};
