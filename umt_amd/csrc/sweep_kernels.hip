@@ -361,12 +361,15 @@ __global__ __launch_bounds__(256) void phi_from_psi_vec(const Real* __restrict__
   double acc[V];
 #pragma unroll
   for (int k = 0; k < V; ++k) acc[k] = accumulate ? phi[i + k] : 0.0;
+  // psi is read once here: non-temporal loads keep it from displacing
+  // reusable lines in L2 and the MALL.
+  typedef Real Raw __attribute__((ext_vector_type(V)));
 #pragma unroll 8
   for (int a = 0; a < A; ++a) {
-    const Vec x = load_groups<Real, V>(psi + (long)a * angleStride + i);
+    const Raw x = __builtin_nontemporal_load((const Raw*)(psi + (long)a * angleStride + i));
     const double w = weight[a];
 #pragma unroll
-    for (int k = 0; k < V; ++k) acc[k] += w * (double)x.v[k];
+    for (int k = 0; k < V; ++k) acc[k] += w * (double)x[k];
   }
 #pragma unroll
   for (int k = 0; k < V; ++k) phi[i + k] = acc[k];
