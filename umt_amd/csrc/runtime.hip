@@ -703,6 +703,45 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
       const int zp = p.x < 0 ? ~p.x : p.x, zq = q.x < 0 ? ~q.x : q.x;
       return zp != zq ? zp < zq : p.y < q.y;
     });
+  // XCD-aware order: workgroup w of a level launch runs on XCD w mod 8, so
+  // slot w takes the next workgroup's worth of items from zone range w mod 8
+  // (eight equal ranges of zone ids), or from the fullest range once its own
+  // is used up. A zone and its upwind neighbours then mostly meet in the same
+  // XCD's L2 from level to level.
+  {
+    constexpr int kXcd = 8;
+    const int K = 4;  // the widest lane layout the autotuner picks for G % 4 == 0
+    const int perWg = std::max(1, 256 / (8 * std::max(1, x->G / K)));
+    std::vector<int2> out;
+    for (size_t h = 0; h + 1 < lo.size(); ++h) {
+      std::array<int, kXcd + 1> start;
+      start[0] = lo[h];
+      for (int r = 1; r < kXcd; ++r) {
+        const int zcut = (int)((long)x->Z * r / kXcd);
+        start[r] = (int)(std::lower_bound(items.begin() + start[r - 1], items.begin() + lo[h + 1], zcut,
+                                          [](int2 it, int zc) { return (it.x < 0 ? ~it.x : it.x) < zc; }) -
+                         items.begin());
+      }
+      start[kXcd] = lo[h + 1];
+      std::array<int, kXcd> next;
+      for (int r = 0; r < kXcd; ++r) next[r] = start[r];
+      out.clear();
+      for (long slot = 0; (int)out.size() < lo[h + 1] - lo[h]; ++slot) {
+        int r = (int)(slot % kXcd);
+        if (next[r] == start[r + 1]) {
+          int best = -1;
+          for (int t = 0; t < kXcd; ++t)
+            if (next[t] < start[t + 1] && (best < 0 || start[t + 1] - next[t] > start[best + 1] - next[best]))
+              best = t;
+          r = best;
+        }
+        const int take = std::min(perWg, start[r + 1] - next[r]);
+        out.insert(out.end(), items.begin() + next[r], items.begin() + next[r] + take);
+        next[r] += take;
+      }
+      std::copy(out.begin(), out.end(), items.begin() + lo[h]);
+    }
+  }
   if (x->stream) HIP_TRY(hipStreamSynchronize(x->stream));
   for (auto& g : set.graphs) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
