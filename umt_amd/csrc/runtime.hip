@@ -710,8 +710,10 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
   // XCD's L2 from level to level.
   {
     constexpr int kXcd = 8;
-    const int K = 4;  // the widest lane layout the autotuner picks for G % 4 == 0
-    const int perWg = std::max(1, 256 / (8 * std::max(1, x->G / K)));
+    // Items per 256-lane workgroup for the widest groups-per-lane layout G
+    // allows (the autotuner's usual pick); another layout only mixes ranges.
+    const int K = x->G % 4 == 0 ? 4 : (x->G % 2 == 0 ? 2 : 1);
+    const int perWg = std::max(1, 32 / std::max(1, x->G / K));
     std::vector<int2> out;
     for (size_t h = 0; h + 1 < lo.size(); ++h) {
       std::array<int, kXcd + 1> start;
