@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -273,7 +274,40 @@ int create_impl(const umt_rz_problem* p, int G, int precision, umt_rz_ctx* x) {
   HIP_TRY(hipMemcpyAsync(x->angles, an.data(), A * sizeof(umt::RzAngle), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipMemcpyAsync(x->weight, q.weight.data(), A * sizeof(double), hipMemcpyHostToDevice, x->stream));
   std::vector<int2> items(nItems);
-  for (size_t i = 0; i < nItems; ++i) items[i] = make_int2(p->lv.items[i][0], p->lv.items[i][1]);
+  // XCD-aware order: workgroup w of a level launch runs on XCD w mod 8, so
+  // slot w takes the next workgroup's worth of items (in the level's own
+  // order) whose zone lies in zone range w mod 8 of eight equal zone-id
+  // ranges, or from the fullest range once its own is used up. A zone and
+  // its upwind neighbours then mostly meet in one XCD's L2 from level to
+  // level. Items of a level are independent, so the order changes no result.
+  {
+    constexpr int kXcd = 8;
+    const int perWg = std::max(1, 64 / std::max(1, x->G / x->K));
+    const std::vector<int>& lo = p->lv.levelOffset;
+    std::array<std::vector<int2>, kXcd> part;
+    for (size_t h = 0; h + 1 < lo.size(); ++h) {
+      for (auto& v : part) v.clear();
+      for (int i = lo[h]; i < lo[h + 1]; ++i) {
+        const int z = p->lv.items[i][0];
+        part[(int)((long)z * kXcd / x->Z)].push_back(make_int2(z, p->lv.items[i][1]));
+      }
+      std::array<size_t, kXcd> next{};
+      size_t o = lo[h];
+      for (long slot = 0; o < (size_t)lo[h + 1]; ++slot) {
+        int r = (int)(slot % kXcd);
+        if (next[r] == part[r].size()) {
+          int best = -1;
+          for (int t = 0; t < kXcd; ++t)
+            if (next[t] < part[t].size() &&
+                (best < 0 || part[t].size() - next[t] > part[best].size() - next[best]))
+              best = t;
+          r = best;
+        }
+        const size_t take = std::min((size_t)perWg, part[r].size() - next[r]);
+        for (size_t k = 0; k < take; ++k) items[o++] = part[r][next[r]++];
+      }
+    }
+  }
   HIP_TRY(hipMemcpyAsync(x->items, items.data(), nItems * sizeof(int2), hipMemcpyHostToDevice, x->stream));
   x->levelOffset = p->lv.levelOffset;
   const umt::RzTiles& tl = p->tiles;
