@@ -633,7 +633,9 @@ void umt_ctx_destroy(umt_ctx* x) {
 }
 
 // Hyperplane zone batches of one quadrature: level h holds plane h of every
-// angle, angles in order, zones in each plane's nextZ order.
+// angle, angles in order, zones in each plane's nextZ order (block_set
+// later reorders each level's items for the device; the order inside a level
+// changes no result).
 static int batch_levels(int Z, int A, const int* nHyperPlanes, const int* zonesInPlane,
                         const int* nextZ, std::vector<int2>& items, std::vector<int>& levelOffset,
                         bool* cyclicZones = nullptr) {
