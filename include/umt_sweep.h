@@ -29,9 +29,12 @@ extern "C" {
  * Psi(:,:,Angle) = Psi1 when *savePsi != 0. Outputs are complete when the call
  * returns (gpu_streamsynchronize is kept for the caller's sequence).
  * Supported: 3D hex zones in the corner-face order setTetonZone produces,
- * acyclic schedules (*numCycles == 0), no reflecting boundary
- * (*nBdyElem == 0 or *Mref < 1). Anything else leaves the outputs untouched
- * and sets a nonzero gpu_sweepucbxyz_status().
+ * acyclic schedules and schedules with cycle lists (*numCycles corners of
+ * cycleList from *cycleOffSet load Psi1 from cyclePsi before the sweep and
+ * store back after it, control/constructDynMemory.F90:118-189; cyclic zones
+ * come as negative nextZ entries), no reflecting boundary (*nBdyElem == 0 or
+ * *Mref < 1). Anything else leaves the outputs untouched and sets a nonzero
+ * gpu_sweepucbxyz_status().
  * ---------------------------------------------------------------------- */
 void gpu_sweepucbxyz(int *Angle, int *nHyperPlanes, int *nZonesInPlane, int *nextZ, int *nextC,
                      double *STotal, double *tau, double *Psi, int *Groups, double *Volume,
@@ -188,7 +191,9 @@ int umt_ctx_set_phi(umt_ctx *ctx, const double *phi);   /* (G,C) */
  * averages volume-weighted. Starts from the phi held by the context. Stops
  * when residual < tol or after maxIters iterations; residual_history (may be
  * NULL) receives up to maxIters values. Returns 0 when converged, 1 when
- * maxIters was reached (the state is still valid). */
+ * maxIters was reached (the state is still valid), 2 when the stall window
+ * ended it, 34 when the residual is not finite (a NaN or Inf zone average:
+ * the iterates diverged) and 52 when some sigs exceeds Sigt. */
 int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
                              double *residual, double *residual_history);
 /* Time-dependent sweeps. tau = 1/(c dt) (control/initializeSets.F90:81):

@@ -424,6 +424,9 @@ int umt_batch_set_cross_sections(umt_batch* b, int domain, const double* sigt, c
     if (!(sigt[i] > 0.0) || !std::isfinite(sigt[i]) || !(sigs[i] >= 0.0) || !std::isfinite(sigs[i]) ||
         !std::isfinite(q[i]))
       return fail(30, "domain %d: cross-section entry %zu is not valid", domain, i + 1);
+  for (size_t i = 0; i < n; ++i)
+    if (sigs[i] > sigt[i])
+      return fail(52, "domain %d: sigs entry %zu exceeds Sigt (scattering ratio above 1)", domain, i + 1);
   const std::vector<double> st = pad_groups(sigt, d.G, d.Gp, d.Z, 1.0);
   const std::vector<double> ss = pad_groups(sigs, d.G, d.Gp, d.Z, 0.0);
   const std::vector<double> qq = pad_groups(q, d.G, d.Gp, d.Z, 0.0);

@@ -32,4 +32,15 @@ __device__ __forceinline__ double quot<double>(double a, double b) {
   return fma(r, fma(-b, q, a), q);
 }
 
+// Max that keeps a NaN once one is seen. The convergence kernels reduce
+// non-negative doubles through atomicMax on their bit patterns; a positive
+// quiet NaN's pattern is above +Inf's and every finite value's, so a NaN
+// that reaches the atomic wins and the host's finiteness check sees it.
+__device__ __forceinline__ double nan_keep_max(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return fmax(a, b);
+}
+constexpr unsigned long long kNaNBits = 0x7ff8000000000000ull;
+
 }  // namespace umt

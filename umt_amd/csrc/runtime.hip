@@ -576,12 +576,16 @@ int umt_draw_cross_sections(uint64_t seed, int nzones, int groups, double sigt_l
 // ---------------------------------------------------------------------------
 // Device-resident context
 
-static void free_set(SweepSet& set) {
+static void drop_graphs(SweepSet& set) {
   for (auto& g : set.graphs) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
   }
   set.graphs.clear();
+}
+
+static void free_set(SweepSet& set) {
+  drop_graphs(set);
   dfree(set.omega);
   dfree(set.weight);
   dfree(set.items);
@@ -618,6 +622,9 @@ static void free_learn(LearnState& l) {
 void umt_ctx_destroy(umt_ctx* x) {
   if (!x) return;
   if (x->stream) (void)hipStreamSynchronize(x->stream);
+  // Captured sweeps go first, then the buffers their kernel nodes name.
+  drop_graphs(x->sn);
+  drop_graphs(x->s2);
   free_learn(x->learn);
   dfree(x->psiPrev);
   dfree(x->Afp); dfree(x->Aez); dfree(x->vol); dfree(x->cFP); dfree(x->vol64); dfree(x->volZ);
@@ -747,11 +754,7 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
     }
   }
   if (x->stream) HIP_TRY(hipStreamSynchronize(x->stream));
-  for (auto& g : set.graphs) {
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (g.graph) (void)hipGraphDestroy(g.graph);
-  }
-  set.graphs.clear();
+  drop_graphs(set);
   for (auto e : set.evLevel) (void)hipEventDestroy(e);
   set.evLevel.clear();
   dfree(set.items);
@@ -985,14 +988,6 @@ static int upload_sigt(umt_ctx* x) {
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
-}
-
-static void drop_graphs(SweepSet& set) {
-  for (auto& g : set.graphs) {
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (g.graph) (void)hipGraphDestroy(g.graph);
-  }
-  set.graphs.clear();
 }
 
 int umt_ctx_set_sigt(umt_ctx* x, const double* sigt) {
@@ -1306,7 +1301,8 @@ int umt_ctx_enable_gta(umt_ctx* x, int nangles, const double* omega, const doubl
                        const int* nHyperPlanes, const int* zonesInPlane, const int* nextZ) {
   if (!x || !omega || !weight || !nHyperPlanes || !zonesInPlane || !nextZ) return fail(1, "null argument");
   if (nangles < 1) return fail(1, "bad angle count");
-  if (x->gta.enabled) {
+  if (x->gta.enabled) {  // replaces the S2 set: no sweep may still read it
+    HIP_TRY(hipStreamSynchronize(x->stream));
     free_set(x->s2);
     free_gta(x->gta);
   }
@@ -1673,6 +1669,11 @@ int umt_ctx_source_iteration(umt_ctx* x, double tol, int maxIters, int* iters, d
                              double* hist) {
   if (!x) return fail(1, "null context");
   if (!x->haveScattering) return fail(33, "source iteration needs umt_ctx_set_scattering first");
+  // sigma_a = sigma_t - sigma_s >= 0: with more scattering than collision
+  // the iteration diverges (and the grey opacities have no real stretch).
+  for (size_t i = 0; i < x->sigsHost.size() && i < x->sigtHost.size(); ++i)
+    if (x->sigsHost[i] > x->sigtHost[i])
+      return fail(52, "sigs entry %zu exceeds Sigt (scattering ratio above 1)", i + 1);
   const long nGC = (long)x->G * x->C, nGZ = (long)x->G * x->Z;
   double res = 1.0;
   int it = 0;

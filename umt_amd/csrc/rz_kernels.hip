@@ -242,9 +242,12 @@ __global__ __launch_bounds__(256) void rz_zone_average(const double* __restrict_
     v = acc / volZ[z];
     phibar[i] = v;
   }
+  // A non-finite average becomes NaN and is kept (the residual is then NaN).
   double m = v > 0.0 ? v : 0.0;
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxBits, (unsigned long long)__double_as_longlong(m));
+  if (!isfinite(v)) m = __longlong_as_double((long long)kNaNBits);
+  for (int off = 32; off > 0; off >>= 1) m = nan_keep_max(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && (m > 0.0 || m != m))
+    atomicMax(maxBits, (unsigned long long)__double_as_longlong(m));
 }
 
 __global__ __launch_bounds__(256) void rz_residual_max(const double* __restrict__ phibar,
@@ -255,13 +258,16 @@ __global__ __launch_bounds__(256) void rz_residual_max(const double* __restrict_
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
   if (i < n) {
-    const double thr = cutoff * __longlong_as_double((long long)*maxBits);
+    const double mx = __longlong_as_double((long long)*maxBits);
+    const double thr = cutoff * mx;
     const double v = phibar[i];
     if (v > thr) r = fabs((v - phibarOld[i]) / v);
+    if (!isfinite(v) || !isfinite(mx)) r = __longlong_as_double((long long)kNaNBits);
     phibarOld[i] = v;
   }
-  for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_down(r, off, 64));
-  if ((threadIdx.x & 63) == 0 && r > 0.0) atomicMax(resBits, (unsigned long long)__double_as_longlong(r));
+  for (int off = 32; off > 0; off >>= 1) r = nan_keep_max(r, __shfl_down(r, off, 64));
+  if ((threadIdx.x & 63) == 0 && (r > 0.0 || r != r))
+    atomicMax(resBits, (unsigned long long)__double_as_longlong(r));
 }
 
 __global__ void rz_fill_u64(unsigned long long* p, int n) {

@@ -56,6 +56,7 @@ struct umt_rz_ctx {
   std::vector<hipEvent_t> evLevel;
   bool haveSigt = false, haveScattering = false;
   std::vector<double> sigtHost;  // sigma_t as set; the sweep sees sigma_t + tau
+  std::vector<double> sigsHost;  // sigma_s as set (checked against sigma_t)
   double tau = 0.0;              // 1/(c dt): time-dependent sweeps when > 0
   void* psiPrev = nullptr;       // previous cycle's psi (G, C, A), sweep precision
   // Captured sweeps: the level launches and the phi reduction (sweep), or
@@ -549,6 +550,7 @@ int umt_rz_ctx_set_scattering(umt_rz_ctx* x, const double* sigs, const double* q
   HIP_TRY(hipMemcpyAsync(x->q, q, n * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
   x->haveScattering = true;
+  x->sigsHost.assign(sigs, sigs + n);
   return 0;
 }
 
@@ -592,6 +594,9 @@ int umt_rz_ctx_source_iteration(umt_rz_ctx* x, double tol, int maxIters, int* it
                                 double* residual, double* hist) {
   if (!x || maxIters < 0) return fail(1, "bad argument");
   if (!x->haveScattering) return fail(33, "umt_rz_ctx_set_scattering first");
+  for (size_t i = 0; i < x->sigsHost.size() && i < x->sigtHost.size(); ++i)
+    if (x->sigsHost[i] > x->sigtHost[i])
+      return fail(52, "sigs entry %zu exceeds Sigt (scattering ratio above 1)", i + 1);
   const long nGC = (long)x->G * x->C, nGZ = (long)x->G * x->Z;
   // The old zone averages come from the starting phi.
   HIP_TRY(umt::launch_rz_residual(x->phi, x->vol64, x->volZ, x->G, nGZ, x->phibar, x->phibarOld,
@@ -611,6 +616,7 @@ int umt_rz_ctx_source_iteration(umt_rz_ctx* x, double tol, int maxIters, int* it
     HIP_TRY(hipStreamSynchronize(x->stream));
     double r;
     std::memcpy(&r, &b[1], sizeof r);
+    if (!std::isfinite(r)) return fail(34, "RZ source iteration residual is not finite at iteration %d", it + 1);
     res = r;
     if (hist) hist[it] = res;
     ++it;

@@ -409,11 +409,15 @@ __global__ __launch_bounds__(256) void zone_average(const double* __restrict__ p
     v = acc / volZ[z];
     phibar[i] = v;
   }
-  // Wave-level max of non-negative values, then one atomic per wave.
+  // Wave-level max of non-negative values, then one atomic per wave. A
+  // non-finite average (NaN, or +-Inf) becomes NaN and is kept, so the
+  // residual that follows is NaN and the host stops the iteration.
   if (magnitude) v = fabs(v);
   double m = v > 0.0 ? v : 0.0;
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxBits, (unsigned long long)__double_as_longlong(m));
+  if (!isfinite(v)) m = __longlong_as_double((long long)kNaNBits);
+  for (int off = 32; off > 0; off >>= 1) m = nan_keep_max(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && (m > 0.0 || m != m))
+    atomicMax(maxBits, (unsigned long long)__double_as_longlong(m));
 }
 
 // Max relative change of zone averages over entries above cutoff * max
@@ -429,14 +433,19 @@ __global__ __launch_bounds__(256) void residual_max(const double* __restrict__ p
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
   if (i < n) {
-    const double thr = cutoff * __longlong_as_double((long long)*maxBits);
+    const double mx = __longlong_as_double((long long)*maxBits);
+    const double thr = cutoff * mx;
     const double v = phibar[i];
     const double a = magnitude ? fabs(v) : v;
     if (a > thr) r = fabs((v - phibarOld[i]) / v);
+    // A non-finite average anywhere (here, or behind a non-finite max)
+    // makes the residual NaN.
+    if (!isfinite(v) || !isfinite(mx)) r = __longlong_as_double((long long)kNaNBits);
     phibarOld[i] = v;
   }
-  for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_down(r, off, 64));
-  if ((threadIdx.x & 63) == 0 && r > 0.0) atomicMax(resBits, (unsigned long long)__double_as_longlong(r));
+  for (int off = 32; off > 0; off >>= 1) r = nan_keep_max(r, __shfl_down(r, off, 64));
+  if ((threadIdx.x & 63) == 0 && (r > 0.0 || r != r))
+    atomicMax(resBits, (unsigned long long)__double_as_longlong(r));
 }
 
 __global__ void fill_u64(unsigned long long* p, int n, unsigned long long v) {
