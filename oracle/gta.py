@@ -160,3 +160,284 @@ def source_iteration_gta(mesh, geom, quad, sched, quad2, sched2, sigt, sigs, q, 
         if res < tol:
             break
     return phi, it, res, np.array(hist), grey.sweeps
+
+
+# ---------------------------------------------------------------------------
+# The reference's "new" GTA solver (Size%useNewGTASolver): per-zone corner
+# transfer matrices and a direct solve of each zone's grey scattering problem
+# with in-place LU factors, in place of the stretched-TSA grey sweeps.
+#   snac/InitSweepGreyUCBxyz.F90:10-253   transfer matrices TT
+#   snac/SweepGreyUCBxyz.F90:137-322      grey sweep with the incident part pInc
+#   snac/GTASweep.F90:80-115              TSA source, PhiInc = sum_a w pInc
+#   snac/UpdateScalarIntensity.F90:11-170 LU of I - TT wtiso sigma_s, solve
+#   rt/GreySweep.F90:12-47                GreySweepNEW
+#   rt/GTASolver.F90:158-198              matrices built once per solve, the
+#                                         first (with-source) sweep decomposes
+
+def _dot_tol(o, a, tol):
+    d = o[0] * a[0] + o[1] * a[1] + o[2] * a[2]
+    if tol > 0.0 and abs(d) <= tol * (abs(a[0]) + abs(a[1]) + abs(a[2])):
+        return 0.0
+    return d
+
+
+def _sez_coefficients(aez, sigv, sig, area_opp):
+    """(B1, B2, gtau): the EZ term of an outgoing face of corner c towards
+    cez as sez = B1 S_c - B2 S_cez + gtau sigv psi_opp, S per steradian
+    (InitSweepGreyUCBxyz.F90:166-214; Q = S / sigma)."""
+    if area_opp > 0.0:
+        sigv2 = sigv * sigv
+        gnum = aez * aez * (1.82 * sigv2 + aez * (4.0 * sigv + 3.0 * aez))
+        gtau = gnum / (gnum + 4.0 * sigv2 * sigv2 + aez * sigv * (6.0 * sigv2 + 2.0 * aez *
+                                                                   (2.0 * sigv + aez)))
+        B0 = 0.5 * aez * (1.0 - gtau)
+        return (B0 - gtau * sigv) / sig, B0 / sig, gtau
+    return 0.5 * aez / sig, 0.5 * aez / sig, None
+
+
+def _zone_faces(mesh, geom, z, omega, par_tol):
+    """Per corner of zone z: afp, aez (with the parallel tolerance), cEZ
+    (0-based, local) and the downstream lists of the corner solve."""
+    c0 = int(mesh["cOffSet"][z])
+    nC = int(mesh["numCorner"][z])
+    afp = [[_dot_tol(omega, geom["A_fp"][c0 + c, f], par_tol)
+            for f in range(int(mesh["nCFaces"][c0 + c]))] for c in range(nC)]
+    aez = [[_dot_tol(omega, geom["A_ez"][c0 + c, f], par_tol)
+            for f in range(int(mesh["nCFaces"][c0 + c]))] for c in range(nC)]
+    cez = [[int(mesh["cEZ"][c0 + c, f]) - 1 for f in range(int(mesh["nCFaces"][c0 + c]))]
+           for c in range(nC)]
+    need = [0] * nC
+    ez_exit = [[] for _ in range(nC)]
+    coef = [[] for _ in range(nC)]
+    for c in range(nC):
+        for f in range(len(aez[c])):
+            e = cez[c][f]
+            if e > c:
+                if aez[c][f] > 0.0:
+                    need[e] += 1
+                    ez_exit[c].append(e)
+                    coef[c].append(aez[c][f])
+                elif aez[c][f] < 0.0:
+                    need[c] += 1
+                    ez_exit[e].append(c)
+                    coef[e].append(-aez[c][f])
+    return c0, nC, afp, aez, cez, need, ez_exit, coef
+
+
+def _area_opp(afp_c, f):
+    """Incoming area of the FP faces opposite EZ face f (the next face for a
+    3-face corner, else the n - 2 following ones), and those faces."""
+    n = len(afp_c)
+    faces = [(f + 1) % n] if n == 3 else [(f + k) % n for k in range(1, n - 1)]
+    area = 0.0
+    inc = []
+    for ifp in faces:
+        if afp_c[ifp] < 0.0:
+            area -= afp_c[ifp]
+            inc.append(ifp)
+    return area, inc
+
+
+def transfer_matrices(mesh, geom, quad2, sigt_g, par_tol=PAR_TOL):
+    """TT (Z, G, 8, 8): TT[z, g, s, c] = scalar flux at corner c of zone z
+    from a unit per-steradian source at corner s of the same zone, summed
+    over the grey directions with their weights, no incident flux
+    (GTA%TT(s, c0 + c), InitSweepGreyUCBxyz.F90:10-253)."""
+    Z = mesh["numCorner"].shape[0]
+    G = sigt_g.shape[1]
+    V = geom["Volume"]
+    TT = np.zeros((Z, G, 8, 8))
+    for z in range(Z):
+        sig = sigt_g[z]
+        for a in range(quad2["weight"].shape[0]):
+            omega = quad2["omega"][a]
+            c0, nC, afp, aez, cez, need, ez_exit, coef = _zone_faces(mesh, geom, z, omega, par_tol)
+            Pvv = np.zeros((G, nC, nC))  # Pvv[g, column (source), row (flux)]
+            for c in range(nC):
+                Pvv[:, c, c] = V[c0 + c]
+            denom = np.zeros((G, nC))
+            for c in range(nC):
+                sigv = V[c0 + c] * sig
+                denom[:, c] = sigv
+                for f in range(len(afp[c])):
+                    if afp[c][f] > 0.0:
+                        denom[:, c] += afp[c][f]
+                for f in range(len(aez[c])):
+                    a_ez = aez[c][f]
+                    if a_ez > 0.0:
+                        e = cez[c][f]
+                        denom[:, c] += a_ez
+                        area, _ = _area_opp(afp[c], f)
+                        B1, B2, _ = _sez_coefficients(a_ez, sigv, sig, area)
+                        Pvv[:, c, c] += B1
+                        Pvv[:, e, c] -= B2
+                        Pvv[:, c, e] -= B1
+                        Pvv[:, e, e] += B2
+            nd = list(need)
+            for _ in range(nC):
+                c = int(np.argmin(nd))
+                Pvv[:, :, c] *= (1.0 / denom[:, c])[:, None]
+                for e, k in zip(ez_exit[c], coef[c]):
+                    nd[e] -= 1
+                    Pvv[:, :, e] += k * Pvv[:, :, c]
+                nd[c] = 99
+            TT[z, :, :nC, :nC] += quad2["weight"][a] * Pvv
+    return TT
+
+
+def pinc_sweep(mesh, geom, quad2, sched2, sigt_g, tsa, par_tol=PAR_TOL):
+    """The grey sweep of the new solver for every grey direction, vacuum
+    boundaries: returns (phi, PhiInc), phi = sum_a w tPsi and PhiInc =
+    sum_a w pInc, pInc being the part of tPsi carried by incident fluxes
+    (SweepGreyUCBxyz.F90:137-322, GTASweep.F90:108-115). tsa is TsaSource
+    (C, G) per steradian."""
+    Z = mesh["numCorner"].shape[0]
+    Cn = mesh["nCFaces"].shape[0]
+    G = sigt_g.shape[1]
+    V = geom["Volume"]
+    phi = np.zeros((Cn, G))
+    phi_inc = np.zeros((Cn, G))
+    for a in range(quad2["weight"].shape[0]):
+        omega = quad2["omega"][a]
+        w = quad2["weight"][a]
+        tpsi = np.zeros((Cn, G))  # incident boundary fluxes are zero (vacuum)
+        pinc = np.zeros((Cn, G))
+        # nextZ order: every zone after the zones it depends on
+        order = [abs(int(z)) - 1 for z in sched2["nextZ"][a]]
+        nextC = sched2["nextC"][a]
+        for z in order:
+            sig = sigt_g[z]
+            c0, nC, afp, aez, cez, _, ez_exit, coef = _zone_faces(mesh, geom, z, omega, par_tol)
+            Q = tsa[c0:c0 + nC] / sig
+            src = V[c0:c0 + nC, None] * tsa[c0:c0 + nC]
+            denom = np.zeros((nC, G))
+            psifp = {}
+            for c in range(nC):
+                sigv = V[c0 + c] * sig
+                denom[c] = sigv
+                for f in range(len(afp[c])):
+                    cfp = int(mesh["cFP"][c0 + c, f]) - 1
+                    if afp[c][f] > 0.0:
+                        denom[c] += afp[c][f]
+                    elif afp[c][f] < 0.0:
+                        psifp[(c, f)] = tpsi[cfp] if cfp < Cn else np.zeros(G)
+                        src[c] -= afp[c][f] * psifp[(c, f)]
+                        pinc[c0 + c] -= afp[c][f] * psifp[(c, f)]
+                for f in range(len(aez[c])):
+                    a_ez = aez[c][f]
+                    if a_ez <= 0.0:
+                        continue
+                    e = cez[c][f]
+                    denom[c] += a_ez
+                    area, inc = _area_opp(afp[c], f)
+                    if area > 0.0:
+                        psi_opp = sum(-afp[c][i] * psifp[(c, i)] for i in inc) / area
+                        B1, B2, gtau = _sez_coefficients(a_ez, sigv, sig, area)
+                        sez = gtau * sigv * (psi_opp - Q[c]) + 0.5 * a_ez * (1.0 - gtau) * (Q[c] - Q[e])
+                        src[c] += sez
+                        src[e] -= sez
+                        pinc[c0 + c] += gtau * sigv * psi_opp
+                        pinc[c0 + e] -= gtau * sigv * psi_opp
+                    else:
+                        sez = 0.5 * a_ez * (Q[c] - Q[e])
+                        src[c] += sez
+                        src[e] -= sez
+            for i in range(nC):
+                c = int(nextC[c0 + i]) - 1
+                tpsi[c0 + c] = src[c] / denom[c]
+                pinc[c0 + c] = pinc[c0 + c] / denom[c]
+                for e, k in zip(ez_exit[c], coef[c]):
+                    src[e] += k * tpsi[c0 + c]
+                    pinc[c0 + e] += k * pinc[c0 + c]
+        phi += w * tpsi
+        phi_inc += w * pinc
+    return phi, phi_inc
+
+
+def scalar_intensity_decompose(TT, sigs_g):
+    """In-place LU factors (Doolittle, no pivoting) of each zone's
+    A = I - TT wtiso sigma_s, A[z, g, c, s] = delta - wtiso sigma_s(s) TT[z, g, s, c]
+    (UpdateScalarIntensity.F90:50-97). Returns LU (Z, G, 8, 8) holding the
+    unit lower factor below and the upper factor on and above the diagonal."""
+    A = -WTISO * sigs_g[:, :, None, None] * np.swapaxes(TT, 2, 3)
+    n = A.shape[-1]
+    A = A + np.eye(n)
+    for i in range(n):
+        t = np.zeros(A.shape[:2])
+        for k in range(i):
+            t = t + A[:, :, i, k] * A[:, :, k, i]
+        A[:, :, i, i] = A[:, :, i, i] - t
+        dinv = 1.0 / A[:, :, i, i]
+        for j in range(i + 1, n):
+            t = np.zeros(A.shape[:2])
+            v = np.zeros(A.shape[:2])
+            for k in range(i):
+                t = t + A[:, :, i, k] * A[:, :, k, j]
+                v = v + A[:, :, j, k] * A[:, :, k, i]
+            A[:, :, i, j] = A[:, :, i, j] - t
+            A[:, :, j, i] = dinv * (A[:, :, j, i] - v)
+    return A
+
+
+def scalar_intensity_solve(LU, rhs):
+    """Forward and back substitution with the factors (ScalarIntensitySolve,
+    UpdateScalarIntensity.F90:113-170); rhs (Z, G, 8)."""
+    x = np.array(rhs, dtype=np.float64)
+    n = x.shape[-1]
+    for j in range(1, n):
+        t = np.zeros(x.shape[:2])
+        for i in range(j):
+            t = t - LU[:, :, j, i] * x[:, :, i]
+        x[:, :, j] = x[:, :, j] + t
+    x[:, :, n - 1] = x[:, :, n - 1] / LU[:, :, n - 1, n - 1]
+    for k in range(n - 2, -1, -1):
+        t = np.zeros(x.shape[:2])
+        for i in range(k + 1, n):
+            t = t + x[:, :, i] * LU[:, :, k, i]
+        x[:, :, k] = (x[:, :, k] - t) / LU[:, :, k, k]
+    return x
+
+
+def _zone_vec(x, G):
+    """(C, G) corner array -> (Z, G, 8)."""
+    return np.swapaxes(x.reshape(-1, 8, G), 1, 2)
+
+
+def _corner_vec(x):
+    """(Z, G, 8) -> (C, G)."""
+    return np.ascontiguousarray(np.swapaxes(x, 1, 2)).reshape(-1, x.shape[1])
+
+
+class NewGreySolver(GreySolver):
+    """GTA with the reference's new solver: the same BiCGStab (solve()),
+    whose operator M(P) is one grey sweep giving PhiInc followed by each
+    zone's direct solve (I - TT wtiso sigma_s) P = PhiInc [+ TT wtiso gsrc]
+    with LU factors made by the with-source call (GreySweepNEW)."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        self.sigs_g = self.sigs_c[::8]
+        self.TT = None
+        self.LU = None
+
+    def build(self):
+        self.TT = transfer_matrices(self.m, self.g, self.q2, self.sigt_g)
+        self.LU = None
+
+    def grey_sweep(self, P, gsrc=None):
+        G = self.G
+        tsa = WTISO * (self.sigs_c * P + (gsrc if gsrc is not None else 0.0))
+        _, phi_inc = pinc_sweep(self.m, self.g, self.q2, self.s2, self.sigt_g, tsa)
+        self.sweeps += 1
+        rhs = _zone_vec(phi_inc, G)
+        if gsrc is not None:
+            if self.TT is None:
+                self.build()
+            # PhiInc + TT wtiso GreySource, then decompose (UpdateScalarIntensity.F90:53-62)
+            rhs = rhs + np.einsum("zgsc,zgs->zgc", self.TT, WTISO * _zone_vec(gsrc, G))
+            self.LU = scalar_intensity_decompose(self.TT, self.sigs_g)
+        return _corner_vec(scalar_intensity_solve(self.LU, rhs))
+
+    def solve(self, gsrc, phi_half, eps_grey, max_sweeps=21, eps_point=1.0e-6):
+        self.build()  # rt/GTASolver.F90:158-178: matrices built for every solve
+        return super().solve(gsrc, phi_half, eps_grey, max_sweeps, eps_point)

@@ -83,3 +83,98 @@ def test_accelerated_iteration_reaches_the_plain_fixed_point(prob):
     assert res < 1e-11 and rres < 1e-13
     assert it * 10 < rit
     np.testing.assert_allclose(phi, ref, rtol=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# The transfer-matrix ("new") solver of the oracle, pinned independently of
+# its own code: the matrices by unit-source sweeps of the oracle's C
+# restatement of SweepUCBxyz, the incident-flux sweep by the same C sweep
+# and by linearity, the LU factors by their residual, and the solver by the
+# grey problem's solution that two other methods reach.
+
+
+@pytest.fixture(scope="module")
+def tile():
+    p = U.TiledProblem(1, 1, 1, 2, 2)
+    m, g = p.mesh(), p.geometry()
+    gq = p.gta()
+    s2 = O.schedule(m, g["A_fp"], g["A_ez"], gq["omega"], par_tol=1e-12)
+    q2 = dict(omega=gq["omega"], weight=gq["weight"])
+    G = 2
+    st, ss, _ = U.draw_cross_sections(3, p.nzones, G, (1e-2, 1e3), (0.5, 0.99), (0, 0))
+    return p, q2, s2, st, ss
+
+
+def test_transfer_matrices_equal_unit_source_sweeps(tile):
+    """T[z, :, s, :] is zone z's scalar flux from a unit source at its corner
+    s alone (no other source anywhere, so no incident flux reaches z)."""
+    p, q2, s2, st, ss = tile
+    sig_g, _, _, _ = OG.grey_opacities(st, ss)
+    TT = OG.transfer_matrices(p.mesh(), p.geometry(), q2, sig_g)
+    G = st.shape[1]
+    for z in range(p.nzones):
+        for s in range(8):
+            S = np.zeros((p.ncorner, G))
+            S[8 * z + s] = 1.0
+            phi, _ = O.sweep_all(p.mesh(), p.geometry(), q2, s2, S, np.ascontiguousarray(sig_g),
+                                 par_tol=1e-12)
+            ref = phi[8 * z:8 * z + 8].T
+            assert np.max(np.abs(TT[z, :, s, :] - ref)) < 1e-12 * np.abs(ref).max(), (z, s)
+
+
+def test_incident_flux_sweep_is_the_sweep_minus_the_local_response(tile):
+    p, q2, s2, st, ss = tile
+    sig_g, _, _, _ = OG.grey_opacities(st, ss)
+    G = st.shape[1]
+    rng = np.random.default_rng(1)
+    tsa = rng.uniform(0.0, 1.0, (p.ncorner, G))
+    phi, phi_inc = OG.pinc_sweep(p.mesh(), p.geometry(), q2, s2, sig_g, tsa)
+    ref, _ = O.sweep_all(p.mesh(), p.geometry(), q2, s2, tsa, np.ascontiguousarray(sig_g),
+                         par_tol=1e-12)
+    np.testing.assert_allclose(phi, ref, rtol=1e-13)
+    TT = OG.transfer_matrices(p.mesh(), p.geometry(), q2, sig_g)
+    local = OG._corner_vec(np.einsum("zgsc,zgs->zgc", TT, OG._zone_vec(tsa, G)))
+    assert np.max(np.abs(phi - local - phi_inc)) < 1e-14 * np.abs(phi).max()
+
+
+def test_zone_lu_factors_solve_the_zone_systems(tile):
+    p, q2, s2, st, ss = tile
+    sig_g, scat_g, _, _ = OG.grey_opacities(st, ss)
+    TT = OG.transfer_matrices(p.mesh(), p.geometry(), q2, sig_g)
+    LU = OG.scalar_intensity_decompose(TT, scat_g)
+    A = np.eye(8) - OG.WTISO * scat_g[:, :, None, None] * np.swapaxes(TT, 2, 3)
+    L = np.tril(LU, -1) + np.eye(8)
+    Uu = np.triu(LU)
+    assert np.max(np.abs(L @ Uu - A)) < 1e-14
+    rng = np.random.default_rng(2)
+    b = rng.uniform(0.0, 1.0, (p.nzones, st.shape[1], 8))
+    x = OG.scalar_intensity_solve(LU, b)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b[..., None])[..., 0], rtol=1e-12)
+
+
+def test_new_solver_reaches_the_grey_solution(tile):
+    """Kernel 2 and kernel 1 of the grey solve give the grey problem's
+    solution, which plain grey source iteration reaches to 1e-14, and
+    kernel 2 uses far fewer grey sweeps."""
+    p, q2, s2, st, ss = tile
+    G = st.shape[1]
+    rng = np.random.default_rng(4)
+    gsrc = rng.uniform(0.0, 1.0, (p.ncorner, G))
+    half = rng.uniform(1.0, 2.0, (p.ncorner, G))
+    old = OG.GreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    new = OG.NewGreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    Fo, _ = old.solve(gsrc, half, 1e-11)
+    Fn, _ = new.solve(gsrc, half, 1e-11)
+    sig_g, scat_g, _, _ = OG.grey_opacities(st, ss)
+    sc = np.repeat(scat_g, 8, axis=0)
+    F = np.zeros_like(gsrc)
+    for _ in range(20000):
+        Fx, _ = O.sweep_all(p.mesh(), p.geometry(), q2, s2, OG.WTISO * (sc * F + gsrc),
+                            np.ascontiguousarray(sig_g), par_tol=1e-12)
+        done = np.max(np.abs(Fx - F)) < 1e-15 * np.abs(Fx).max()
+        F = Fx
+        if done:
+            break
+    assert np.max(np.abs(Fn - F)) < 1e-9 * np.abs(F).max()
+    assert np.max(np.abs(Fo - F)) < 1e-9 * np.abs(F).max()
+    assert 2 * new.sweeps < old.sweeps
