@@ -205,6 +205,14 @@ int umt_ctx_source_iteration(umt_ctx *ctx, double tol, int maxIters, int *iters,
  * source from the current phi, one sweep, Psi = its psi, phi = its phi.
  * Psi arrays on the host are Teton's Psi(G, C, A). GTA accounts for tau in
  * its grey total cross section. */
+/* Incident boundary fluxes for the context's S_N sweeps: PsiB(G, nbelem, A)
+ * as Teton's PsiB, loaded into each angle's boundary tail before its levels
+ * (snac/SweepUCBxyz.F90:95-97, control/setBoundarySources.F90); NULL returns
+ * to vacuum boundaries (the default). umt_ctx_get_boundary_flux gives
+ * PsiB(G, nbelem, A) after a sweep: exiting fluxes in outgoing slots,
+ * incident ones in the others (needs one angle block). */
+int umt_ctx_set_incident_flux(umt_ctx *ctx, const double *PsiB);
+int umt_ctx_get_boundary_flux(const umt_ctx *ctx, double *PsiB);
 int umt_ctx_set_tau(umt_ctx *ctx, double tau);
 int umt_ctx_set_psi_prev(umt_ctx *ctx, const double *Psi);
 int umt_ctx_get_psi_prev(const umt_ctx *ctx, double *Psi);
@@ -253,6 +261,23 @@ int umt_ctx_grey_sweep(umt_ctx *ctx, const double *P, const double *gsrc, double
  * BiCGStab iterations it took. */
 int umt_ctx_grey_solve(umt_ctx *ctx, const double *gsrc, const double *phiHalf, double epsGrey,
                        double *F, int64_t *greyIterations);
+/* GTA solver kernel, numbered as the reference driver's option
+ * (driver/test_driver.cc:1520-1531): 1 = grey S2 sweep followed by a
+ * stretched-TSA sweep per operator application (rt/GreySweep.F90:50-110,
+ * the default, as in the reference); 2 = the transfer-matrix solver
+ * (Size%useNewGTASolver): per-zone corner transfer matrices
+ * (snac/InitSweepGreyUCBxyz.F90:10-253) and each zone's direct solve of its
+ * grey scattering problem with in-place LU factors
+ * (snac/UpdateScalarIntensity.F90:11-170) after one grey sweep
+ * (rt/GreySweep.F90:12-47), the factors made once per set of opacities. */
+int umt_ctx_set_gta_solver(umt_ctx *ctx, int kernel);
+/* Kernel 2's matrices for checking: T(g, c, s) (Fortran order, G x C x 8)
+ * is the scalar flux at corner c (1..C) from a unit per-steradian source at
+ * local corner s (1..8) of the same zone, summed over the grey directions
+ * (GTA%TT(s, c), one per group); LU (same shape, may be NULL) holds, for
+ * element (row c, column s) of each zone's matrix I - wtiso sigma_s T, its
+ * Doolittle factors (unit lower below the diagonal, upper on and above). */
+int umt_ctx_grey_transfer(umt_ctx *ctx, double *T, double *LU);
 
 /* Groups each lane of the sweep kernels carries (K = 1, 2 or 4, dividing
  * the group count; default 1). No counterpart in the reference. */
@@ -290,6 +315,17 @@ int umt_ctx_sweep_n(umt_ctx *ctx, int n);
  * the algorithmic-bytes model (Omega.A_fp < 0 towards a corner, not a
  * boundary element), counted over all corners and angles. */
 int umt_ctx_incoming_faces(const umt_ctx *ctx, double *n_in);
+
+/* Level structure of the S_N sweep as launched: stats[0] level launches per
+ * sweep, stats[1] (zone, angle) items per sweep, stats[2] distinct (level,
+ * zone) pairs (a zone met by several directions in one level launch counts
+ * once: those directions share its source, cross section and geometry
+ * reads), stats[3] angle blocks. No counterpart in the reference. */
+int umt_ctx_level_stats(const umt_ctx *ctx, int64_t *stats);
+
+/* Free and total device memory of the current GPU in bytes (hipMemGetInfo),
+ * for sizing angle blocks. No counterpart in the reference. */
+int umt_device_memory_info(int64_t *freeBytes, int64_t *totalBytes);
 
 /* Calibration probe for HBM byte counters (no counterpart in the
  * reference): copies n doubles device-to-device reps times with a plain

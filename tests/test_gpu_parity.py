@@ -248,3 +248,65 @@ def test_distorted_hex_mesh_sweep_and_iteration_match_oracle():
     ref, _, _, _ = O.source_iteration(m, g, q, s, st, ss, qq, 0.0, 10)
     assert rel_err(ctx.phi(), ref) < 1e-12
     ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# Incident boundary fluxes in the resident all-angle sweep (the caller's
+# PsiB loaded into each angle's boundary tail, snac/SweepUCBxyz.F90:95-97):
+# phi and the exiting boundary fluxes against the oracle's sweep fed the
+# same PsiB, for one angle block and for several; a few source iterations
+# against the oracle's sweeps with the same incident fluxes.
+
+
+@pytest.mark.parametrize("precision,apb", [(64, 0), (64, 5), (32, 0)])
+def test_incident_boundary_flux_matches_oracle(small_problem, precision, apb):
+    p = small_problem
+    G = 3
+    print(f"case incident flux FP{precision} angles per block {apb or p.nangles}", flush=True)
+    st, ss, qq = U.draw_cross_sections(41, p.nzones, G, (0.1, 10.0), (0.0, 0.9), (0.0, 1.0))
+    S = np.repeat(qq, 8, axis=0) / (4 * np.pi)
+    rng = np.random.default_rng(42)
+    PsiB = rng.uniform(0.0, 2.0, (p.nangles, p.nbelem, G))
+    ctx = U.SweepContext(p, G, precision=precision)
+    if apb:
+        ctx.set_angles_per_block(apb)
+    ctx.set_sigt(st)
+    ctx.set_source(S)
+    ctx.set_incident_flux(PsiB)
+    ctx.sweep()
+    phi = ctx.phi()
+    ref_B = PsiB.copy()
+    ref, ref_B = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st,
+                             PsiB=ref_B)
+    tol = 1e-12 if precision == 64 else 2e-5
+    assert rel_err(phi, ref) < tol
+    if not apb:
+        got_B = ctx.boundary_flux()
+        assert np.max(np.abs(got_B - ref_B)) <= tol * np.abs(ref_B).max()
+    # back to vacuum: the plain sweep again
+    ctx.set_incident_flux(None)
+    ctx.sweep()
+    ref0, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st)
+    assert rel_err(ctx.phi(), ref0) < tol
+    ctx.close()
+
+
+def test_source_iteration_with_incident_flux_matches_oracle(small_problem):
+    p = small_problem
+    G = 2
+    st, ss, qq = U.draw_cross_sections(43, p.nzones, G, (0.1, 10.0), (0.0, 0.9), (0.0, 1.0))
+    rng = np.random.default_rng(44)
+    PsiB = rng.uniform(0.0, 1.0, (p.nangles, p.nbelem, G))
+    ctx = U.SweepContext(p, G)
+    ctx.set_sigt(st)
+    ctx.set_scattering(ss, qq)
+    ctx.set_incident_flux(PsiB)
+    ctx.source_iteration(tol=0.0, max_iters=4)
+    phi = np.zeros((p.ncorner, G))
+    sc, qc = np.repeat(ss, 8, axis=0), np.repeat(qq, 8, axis=0)
+    for _ in range(4):
+        S = (sc * phi + qc) / (4 * np.pi)
+        phi, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st,
+                             PsiB=PsiB.copy())
+    assert rel_err(ctx.phi(), phi) < 1e-12
+    ctx.close()

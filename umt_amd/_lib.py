@@ -57,6 +57,8 @@ SIGNATURES = {
     "umt_ctx_dims": (_i, [_vp, _pi64]),
     "umt_ctx_sweep_n": (_i, [_vp, _i]),
     "umt_ctx_incoming_faces": (_i, [_vp, _pd]),
+    "umt_ctx_level_stats": (_i, [_vp, C.POINTER(C.c_int64)]),
+    "umt_device_memory_info": (_i, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "umt_hbm_copy_probe": (_i, [_i64, _i, _pd]),
     "umt_ctx_set_groups_per_lane": (_i, [_vp, _i]),
     "umt_ctx_get_groups_per_lane": (_i, [_vp, _pi]),
@@ -78,6 +80,10 @@ SIGNATURES = {
     "umt_ctx_grey_sweep": (_i, [_vp, _pd, _pd, _pd]),
     "umt_ctx_autotune_grey": (_i, [_vp, _i, _pi, _pd]),
     "umt_ctx_grey_solve": (_i, [_vp, _pd, _pd, _d, _pd, _pi64]),
+    "umt_ctx_set_gta_solver": (_i, [_vp, _i]),
+    "umt_ctx_set_incident_flux": (_i, [_vp, _pd]),
+    "umt_ctx_get_boundary_flux": (_i, [_vp, _pd]),
+    "umt_ctx_grey_transfer": (_i, [_vp, _pd, _pd]),
     "umt_batch_create": (_i, [_i, C.POINTER(_vp), _pi, _i, C.POINTER(_vp)]),
     "umt_batch_destroy": (None, [_vp]),
     "umt_batch_dims": (_i, [_vp, _pi64]),

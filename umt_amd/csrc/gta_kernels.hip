@@ -200,6 +200,114 @@ __global__ __launch_bounds__(256) void gta_reduce_error(const double* __restrict
 }
 
 // ----------------------------------------------------------------------------
+// Transfer-matrix solver (the reference's "new" GTA: per-zone corner transfer
+// matrices, snac/InitSweepGreyUCBxyz.F90:10-253, and each zone's direct
+// solve with in-place LU factors, snac/UpdateScalarIntensity.F90:11-170).
+//
+// T(s -> c) of a zone and group is the scalar flux at corner c from a unit
+// per-steradian source at corner s of the same zone with no incident flux,
+// summed over the grey directions. Column s of every zone comes from one
+// sweep of the grey directions with "no inflow" (every zone on its own, so
+// all (zone, direction) items go in one launch) of the unit source at local
+// corner s. T is stored as 8 arrays (G, C): T(s -> c) of zone z, group g at
+// s * C * G + (8 z + c) * G + g; the LU factors replace T element by element.
+
+// S = 1 at local corner s of every zone and group, 0 elsewhere.
+template <typename Real>
+__global__ __launch_bounds__(256) void gta_unit_source(int s, int G, long n, Real* __restrict__ S) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long c = i / G;
+  S[i] = (Real)((c & 7) == s ? 1 : 0);
+}
+
+// A = I - wtiso sigma_s T per (zone, group), A(c, s) = delta_cs - wtiso
+// sigma_s T(s -> c) (the zone's sigma_s is the same at all its corners),
+// factored in place as A = L U (Doolittle, no pivoting: unit lower L below
+// the diagonal, U on and above it), in the reference's loop order
+// (UpdateScalarIntensity.F90:64-97). Element (c, s) lives where T(s -> c)
+// was. One thread per (zone, group).
+__global__ __launch_bounds__(128) void gta_zone_lu(double* __restrict__ T,
+                                                   const double* __restrict__ sigs, int G,
+                                                   long nzg, long CG) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nzg) return;
+  const long z = t / G;
+  const int g = (int)(t - z * G);
+  double* base = T + (8 * z) * G + g;
+  const double ws = kWtiso * sigs[t];
+  double a[8][8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a[c][s] = (c == s ? 1.0 : 0.0) - ws * base[s * CG + (long)c * G];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    double u = 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) u += a[i][k] * a[k][i];
+    a[i][i] -= u;
+    const double dinv = 1.0 / a[i][i];
+#pragma unroll
+    for (int j = i + 1; j < 8; ++j) {
+      double r = 0.0, v = 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) {
+        r += a[i][k] * a[k][j];
+        v += a[j][k] * a[k][i];
+      }
+      a[i][j] -= r;
+      a[j][i] = dinv * (a[j][i] - v);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) base[s * CG + (long)c * G] = a[c][s];
+}
+
+// out = P + (L U)^-1 (phi - P) per (zone, group): the new solver's grey
+// operator (P null stands for zero). phi is the grey sweep's scalar flux with
+// source wtiso (sigma_s P [+ gsrc]); P + (I - B)^-1 (phi - P), B = wtiso
+// T sigma_s, equals the reference's (I - B)^-1 (PhiInc [+ T wtiso gsrc])
+// because the sweep is linear: PhiInc = phi - T wtiso (sigma_s P [+ gsrc]).
+// Forward and back substitution as ScalarIntensitySolve
+// (UpdateScalarIntensity.F90:136-167). out may alias phi.
+__global__ __launch_bounds__(256) void gta_zone_solve(const double* __restrict__ LU,
+                                                      const double* P, const double* phi, int G,
+                                                      long nzg, long CG, double* out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nzg) return;
+  const long z = t / G;
+  const int g = (int)(t - z * G);
+  const double* base = LU + (8 * z) * G + g;
+  const long k0 = (8 * z) * G + g;
+  double p[8], x[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    p[c] = P ? P[k0 + (long)c * G] : 0.0;
+    x[c] = phi[k0 + (long)c * G] - p[c];
+  }
+#pragma unroll
+  for (int j = 1; j < 8; ++j) {
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < j; ++i) r -= base[i * CG + (long)j * G] * x[i];
+    x[j] += r;
+  }
+  x[7] = x[7] / base[7 * CG + 7L * G];
+#pragma unroll
+  for (int k = 6; k >= 0; --k) {
+    double r = 0.0;
+#pragma unroll
+    for (int i = k + 1; i < 8; ++i) r += x[i] * base[i * CG + (long)k * G];
+    x[k] = (x[k] - r) / base[k * CG + (long)k * G];
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) out[k0 + (long)c * G] = p[c] + x[c];
+}
+
+// ----------------------------------------------------------------------------
 
 hipError_t launch_gta_grey_source(const double* phiHalf, const double* phiPrev, const double* sigs,
                                   int G, long n, double* gsrc, hipStream_t s) {
@@ -244,6 +352,28 @@ hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const d
   hipLaunchKernelGGL(gta_reduce_error, dim3(1), dim3(256), 0, s, partial, nblocks, out);
   return hipGetLastError();
 }
+
+template <typename Real>
+hipError_t launch_gta_unit_source(int s, int G, long n, Real* S, hipStream_t st) {
+  hipLaunchKernelGGL((gta_unit_source<Real>), dim3(blocks_for(n, 256)), dim3(256), 0, st, s, G, n, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_gta_zone_lu(double* T, const double* sigs, int G, long nzg, long CG,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(gta_zone_lu, dim3(blocks_for(nzg, 128)), dim3(128), 0, s, T, sigs, G, nzg, CG);
+  return hipGetLastError();
+}
+
+hipError_t launch_gta_zone_solve(const double* LU, const double* P, const double* phi, int G,
+                                 long nzg, long CG, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(gta_zone_solve, dim3(blocks_for(nzg, 256)), dim3(256), 0, s, LU, P, phi, G,
+                     nzg, CG, out);
+  return hipGetLastError();
+}
+
+template hipError_t launch_gta_unit_source<double>(int, int, long, double*, hipStream_t);
+template hipError_t launch_gta_unit_source<float>(int, int, long, float*, hipStream_t);
 
 template hipError_t launch_gta_sweep_source<double>(int, const double*, const double*,
                                                     const double*, const double*, int, long,

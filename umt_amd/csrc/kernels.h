@@ -46,6 +46,9 @@ struct SweepLevelArgs {
   long psiAngleStride;      // (C + nbelem) * G
   int angle0;               // first angle held in psi (angle blocks)
   Real parTol;              // corner faces with |Omega.A| <= parTol |A|_1 are parallel (0: exact)
+  int noInflow;             // every zone on its own: incident FP fluxes are zero
+  int boundaryIn;           // all-angle sweeps: incoming boundary faces read psi's
+                            // boundary tail (incident flux), else they bring zero
   // Single-angle (drop-in) extras.
   double* phi;              // (G, C), phi += quadwt * psi
   double quadwt;
@@ -122,6 +125,14 @@ hipError_t launch_gta_sweep_source(int kind, const double* P, const double* othe
                                    hipStream_t s);
 hipError_t launch_gta_lincomb(const double* x, const double* y, const double* z,
                               const double* coef, int G, long n, double* out, hipStream_t s);
+// Transfer-matrix GTA solver: unit sources, per-zone LU factors of
+// I - wtiso sigma_s T, and the operator P + (LU)^-1 (phi - P).
+template <typename Real>
+hipError_t launch_gta_unit_source(int s, int G, long n, Real* S, hipStream_t st);
+hipError_t launch_gta_zone_lu(double* T, const double* sigs, int G, long nzg, long CG,
+                              hipStream_t s);
+hipError_t launch_gta_zone_solve(const double* LU, const double* P, const double* phi, int G,
+                                 long nzg, long CG, double* out, hipStream_t s);
 int gta_dot_block(int G);
 hipError_t launch_gta_wdot(const double* x, const double* y, const double* sigs, const double* vol,
                            int G, long n, int nblocks, double* partial, double* out, hipStream_t s);
@@ -133,6 +144,11 @@ hipError_t launch_convert(const Src* s, Dst* d, long n, hipStream_t s_);
 // Cycle-list psi: entries (corner, angle) first .. first + n - 1; toPsi loads
 // cyclePsi (G, entries) into psi (G, C + nbelem, angles from angle0), else
 // stores psi into cyclePsi.
+// Incident boundary fluxes of angles a0 .. a0+nA-1 into the boundary tails
+// of their psi blocks: psi[(a - a0) angleStride + C G + i] = inc[a nb G + i].
+template <typename Real>
+hipError_t launch_boundary_in(const Real* inc, long nbG, long CG, int a0, int nA, long angleStride,
+                              Real* psi, hipStream_t s);
 template <typename Real>
 hipError_t launch_cycle_copy(const int2* entries, long first, long n, int G, Real* cyclePsi,
                              Real* psi, long angleStride, int angle0, bool toPsi, hipStream_t s);

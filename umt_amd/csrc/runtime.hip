@@ -98,6 +98,7 @@ struct SweepSet {
   double* weight = nullptr;  // (A)
   int2* items = nullptr;     // (zone, angle), block by block, level by level
   std::vector<int> levelOffset;  // item offsets of the levels, all blocks
+  long zoneVisits = 0;           // distinct (level, zone) pairs over all levels
   std::vector<int> blockLevel;   // first level of each block (+ end)
   std::vector<int2> allItems;    // the unblocked levels (plane h of every angle)
   std::vector<int> allLevelOffset;
@@ -138,6 +139,13 @@ struct GtaState {
   double *gsrc = nullptr, *phiHalf = nullptr, *pzOld = nullptr;
   double *partial = nullptr, *dots = nullptr, *coef = nullptr, *err = nullptr;
   int dotBlocks = 0, errBlocks = 0;
+  // Solver kernel, numbered as the reference driver's GTA kernel option
+  // (driver/test_driver.cc:1520-1531): 1 grey + stretched-TSA sweeps, 2 the
+  // transfer-matrix solver (Size%useNewGTASolver).
+  int solver = 1;
+  double* lu = nullptr;       // kernel 2: T, then its LU factors, 8 (G, C) arrays
+  bool luValid = false;       // lu holds the factors of the current opacities
+  int2* localItems = nullptr; // kernel 2: every (zone, grey direction), zone by zone
   int maxSweeps = 21;         // grey_max_sweeps (mods/default_iter_controls_mod.F90)
   double epsPoint = 1.0e-6;   // grey_reltol
   double epsGreyInit = 1.0e-5;  // rt/rtmainsn.F90:129
@@ -225,6 +233,7 @@ struct umt_ctx {
   double nIn = 0.0;       // mean incoming interior FP faces per (corner, angle)
   double tau = 0.0;       // 1/(c dt): time-dependent sweeps when > 0
   void* psiPrev = nullptr;  // previous cycle's psi, (G, C + nbelem, A), sweep precision
+  void* psiInc = nullptr;   // incident boundary psi, (G, nbelem, A), sweep precision (null: vacuum)
   std::vector<int> cFPHost;  // (3, C) corner-face neighbours, for splitting cyclic planes
   GtaState gta;
   LearnState learn;
@@ -256,6 +265,7 @@ umt::SweepLevelArgs<Real> level_args(const umt_ctx* x, const SweepSet& set, cons
   a.psiAngleStride = x->psiStride();
   a.angle0 = angle0;
   a.parTol = (Real)set.parTol;
+  a.boundaryIn = (&set == &x->sn && x->psiInc) ? 1 : 0;
   if (&set == &x->sn && x->tau > 0.0) {  // tau Psi source of the S_N sweep only
     a.tauPsi = (const Real*)x->psiPrev;
     a.tau = (Real)x->tau;
@@ -279,6 +289,11 @@ int launch_sweep(umt_ctx* x, SweepSet& set, const void* S, const void* sigt, dou
       c0 = set.cycleOffset[a0];
       nc = set.cycleOffset[a0 + nA] - c0;
     }
+    // Incident boundary fluxes of the block's angles (SweepUCBxyz.F90:95-97).
+    if (&set == &x->sn && x->psiInc)
+      HIP_TRY(umt::launch_boundary_in<Real>((const Real*)x->psiInc, (long)x->nb * x->G,
+                                            (long)x->C * x->G, a0, nA, x->psiStride(),
+                                            (Real*)set.psi, x->stream));
     // initFromCycleList for the block's angles (SetSweep.F90:149)
     HIP_TRY(umt::launch_cycle_copy<Real>(set.cycleEntries, c0, nc, x->G, (Real*)set.cyclePsi,
                                          (Real*)set.psi, x->psiStride(), a0, true, x->stream));
@@ -599,7 +614,8 @@ static void free_set(SweepSet& set) {
 static void free_gta(GtaState& g) {
   dfree(g.sigt2); dfree(g.eps); dfree(g.S2); dfree(g.F); dfree(g.r); dfree(g.d); dfree(g.Ad);
   dfree(g.As); dfree(g.P1); dfree(g.gsrc); dfree(g.phiHalf); dfree(g.pzOld); dfree(g.partial);
-  dfree(g.dots); dfree(g.coef); dfree(g.err);
+  dfree(g.dots); dfree(g.coef); dfree(g.err); dfree(g.lu); dfree(g.localItems);
+  g.luValid = false;
   g.enabled = false;
 }
 
@@ -627,6 +643,7 @@ void umt_ctx_destroy(umt_ctx* x) {
   drop_graphs(x->s2);
   free_learn(x->learn);
   dfree(x->psiPrev);
+  dfree(x->psiInc);
   dfree(x->Afp); dfree(x->Aez); dfree(x->vol); dfree(x->cFP); dfree(x->vol64); dfree(x->volZ);
   dfree(x->S); dfree(x->sigt); dfree(x->sigs); dfree(x->q); dfree(x->phi); dfree(x->phiPrev);
   dfree(x->phibar); dfree(x->phibarOld); dfree(x->bits);
@@ -707,11 +724,17 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
   // hyperplanes) sit side by side, so their reads of the zone's source,
   // cross section and corner geometry hit the same cache lines. Items of a
   // level are independent, so the order changes no result.
-  for (size_t h = 0; h + 1 < lo.size(); ++h)
+  long zoneVisits = 0;
+  for (size_t h = 0; h + 1 < lo.size(); ++h) {
     std::sort(items.begin() + lo[h], items.begin() + lo[h + 1], [](int2 p, int2 q) {
       const int zp = p.x < 0 ? ~p.x : p.x, zq = q.x < 0 ? ~q.x : q.x;
       return zp != zq ? zp < zq : p.y < q.y;
     });
+    for (int i = lo[h]; i < lo[h + 1]; ++i) {
+      const int z = items[i].x < 0 ? ~items[i].x : items[i].x;
+      if (i == lo[h] || z != (items[i - 1].x < 0 ? ~items[i - 1].x : items[i - 1].x)) ++zoneVisits;
+    }
+  }
   // XCD-aware order: workgroup w of a level launch runs on XCD w mod 8, so
   // slot w takes the next workgroup's worth of items from zone range w mod 8
   // (eight equal ranges of zone ids), or from the fullest range once its own
@@ -769,6 +792,7 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
   else
     HIP_TRY(umt::launch_fill_zero<double>((double*)set.psi, (long)nPsi, x->stream));
   set.anglesPerBlock = n;
+  set.zoneVisits = zoneVisits;
   set.levelOffset = lo;
   set.blockLevel = bl;
   set.evLevel.resize(2 * (size_t)set.levels());
@@ -1017,6 +1041,44 @@ int umt_ctx_set_tau(umt_ctx* x, double tau) {
   return 0;
 }
 
+int umt_ctx_set_incident_flux(umt_ctx* x, const double* PsiB) {
+  if (!x) return fail(1, "null context");
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  drop_graphs(x->sn);  // captured sweeps hold no boundary copies
+  if (!PsiB) {
+    dfree(x->psiInc);
+    x->psiInc = nullptr;
+    return 0;
+  }
+  const size_t n = (size_t)x->nb * x->G * x->sn.A;
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(PsiB[i])) return fail(56, "PsiB entry %zu is not finite", i + 1);
+  if (n == 0) return 0;
+  if (!x->psiInc)
+    if (int rc = dalloc((char**)&x->psiInc, n * x->rsize())) return rc;
+  return upload_real(x, x->psiInc, PsiB, n);
+}
+
+int umt_ctx_get_boundary_flux(const umt_ctx* x, double* PsiB) {
+  if (!x || !PsiB) return fail(1, "null argument");
+  if (x->sn.blocks() != 1)
+    return fail(45, "boundary fluxes of every angle are held only with one angle block");
+  const size_t nbG = (size_t)x->nb * x->G, CG = (size_t)x->C * x->G;
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  for (int a = 0; a < x->sn.A; ++a) {
+    const size_t off = (size_t)a * x->psiStride() + CG;
+    if (x->precision == 64) {
+      HIP_TRY(hipMemcpy(PsiB + a * nbG, (const double*)x->sn.psi + off, nbG * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> f(nbG);
+      HIP_TRY(hipMemcpy(f.data(), (const float*)x->sn.psi + off, nbG * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < nbG; ++i) PsiB[a * nbG + i] = f[i];
+    }
+  }
+  return 0;
+}
+
 int umt_ctx_set_psi_prev(umt_ctx* x, const double* psi) {
   if (!x || !psi) return fail(1, "null argument");
   if (!x->psiPrev) return fail(49, "umt_ctx_set_tau with tau > 0 first");
@@ -1233,6 +1295,23 @@ int umt_ctx_incoming_faces(const umt_ctx* x, double* n_in) {
   return 0;
 }
 
+int umt_device_memory_info(int64_t* freeBytes, int64_t* totalBytes) {
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  if (freeBytes) *freeBytes = (int64_t)f;
+  if (totalBytes) *totalBytes = (int64_t)t;
+  return 0;
+}
+
+int umt_ctx_level_stats(const umt_ctx* x, int64_t* stats) {
+  if (!x || !stats) return fail(1, "null argument");
+  stats[0] = x->sn.levels();
+  stats[1] = x->sn.levelOffset.back();
+  stats[2] = x->sn.zoneVisits;
+  stats[3] = x->sn.blocks();
+  return 0;
+}
+
 int umt_hbm_copy_probe(int64_t n, int reps, double* ms) {
   if (n < 1 || reps < 1 || !ms) return fail(1, "bad argument");
   double *a = nullptr, *b = nullptr;
@@ -1337,6 +1416,13 @@ int umt_ctx_set_acceleration(umt_ctx* x, int mode) {
   return 0;
 }
 
+int umt_ctx_set_gta_solver(umt_ctx* x, int kernel) {
+  if (!x) return fail(1, "null context");
+  if (kernel != 1 && kernel != 2) return fail(55, "GTA kernel must be 1 (grey + stretched TSA) or 2 (transfer matrices)");
+  x->gta.solver = kernel;
+  return 0;
+}
+
 int umt_ctx_gta_stats(const umt_ctx* x, int64_t* greyIterations, int64_t* greySweeps) {
   if (!x) return fail(1, "null context");
   if (greyIterations) *greyIterations = x->gta.greyIterations;
@@ -1376,7 +1462,58 @@ int gta_prepare(umt_ctx* x) {
   if (int rc = upload_real(x, g.sigt2, st.data(), 2 * GZ)) return rc;
   HIP_TRY(hipMemcpyAsync(g.eps, ep.data(), 2 * GZ * sizeof(double), hipMemcpyHostToDevice, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
+  g.luValid = false;  // the transfer matrices follow the opacities
   return 0;
+}
+
+// Transfer-matrix solver: T of every zone and group from 8 sweeps of the grey
+// directions with no inflow (one per source corner), then the LU factors of
+// I - wtiso sigma_s T in place (snac/InitSweepGreyUCBxyz.F90,
+// snac/UpdateScalarIntensity.F90:50-97). Tout (may be null) receives T
+// before it is factored. The factors stay valid until the opacities change.
+template <typename Real>
+int gta_build_lu_impl(umt_ctx* x, double* Tout) {
+  GtaState& g = x->gta;
+  SweepSet& s2 = x->s2;
+  const long CG = (long)x->C * x->G, nzg = (long)x->Z * x->G;
+  const int A2 = s2.A;
+  if (s2.blocks() != 1) return fail(54, "the grey sweeps must hold all their directions in one block");
+  if (!g.lu)
+    if (int rc = dalloc(&g.lu, 8 * (size_t)CG)) return rc;
+  if (!g.localItems) {
+    std::vector<int2> it((size_t)x->Z * A2);
+    for (int z = 0; z < x->Z; ++z)
+      for (int a = 0; a < A2; ++a) it[(size_t)z * A2 + a] = make_int2(z, a);
+    if (int rc = dalloc(&g.localItems, it.size())) return rc;
+    HIP_TRY(hipMemcpyAsync(g.localItems, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  }
+  const void* sigGrey = g.sigt2;
+  for (int sc = 0; sc < 8; ++sc) {
+    HIP_TRY(umt::launch_gta_unit_source<Real>(sc, x->G, CG, (Real*)g.S2, x->stream));
+    umt::SweepLevelArgs<Real> a = level_args<Real>(x, s2, g.S2, sigGrey, 0, 0);
+    a.items = g.localItems;
+    a.nItems = x->Z * A2;
+    a.noInflow = 1;
+    HIP_TRY(umt::launch_sweep_level<Real>(a, false, x->stream));
+    HIP_TRY(umt::launch_phi_from_psi_acc<Real>((const Real*)s2.psi, s2.weight, A2, x->psiStride(), CG,
+                                               g.lu + sc * CG, false, x->stream));
+  }
+  if (Tout) {
+    HIP_TRY(hipMemcpyAsync(Tout, g.lu, 8 * (size_t)CG * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  }
+  const size_t GZ = (size_t)x->G * x->Z;
+  HIP_TRY(umt::launch_gta_zone_lu(g.lu, g.eps + GZ, x->G, nzg, CG, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  g.luValid = true;
+  return 0;
+}
+
+int gta_build_lu(umt_ctx* x, double* Tout = nullptr) {
+  if (x->gta.luValid && !Tout) return 0;
+  return x->precision == 32 ? gta_build_lu_impl<float>(x, Tout) : gta_build_lu_impl<double>(x, Tout);
 }
 
 template <typename Real>
@@ -1395,6 +1532,18 @@ int gta_source(umt_ctx* x, int kind, const double* P, const double* other) {
 int grey_sweep(umt_ctx* x, const double* P, bool withSource, double* out) {
   GtaState& g = x->gta;
   const size_t GZ = (size_t)x->G * x->Z;
+  if (g.solver == 2) {
+    // One grey sweep of wtiso (sigma_s P [+ gsrc]), then every zone's direct
+    // solve: out = P + (I - B)^-1 (phi - P) (rt/GreySweep.F90:12-47).
+    if (int rc = gta_build_lu(x)) return rc;
+    int rc2 = x->precision == 32 ? gta_source<float>(x, 1, P, withSource ? g.gsrc : nullptr)
+                                 : gta_source<double>(x, 1, P, withSource ? g.gsrc : nullptr);
+    if (rc2) return rc2;
+    if ((rc2 = sweep_set(x, x->s2, g.S2, g.sigt2, out, false, false))) return rc2;
+    HIP_TRY(umt::launch_gta_zone_solve(g.lu, P, out, x->G, (long)GZ, (long)x->C * x->G, out, x->stream));
+    g.greySweeps += 1;
+    return 0;
+  }
   const char* st = (const char*)g.sigt2;
   const void* sigGrey = st;
   const void* sigStretch = st + GZ * x->rsize();
@@ -1645,6 +1794,19 @@ int umt_ctx_grey_sweep(umt_ctx* x, const double* P, const double* gsrc, double* 
   if ((rc = grey_sweep(x, P ? g.P1 : nullptr, gsrc != nullptr, g.As))) return rc;
   HIP_TRY(hipMemcpyAsync(out, g.As, GC * sizeof(double), hipMemcpyDeviceToHost, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int umt_ctx_grey_transfer(umt_ctx* x, double* T, double* LU) {
+  if (!x || !T) return fail(1, "null argument");
+  if (!x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
+  if (int rc = gta_prepare(x)) return rc;
+  if (int rc = gta_build_lu(x, T)) return rc;
+  if (LU) {
+    HIP_TRY(hipMemcpyAsync(LU, x->gta.lu, 8 * (size_t)x->C * x->G * sizeof(double), hipMemcpyDeviceToHost,
+                           x->stream));
+    HIP_TRY(hipStreamSynchronize(x->stream));
+  }
   return 0;
 }
 
@@ -2220,12 +2382,16 @@ int umt_ctx_learn_stats(const umt_ctx* x, int64_t* corrections, int* fellBackAt)
 
 namespace {
 
-struct DropIn {
-  hipStream_t stream = nullptr;
+// Device buffers of one drop-in call: the host arrays' device copies.
+struct DropBufs {
   double *Afp = nullptr, *Aez = nullptr, *vol = nullptr, *S = nullptr, *sigt = nullptr,
          *omega = nullptr, *psi1 = nullptr, *phi = nullptr, *tauPsi = nullptr;
   int* cFP = nullptr;
   int2* items = nullptr;
+};
+
+struct DropIn : DropBufs {
+  hipStream_t stream = nullptr;
   size_t capC = 0, capZ = 0, capG = 0, capNb = 0, capItems = 0;
 };
 
@@ -2290,10 +2456,13 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   }
   const int nLevels = (int)planeOffset.size() - 1;
 
-  DropIn& d = g_drop[streamId % kMaxStreams];
-  if (!d.stream) HIP_TRY(hipStreamCreate(&d.stream));
-  if (int rc = dropin_alloc(d, C, Z, G, nb)) return rc;
-  hipStream_t s = d.stream;
+  // Device buffers: the stream's own allocation (grown as needed).
+  const bool useTau = tau != 0.0;
+  DropIn& di = g_drop[streamId % kMaxStreams];
+  if (!di.stream) HIP_TRY(hipStreamCreate(&di.stream));
+  if (int rc = dropin_alloc(di, C, Z, G, nb)) return rc;
+  DropBufs d = di;
+  hipStream_t s = di.stream;
   const size_t GC = (size_t)G * C;
   const std::vector<double> afpDev = umt::zone_blocked(A_fp, 9, (size_t)C);
   const std::vector<double> aezDev = umt::zone_blocked(A_ez, 9, (size_t)C);
@@ -2314,7 +2483,6 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   for (int m = cycleOffSet; m < cycleOffSet + numCycles; ++m)
     std::memcpy(Psi1 + (size_t)(cycleList[m] - 1) * G, cyclePsi + (size_t)m * G, (size_t)G * 8);
   HIP_TRY(hipMemcpyAsync(d.psi1, Psi1, (GC + (size_t)G * nb) * 8, hipMemcpyHostToDevice, s));
-  const bool useTau = tau != 0.0;
   if (useTau) HIP_TRY(hipMemcpyAsync(d.tauPsi, Psi, GC * 8, hipMemcpyHostToDevice, s));
 
   umt::SweepLevelArgs<double> a{};

@@ -29,6 +29,9 @@
 
 namespace umt {
 
+// Zeros read in place of the upwind psi of a face that brings none.
+__device__ __align__(32) const double kZeroGroups[4] = {0.0, 0.0, 0.0, 0.0};
+
 // The kernels assume the canonical hex topology of the tiled-mesh generator:
 // the EZ neighbour of local corner c (0-based) in corner-face slots 0..2 is
 //   {3,1,4} {0,2,5} {1,3,6} {2,0,7} {5,7,0} {6,4,1} {7,5,2} {4,6,3}
@@ -58,19 +61,41 @@ __device__ __forceinline__ T from_ez(T x) {
   else return dpp_move<kRowHalfMirror>(x);
 }
 
+// The EZ coupling term of EZ face f of a corner (SweepUCBxyz.F90:217-252),
+// fouralpha = 1.82, t = sigma v:
+//   gnum = A^2 (1.82 t^2 + A (4 t + 3 A)), gden = v (4 t^3 + A (6 t^2 + 2 A (2 t + A))),
+//   sez = (v gnum (sigma psiOpp - Qu) + A/2 gden (Qu - Qd)) / (gnum + gden sigma)
+// when the face is outgoing (A = Omega.A_ez > 0) and the opposite FP face
+// incoming, A (Qu - Qd) / (2 sigma) when the opposite face is not incoming,
+// and 0 on a face that is not outgoing.
+//
+// FP64 evaluates all three cases with the first expression and no selects:
+// A = max(Omega.A_ez, 0) and gnum scaled by mOpp (1 with an incoming
+// opposite face, else 0). gnum = 0 leaves A/2 gden (Qu - Qd) / (gden sigma),
+// the second case; A = 0 leaves 0 / (4 v t^3 sigma) = 0, the third (t^3
+// stays far above the FP64 underflow). The first case is bit for bit the
+// reference expression. FP32 keeps the selects (an FP32 select is one
+// instruction, and 4 v t^3 sigma can underflow in FP32).
 template <typename Real>
-__device__ __forceinline__ Real sez_term(Real A, Real sig, Real v, Real halfSigInv, Real Qu,
-                                         Real Qd, Real areaOpp, Real psiOpp) {
-  const Real sigv = sig * v;
-  const Real sigv2 = sigv * sigv;
-  const Real A2 = A * A;
-  const Real gnum = A2 * ((Real)1.82 * sigv2 + A * ((Real)4 * sigv + (Real)3 * A));
-  const Real gden =
-      v * ((Real)4 * sigv * sigv2 + A * ((Real)6 * sigv2 + (Real)2 * A * ((Real)2 * sigv + A)));
-  const Real withOpp = quot<Real>(v * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd),
-                                  gnum + gden * sig);
-  const Real noOpp = A * halfSigInv * (Qu - Qd);
-  return areaOpp > 0 ? withOpp : noOpp;
+__device__ __forceinline__ Real sez_term(Real aez, Real mOpp, Real t, Real t2, Real sig, Real v,
+                                         Real Qu, Real Qd, Real psiOpp) {
+  if constexpr (sizeof(Real) == 8) {
+    const Real A = aez > 0 ? aez : (Real)0;
+    const Real gnum = mOpp * (A * A) * ((Real)1.82 * t2 + A * ((Real)4 * t + (Real)3 * A));
+    const Real gden = v * ((Real)4 * t * t2 + A * ((Real)6 * t2 + (Real)2 * A * ((Real)2 * t + A)));
+    return quot<Real>(v * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd),
+                      gnum + gden * sig);
+  } else {
+    const Real A = aez;
+    const Real A2 = A * A;
+    const Real gnum = A2 * ((Real)1.82 * t2 + A * ((Real)4 * t + (Real)3 * A));
+    const Real gden = v * ((Real)4 * t * t2 + A * ((Real)6 * t2 + (Real)2 * A * ((Real)2 * t + A)));
+    const Real withOpp = quot<Real>(v * gnum * (sig * psiOpp - Qu) + (Real)0.5 * A * gden * (Qu - Qd),
+                                    gnum + gden * sig);
+    const Real noOpp = A * quot<Real>((Real)0.5, sig) * (Qu - Qd);
+    const Real x = mOpp > 0 ? withOpp : noOpp;
+    return aez > 0 ? x : (Real)0;
+  }
 }
 
 // One lane: one corner c of one zone for one direction and K consecutive
@@ -139,14 +164,21 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
 #pragma unroll
     for (int k = 0; k < K; ++k) Q.v[k] += p.tau * tp.v[k];
   }
-  // Incoming FP faces (SweepUCBxyz.F90:139-161). Sweeps of all angles are
-  // vacuum-bounded: an incoming boundary face brings zero, and the psi
-  // buffer's boundary tail (which angle blocks reuse) is only written. The
-  // single-angle drop-in reads incoming boundary fluxes from the tail.
+  // Incoming FP faces (SweepUCBxyz.F90:139-161). An incoming boundary face
+  // reads the boundary tail of psi when the sweep has incident fluxes (the
+  // single-angle drop-in, or an all-angle sweep whose tails were loaded
+  // before its levels); otherwise it brings zero (vacuum) and the tail is
+  // only written. noInflow sweeps every zone on its own (no upwind reads).
   GroupVec<Real, K> psf[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    if (afp[f] < 0 && (kSingleAngle || cfp[f] < p.C)) {
+    const bool rd = afp[f] < 0 && (kSingleAngle || p.boundaryIn || cfp[f] < p.C) && !p.noInflow;
+    if constexpr (sizeof(Real) == 8) {
+      // FP64: every face loads, a face without upwind psi from a zero
+      // buffer (no branches, and fewer instructions; FP32 keeps the branch,
+      // which needs fewer registers there).
+      psf[f] = load_groups<Real, K>(rd ? psiA + (long)cfp[f] * G + g0 : (const Real*)kZeroGroups);
+    } else if (rd) {
       psf[f] = load_groups<Real, K>(psiA + (long)cfp[f] * G + g0);
     } else {
 #pragma unroll
@@ -161,6 +193,10 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   Real afpIn[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) afpIn[f] = afp[f] < 0 ? afp[f] : (Real)0;
+  // 1 where the FP face opposite EZ face f is incoming, else 0.
+  Real mOpp[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) mOpp[f] = afp[(f + 1) % 3] < 0 ? (Real)1 : (Real)0;
   Real src[K], rden[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -183,16 +219,12 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
       for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
     }
     rden[k] = quot<Real>(1, outArea + sig * v);
-    const Real halfSigInv = quot<Real>((Real)0.5, sig);
     const Real Qn[3] = {from_ez<0>(q), from_ez<1>(q), from_ez<2>(q)};
+    const Real t = sig * v, t2 = t * t;
     Real sezOut[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const int fo = (f + 1) % 3;
-      const Real areaOpp = afp[fo] < 0 ? -afp[fo] : (Real)0;
-      const Real t = sez_term<Real>(aez[f], sig, v, halfSigInv, q, Qn[f], areaOpp, psf[fo].v[k]);
-      sezOut[f] = aez[f] > 0 ? t : (Real)0;
-    }
+    for (int f = 0; f < 3; ++f)
+      sezOut[f] = sez_term<Real>(aez[f], mOpp[f], t, t2, sig, v, q, Qn[f], psf[(f + 1) % 3].v[k]);
     s += sezOut[0];
     s += sezOut[1];
     s += sezOut[2];
@@ -205,18 +237,19 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
 
   // Corner solve psi_c = (src_c + sum over upstream EZ neighbours of
   // |aez| psi_n) / den_c, relaxed until every corner saw final upstream values.
-  const Real k0 = aez[0] < 0 ? -aez[0] : (Real)0;
-  const Real k1 = aez[1] < 0 ? -aez[1] : (Real)0;
-  const Real k2 = aez[2] < 0 ? -aez[2] : (Real)0;
+  Real k0 = aez[0] < 0 ? -aez[0] : (Real)0;
+  Real k1 = aez[1] < 0 ? -aez[1] : (Real)0;
+  Real k2 = aez[2] < 0 ? -aez[2] : (Real)0;
   GroupVec<Real, K> psi;
 #pragma unroll
   for (int k = 0; k < K; ++k) psi.v[k] = src[k] * rden[k];
-  int fin = (k0 == 0 && k1 == 0 && k2 == 0) ? 1 : 0;
   if (__any(cyclic)) {
     // A cyclic zone is solved once from the previous values of its own
     // corners (SweepUCBxyz.F90:281-295), which psi holds until the zone is
     // computed: its corners are on the angle's cycle list. The 8 lanes of
     // an item are all cyclic or all not, so the moves stay inside the item.
+    // Its lanes then take no upstream terms, a unit reciprocal and the
+    // solve as source, so the relaxation below reproduces it exactly.
     GroupVec<Real, K> old{};
     if (cyclic) old = load_groups<Real, K>(psiA + cc * G + g0);
 #pragma unroll
@@ -226,9 +259,12 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
                  n2 = from_ez<2>(x);
       const Real once = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
       psi.v[k] = cyclic ? once : psi.v[k];
+      src[k] = cyclic ? once : src[k];
+      rden[k] = cyclic ? (Real)1 : rden[k];
     }
-    fin = cyclic ? 1 : fin;
+    if (cyclic) k0 = k1 = k2 = 0;
   }
+  int fin = (k0 == 0 && k1 == 0 && k2 == 0) ? 1 : 0;
   for (int iter = 0; iter < 8; ++iter) {
     if (__all(fin)) break;
     const int f0 = from_ez<0>(fin), f1 = from_ez<1>(fin),
@@ -238,10 +274,9 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
       const Real x = psi.v[k];
       const Real n0 = from_ez<0>(x), n1 = from_ez<1>(x),
                  n2 = from_ez<2>(x);
-      const Real upd = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
-      psi.v[k] = cyclic ? psi.v[k] : upd;  // a cyclic zone keeps its single solve
+      psi.v[k] = (src[k] + k0 * n0 + k1 * n1 + k2 * n2) * rden[k];
     }
-    fin = cyclic || ((k0 == 0 || f0) && (k1 == 0 || f1) && (k2 == 0 || f2));
+    fin = (k0 == 0 || f0) && (k1 == 0 || f1) && (k2 == 0 || f2);
   }
 
   if (valid) {
@@ -308,6 +343,20 @@ __global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real
   const int item = sg.itemStart + (int)q;
   const int g0 = K * (int)(local - q * blocksPerItem);
   sweep_corner<Real, K, false>(p, item, g0, valid);
+}
+
+// Incident boundary fluxes into the boundary tails of an angle block's psi
+// (the caller's PsiB(:, :, Angle) loaded into Psi1's tail,
+// snac/SweepUCBxyz.F90:95-97). One thread per (angle, boundary element, group).
+template <typename Real>
+__global__ __launch_bounds__(256) void boundary_in(const Real* __restrict__ inc, long nbG, long CG,
+                                                   int a0, int nA, long angleStride,
+                                                   Real* __restrict__ psi) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbG * nA) return;
+  const long a = i / nbG;
+  const long k = i - a * nbG;
+  psi[a * angleStride + CG + k] = inc[(a0 + a) * nbG + k];
 }
 
 // Cycle-list corners (control/constructDynMemory.F90:118-189): entry m is
@@ -587,6 +636,15 @@ hipError_t launch_cycle_copy(const int2* entries, long first, long n, int G, Rea
   return hipGetLastError();
 }
 
+template <typename Real>
+hipError_t launch_boundary_in(const Real* inc, long nbG, long CG, int a0, int nA, long angleStride,
+                              Real* psi, hipStream_t s) {
+  if (nbG <= 0 || nA <= 0) return hipSuccess;
+  hipLaunchKernelGGL((boundary_in<Real>), dim3(blocks_for(nbG * nA, 256)), dim3(256), 0, s, inc, nbG,
+                     CG, a0, nA, angleStride, psi);
+  return hipGetLastError();
+}
+
 template <typename Dst, typename Src>
 hipError_t launch_convert(const Src* s_, Dst* d, long n, hipStream_t s) {
   if (n == 0) return hipSuccess;
@@ -615,6 +673,10 @@ template hipError_t launch_fill_zero<float>(float*, long, hipStream_t);
 template hipError_t launch_convert<float, double>(const double*, float*, long, hipStream_t);
 template hipError_t launch_convert<double, float>(const float*, double*, long, hipStream_t);
 template hipError_t launch_convert<float, float>(const float*, float*, long, hipStream_t);
+template hipError_t launch_boundary_in<double>(const double*, long, long, int, int, long, double*,
+                                               hipStream_t);
+template hipError_t launch_boundary_in<float>(const float*, long, long, int, int, long, float*,
+                                              hipStream_t);
 template hipError_t launch_cycle_copy<double>(const int2*, long, long, int, double*, double*, long,
                                               int, bool, hipStream_t);
 template hipError_t launch_cycle_copy<float>(const int2*, long, long, int, float*, float*, long,

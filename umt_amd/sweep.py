@@ -245,6 +245,13 @@ class SweepContext:
         check("umt_ctx_incoming_faces", _lib.load().umt_ctx_incoming_faces(self._h, C.byref(v)))
         return v.value
 
+    def level_stats(self):
+        """{launches, items, zone_visits, blocks} of the S_N sweep as launched."""
+        d = np.zeros(4, dtype=np.int64)
+        check("umt_ctx_level_stats", _lib.load().umt_ctx_level_stats(
+            self._h, d.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(launches=int(d[0]), items=int(d[1]), zone_visits=int(d[2]), blocks=int(d[3]))
+
     def phi(self):
         out = np.zeros((self.problem.ncorner, self.G))
         check("umt_ctx_get_phi", _lib.load().umt_ctx_get_phi(self._h, _dp(out)))
@@ -262,9 +269,13 @@ class SweepContext:
         return out
 
     def set_acceleration(self, mode):
-        """"none" (plain source iteration), "gta" (grey-transport acceleration),
-        "learned" (the network given to set_learner) or "gta+learned"."""
-        m = {"none": 0, "gta": 1, "learned": 2, "gta+learned": 3}[mode]
+        """"none" (plain source iteration), "gta" (grey-transport acceleration,
+        kernel 1: grey and stretched-TSA sweeps), "gta2" (GTA kernel 2: the
+        transfer-matrix solver), "learned" (the network given to set_learner),
+        "gta+learned" or "gta2+learned"."""
+        m = {"none": 0, "gta": 1, "gta2": 1, "learned": 2, "gta+learned": 3, "gta2+learned": 3}[mode]
+        if mode.startswith("gta"):
+            self.set_gta_solver(2 if mode.startswith("gta2") else 1)
         check("umt_ctx_set_acceleration", _lib.load().umt_ctx_set_acceleration(self._h, m))
 
     def set_learner(self, learner, every=2, guard=True):
@@ -326,6 +337,34 @@ class SweepContext:
         check("umt_ctx_grey_solve", _lib.load().umt_ctx_grey_solve(
             self._h, _dp(a), _dp(b), eps_grey, _dp(F), C.byref(n)))
         return F, n.value
+
+    def set_incident_flux(self, PsiB):
+        """Incident boundary psi as (A, nbelem, G) = Teton PsiB(G, nbelem, A); None: vacuum."""
+        a = _f64(PsiB) if PsiB is not None else None
+        check("umt_ctx_set_incident_flux", _lib.load().umt_ctx_set_incident_flux(self._h, _dp(a)))
+
+    def boundary_flux(self):
+        """PsiB (A, nbelem, G) after a sweep: exiting fluxes, incident ones elsewhere."""
+        p = self.problem
+        out = np.zeros((p.nangles, p.nbelem, self.G))
+        check("umt_ctx_get_boundary_flux", _lib.load().umt_ctx_get_boundary_flux(self._h, _dp(out)))
+        return out
+
+    def set_gta_solver(self, kernel):
+        """GTA kernel 1 (grey + stretched-TSA sweeps) or 2 (transfer matrices)."""
+        check("umt_ctx_set_gta_solver", _lib.load().umt_ctx_set_gta_solver(self._h, int(kernel)))
+
+    def grey_transfer(self):
+        """Kernel 2's matrices as (Z, G, 8, 8) arrays: T[z, g, s, c] (flux at
+        corner c from a unit source at corner s) and the LU factors
+        LU[z, g, c, s] of I - wtiso sigma_s T."""
+        p, G = self.problem, self.G
+        T = np.zeros((8, p.ncorner, G))
+        LU = np.zeros((8, p.ncorner, G))
+        check("umt_ctx_grey_transfer", _lib.load().umt_ctx_grey_transfer(self._h, _dp(T), _dp(LU)))
+        T = T.reshape(8, p.nzones, 8, G).transpose(1, 3, 0, 2)    # [z, g, s, c]
+        LU = LU.reshape(8, p.nzones, 8, G).transpose(1, 3, 2, 0)  # [z, g, c, s]
+        return np.ascontiguousarray(T), np.ascontiguousarray(LU)
 
     def set_angles_per_block(self, n):
         """Sweep angles in blocks of n (psi memory for n angles)."""
@@ -397,6 +436,13 @@ class SweepContext:
         check("umt_ctx_time_sweeps", _lib.load().umt_ctx_time_sweeps(
             self._h, warmup, reps, C.byref(ms), C.byref(msl), C.byref(n)))
         return ms.value, msl.value, n.value
+
+
+def device_memory():
+    """(free, total) bytes of device memory on the current GPU."""
+    f, t = C.c_int64(0), C.c_int64(0)
+    check("umt_device_memory_info", _lib.load().umt_device_memory_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def hbm_copy_probe(n_doubles, reps=10):
