@@ -204,3 +204,53 @@ def test_fortran_batch_matches_oracle(tmp_path):
         off += p.ncorner * Gd
         assert rel_err(phi, ref) < 1e-12, d
     assert off == phi_all.size
+
+
+def _teton_psib(A, nb, G):
+    """The driver's incident fluxes PsiB(g, b, a) = 0.01 (1 + mod(g + 3 b + 7 a, 11)),
+    1-based, as (A, nb, G)."""
+    a, b, g = np.meshgrid(np.arange(1, A + 1), np.arange(1, nb + 1), np.arange(1, G + 1),
+                          indexing="ij")
+    return 0.01 * (1 + (g + 3 * b + 7 * a) % 11)
+
+
+@pytest.mark.parametrize("mesh_case", ["tiled", "cyclic"])
+def test_fortran_teton_entry_point_matches_oracle(tmp_path, mesh_case):
+    """SetSweep's angle loop through SweepUCBxyz(Set, setID, Groups, Angle,
+    savePsi) of umt_teton_mod (the body teton/SweepUCBxyz.F90 compiles into
+    a Teton build): Phi, the exiting boundary fluxes and the saved Psi
+    against the oracle's sweep with the same incident fluxes; on a distorted
+    mesh whose schedule has cycle lists as well."""
+    seed, G = 20261212, 3
+    print(f"case Teton entry point, {mesh_case} mesh", flush=True)
+    if mesh_case == "tiled":
+        p = U.TiledProblem(2, 2, 1, 2, 2)
+        args = ["setsweep", 2, 2, 1]
+        extra = None
+    else:
+        pts, hexes = hex_grid(4, 0.3, 1)
+        p = U.TiledProblem.from_hex_mesh(pts, hexes, 2, 2)
+        assert int(np.sum(p.schedule()["numCycles"])) > 0
+        extra = tmp_path / "cyclic_mesh.bin"
+        with open(extra, "wb") as f:
+            np.array([pts.shape[0], hexes.shape[0]], np.int32).tofile(f)
+            np.ascontiguousarray(pts, np.float64).tofile(f)
+            np.ascontiguousarray(hexes, np.int32).tofile(f)
+        args = ["setsweep", 0, 0, 0]
+    out_all, out = _run(args + [2, 2, G, 64, 0, 0.0, seed, 0.1, 10.0, 0.0, 0.9, 0.0, 1.0, 1],
+                        "phi_setsweep.bin", tmp_path, extra=extra)
+    assert "setsweep angles=" in out
+    A, Cn, nb = p.nangles, p.ncorner, p.nbelem
+    phi = out_all[:Cn * G].reshape(Cn, G)
+    psib = out_all[Cn * G:Cn * G + A * nb * G].reshape(A, nb, G)
+    psi = out_all[Cn * G + A * nb * G:].reshape(A, Cn, G)
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (0.1, 10.0), (0.0, 0.9), (0.0, 1.0))
+    S = np.repeat(ss + qq, 8, axis=0) / (4 * np.pi)
+    s = p.schedule()
+    cyc = np.zeros((max(len(s["cycleList"]), 1), G)) if len(s["cycleList"]) else None
+    ref, ref_B = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), s, S, st,
+                             PsiB=_teton_psib(A, nb, G), cycle_psi=cyc)
+    assert rel_err(phi, ref) < 1e-12
+    assert np.max(np.abs(psib - ref_B)) <= 1e-12 * np.abs(ref_B).max()
+    w = p.quadrature()["weight"]
+    np.testing.assert_allclose(np.tensordot(w, psi, axes=(0, 0)), phi, rtol=1e-12, atol=1e-300)

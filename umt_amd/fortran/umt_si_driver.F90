@@ -18,6 +18,14 @@
 !                   source iteration with the tau Psi source of
 !                   snac/SweepUCBxyz.F90:121, then the savePsi sweep, from a
 !                   Psi(G, C, A) the host holds; appends the final Psi;
+!    mode "setsweep": one pass of SetSweep's angle loop (snac/SetSweep.F90:
+!                   113-170) through SweepUCBxyz(Set, setID, Groups, Angle,
+!                   savePsi) of umt_teton_mod, the Teton-shaped entry point,
+!                   with incident boundary fluxes PsiB(g, b, a) =
+!                   0.01 (1 + mod(g + 3 b + 7 a, 11)) and the problem's cycle
+!                   lists; on the hex mesh of the file named by the 20th
+!                   argument when one is given; appends the final PsiB(G,
+!                   nbelem, A) and Psi(G, C, A) after Phi;
 !    mode "batch":  `angle` uneven domains (tiles nx + d - 1, ny, nz and
 !                   groups + d - 1 for domain d, cross sections drawn with
 !                   seed + d) iterated together in one batch; writes every
@@ -33,6 +41,7 @@ program umt_si_driver
 
   use iso_c_binding
   use umt_sweep_mod
+  use umt_teton_mod, only: SetData, HypPlane, Geom, Size, Quad, SweepUCBxyz
 
   implicit none
 
@@ -71,7 +80,7 @@ program umt_si_driver
   call get_command_argument(19, outfile)
   call get_command_argument(20, extra)
 
-  if (trim(mode) == "mesh") then
+  if (trim(mode) == "mesh" .or. (trim(mode) == "setsweep" .and. len_trim(extra) > 0)) then
     block
       integer(c_int) :: npts, nzs
       real(c_double), allocatable :: coords(:,:)
@@ -168,6 +177,75 @@ program umt_si_driver
     status = gpu_sweepucbxyz_status()
     if (status /= 0) call check(status, "gpu_sweepucbxyz")
     write(*, '(a,i4,a,es22.15)') "dropin angle=", angle, " sumPhi=", sum(Phi)
+
+  elseif (trim(mode) == "setsweep") then
+
+    ! Teton-shaped data (Geom, Quad with one angle and group set, Set) from
+    ! the host arrays, then SetSweep's loop: Phi = 0, SweepUCBxyz per angle.
+    block
+      type(SetData), target :: Set
+      integer(c_int) :: totalCycles
+      integer        :: a, b, g, h, ncl
+      allocate(Geom% numCorner(Z), Geom% cOffSet(Z), Geom% nCFacesArray(C), Geom% cFP(3,C), &
+               Geom% cEZ(3,C), Geom% Volume(C), Geom% A_fp(3,3,C), Geom% A_ez(3,3,C))
+      Geom% numCorner = numCorner; Geom% cOffSet = cOffSet; Geom% nCFacesArray = nCFaces
+      Geom% cFP = cFP; Geom% cEZ = cEZ; Geom% Volume = Volume; Geom% A_fp = A_fp
+      Geom% A_ez = A_ez
+      Size% tau = 0.0d0
+      allocate(Quad% AngSet(1), Quad% GrpSet(1))
+      associate (ASet => Quad% AngSet(1), GSet => Quad% GrpSet(1))
+        ASet% NumAngles = A
+        allocate(ASet% omega(3,A), ASet% weight(A), ASet% nextZ(Z,A), ASet% nextC(C,A), &
+                 ASet% nHyperPlanes(A), ASet% numCycles(A), ASet% cycleOffSet(A),         &
+                 ASet% HypPlanePtr(A))
+        ASet% omega = omega; ASet% weight = weight; ASet% nextZ = nextZ; ASet% nextC = nextC
+        ASet% nHyperPlanes = nHyperPlanes
+        start = 0
+        do a = 1, A
+          allocate(ASet% HypPlanePtr(a)% zonesInPlane(nHyperPlanes(a)))
+          do h = 1, nHyperPlanes(a)
+            ASet% HypPlanePtr(a)% zonesInPlane(h) = zonesInPlane(start + h)
+          enddo
+          start = start + nHyperPlanes(a)
+        enddo
+        ! cycle lists of the schedule (rt/rtorder.F90:62-90 offsets)
+        rc = umt_problem_cycles(prob, totalCycles, c_loc(ASet% numCycles), c_null_ptr)
+        call check(rc, "umt_problem_cycles")
+        ncl = max(int(totalCycles), 1)
+        allocate(ASet% cycleList(ncl))
+        ASet% cycleList = 0
+        rc = umt_problem_cycles(prob, totalCycles, c_loc(ASet% numCycles), c_loc(ASet% cycleList))
+        call check(rc, "umt_problem_cycles")
+        ASet% cycleOffSet(1) = 0
+        do a = 2, A
+          ASet% cycleOffSet(a) = ASet% cycleOffSet(a-1) + ASet% numCycles(a-1)
+        enddo
+        allocate(GSet% STotal(G,C), GSet% Sigt(G,Z))
+        do i = 1, C
+          GSet% STotal(:,i) = (sigs(:, (i-1)/8 + 1) + q(:, (i-1)/8 + 1)) / (16.0d0*atan(1.0d0))
+        enddo
+        GSet% Sigt = Sigt
+        Set% Groups = G; Set% nCorner = C; Set% nbelem = nb; Set% NumAngles = A
+        allocate(Set% Psi(G,C,A), Set% Psi1(G,C+nb), Set% PsiB(G,nb,A), Set% Phi(G,C), &
+                 Set% cyclePsi(G,ncl))
+        Set% Psi = 0.0d0; Set% Psi1 = 0.0d0; Set% Phi = 0.0d0; Set% cyclePsi = 0.0d0
+        do a = 1, A
+          do b = 1, nb
+            do g = 1, G
+              Set% PsiB(g,b,a) = 0.01d0 * (1 + mod(g + 3*b + 7*a, 11))
+            enddo
+          enddo
+        enddo
+        do a = 1, A
+          call SweepUCBxyz(Set, 1, G, a, logical(.true., kind=1))
+        enddo
+        Phi = Set% Phi
+        allocate(Psi3(G, nb*A + C*A, 1))
+        Psi3(:,:,1) = reshape([reshape(Set% PsiB, [G*nb*A]), reshape(Set% Psi, [G*C*A])], &
+                              [G, nb*A + C*A])
+      end associate
+      write(*, '(a,i4,a,es22.15)') "setsweep angles=", A, " sumPhi=", sum(Phi)
+    end block
 
   elseif (trim(mode) == "time") then
 
