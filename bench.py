@@ -1,25 +1,31 @@
 #!/usr/bin/env python3
 """Sweep throughput and time to convergence of the MI355X UCB xyz sweep.
 
-Workload (one step = one full sweep of all angles and groups): the unit cube
-as 20x20x20 tiles of UMT's 24-zone tile (192,000 hex zones, 1,536,000
-corners), 4x4 product quadrature per octant (128 angles), 16 groups,
-sigma_t ~ log-uniform[1e-2, 1e3] per (zone, group), c ~ U[0, 0.95],
-q ~ U[0, 1], vacuum boundaries, seed 20261204. Unknowns per sweep =
-corners x angles x groups = 3,145,728,000 (driver/test_driver.cc:1895).
+Headline workload (one step = one full sweep of all angles and groups): the
+unit cube as 32x32x32 tiles of UMT's 24-zone tile (786,432 hex zones,
+6,291,456 corners), 4x4 product quadrature per octant (128 angles), 32
+groups, sigma_t ~ log-uniform[1e-2, 1e3] per (zone, group), c ~ U[0.9,
+0.999], q ~ U[0, 1], vacuum boundaries, seed 20261207: the largest
+single-GPU workload (FP32 psi of every angle is 107 GB). Unknowns per sweep
+= corners x angles x groups = 25,769,803,776 (driver/test_driver.cc:1895).
 
 Prints one JSON line. ``value`` is FP64 sweep unknowns/s with every input
-resident in HBM. Also reported: the FP32 sweep (FP64 phi), the wall time of
-source iteration to residual < 1e-8 (plain FP64 against grey-transport
-acceleration and the learned correction, FP64 and FP32 then FP64), the
-sweep-level kernel against the HBM roofline, and the oracle's CPU sweep
-timed on this host's cores; and the 2D cylindrical (RZ) sweep on a regular
-512x512 quad mesh with the RZ product quadrature 4x4 (FP64, FP32, roofline,
-CPU oracle).
+resident in HBM. Also reported for the same workload: the FP32 sweep (FP64
+phi), the level kernel against the HBM roofline (byte model of this
+design's dataflow, and the measured HBM traffic from committed rocprofv3
+summaries), the wall time of source iteration to residual < 1e-8 (plain
+FP64 run to convergence against FP32-then-FP64 with grey-transport
+acceleration, with and without the learned correction), and the oracle's
+CPU sweep timed on this host's cores. Secondary sections: the 20x20x20-tile
+single-domain workload (sweeps, time to convergence of every method,
+per-angle drop-in with host arrays), the learned correction's own
+4x4x4-tile workload, 64 batched uneven domains (FP32 and FP64), and the 2D
+cylindrical (RZ) sweep on a 512x512 quad mesh.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -28,22 +34,48 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-SEED = 20261204
 HBM_PEAK_GBS = 8000.0
 
+# name: (tiles, npolar x nazimuthal per octant, groups, c range, seed)
+WORKLOADS = {
+    "large": dict(tiles=(32, 32, 32), quad=(4, 4), groups=32, c=(0.9, 0.999), seed=20261207),
+    "single": dict(tiles=(20, 20, 20), quad=(4, 4), groups=16, c=(0.0, 0.95), seed=20261204),
+}
+GRAPHS = [True]  # replay captured sweeps (off for profiler runs)
 
-def bytes_per_unknown(precision, n_in, G):
-    """Algorithmic HBM bytes per unknown of the reference dataflow (one angle
-    at a time, no reuse across angles): source read, psi write, n_in upwind
-    psi reads, phi read+write in FP64, sigma_t shared by 8 corners, and the
-    per-corner geometry/connectivity bytes shared by the G groups."""
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Byte models
+
+
+def design_bytes_per_unknown(precision, n_in, G, d):
+    """Bytes per unknown the level kernel of this design must move between
+    itself and memory: the psi write, the n_in upwind psi reads of interior
+    faces, and per (level, zone) visit the zone's source S (G x 8 corners),
+    sigma_t (G) and corner geometry (A_fp, A_ez, Volume, cFP: 164 B per
+    corner in FP64, 88 B in FP32), shared by the d directions that meet the
+    zone in the same launch; plus the (zone, angle) item (8 B per 8 corners
+    x G groups). Re-reads inside a launch are not counted; psi written by an
+    earlier launch and read from L2 or the MALL still counts."""
+    r = 8 if precision == 64 else 4
+    geo = 164.0 if precision == 64 else 88.0
+    return r + r * n_in + (r + r / 8.0 + geo / G) / d + 1.0 / G
+
+
+def reference_bytes_per_unknown(precision, n_in, G):
+    """The reference dataflow's bytes per unknown (one angle at a time, no
+    reuse across angles, phi read-modify-write per angle; SweepUCBxyz.F90),
+    reported for comparison only: this design moves far less."""
     if precision == 64:
         return 8 + 8 + 8 * n_in + 16 + 1 + 185.5 / G
     return 4 + 4 + 4 * n_in + 16 + 0.5 + 109.5 / G
 
 
-def load_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any."""
+def load_profile(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -54,25 +86,70 @@ def load_traffic(name):
         return None
 
 
-def cpu_model():
+def measured_traffic(precision, tiles, G, A, apb):
+    """HBM bytes per launch of the level kernel from the committed
+    rocprofv3 --pmc summary for this workload and angle blocking, if any."""
+    t = "x".join(str(v) for v in tiles)
+    prof = load_profile(f"ucb_sweep_level_fp{precision}_{t}_traffic.json")
+    if not prof:
+        return None
+    w = prof.get("workload_key", {})
+    if (tuple(w.get("tiles", ())) != tuple(tiles) or w.get("groups") != G or w.get("angles") != A
+            or w.get("angles_per_block") != apb):
+        return None
+    return prof
+
+
+# ---------------------------------------------------------------------------
+# Host
+
+
+def host_info():
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                     "NUMA node(s)"):
+                info["lscpu " + k] = v
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    if "lscpu Model name" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["cpuinfo model name"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
 
 
-def cpu_baseline(p, G, st, S, seconds):
-    """Oracle sweep (OpenMP over angles, one angle per thread) on a subset of
-    angles, repeated for about `seconds` seconds."""
+def cpu_threads():
+    """Every CPU this process may run on, or OMP_NUM_THREADS when the
+    machine sets it lower (the GPU box sets it to this GPU's CPU share)."""
+    n = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, env) if env > 0 else n)
+
+
+def cpu_baseline(p, G, st, S, seconds, host):
+    """The oracle's C sweep (OpenMP over angles, one angle per thread as
+    SetSweep runs one set per thread) on as many angles as threads,
+    repeated until about `seconds` have passed (at least once)."""
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16, p.nangles))
-    k = threads
+    threads = cpu_threads()
+    # Each thread holds its angle's Psi1 and its own phi (oracle.c): keep the
+    # host memory of the sample under ~120 GB.
+    per_thread = ((p.ncorner + p.nbelem) * G + p.ncorner * G) * 8
+    mem_cap = max(1, int(120e9 // per_thread))
+    k = max(1, min(threads, mem_cap, p.nangles))
+    threads = k
     m, g, q, s = p.mesh(), p.geometry(), p.quadrature(), p.schedule()
     nplanes = int(np.sum(s["nHyperPlanes"][:k]))
     sub_q = dict(omega=np.ascontiguousarray(q["omega"][:k]),
@@ -89,23 +166,55 @@ def cpu_baseline(p, G, st, S, seconds):
         t_total += time.perf_counter() - t0
         reps += 1
     unknowns = p.ncorner * k * G * reps
+    model = host.get("lscpu Model name") or host.get("cpuinfo model name", "unknown")
     return {
         "value": unknowns / t_total,
         "unit": "unknowns/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle C sweep (liboracle_fast.so, -O3 -march=x86-64-v3, OpenMP over "
-                   f"angles) of {k} of the {p.nangles} angles x {G} groups x {p.ncorner} corners, "
-                   f"{reps} repetitions in {t_total:.1f} s on {threads} threads of '{cpu_model()}'"),
+        "sample": (f"oracle C sweep (liboracle_fast.so, -O3 -march=x86-64-v3, OpenMP over angles, "
+                   f"one angle per thread) of {k} of the {p.nangles} angles x {G} groups x "
+                   f"{p.ncorner} corners of the same workload, {reps} repetition(s) in "
+                   f"{t_total:.1f} s on {threads} threads of '{model}' (nproc {host.get('nproc')}, "
+                   f"{host.get('affinity_cpus')} CPUs in this process's affinity, "
+                   f"OMP_NUM_THREADS={host.get('OMP_NUM_THREADS')}, memory cap {mem_cap} threads)"),
+        "host": host,
     }
 
 
-def measure(p, G, precision, st, S, warmup, steps, lib):
-    import umt_amd as U
+# ---------------------------------------------------------------------------
+# Sweeps
 
+
+def psi_bytes(p, G, precision, n_angles):
+    return (p.ncorner + p.nbelem) * G * n_angles * (8 if precision == 64 else 4)
+
+
+def context_extra_bytes(p, G, precision):
+    """Device memory of a context besides its S_N psi: fields, GTA vectors
+    and the S2 psi (8 directions)."""
+    rs = 8 if precision == 64 else 4
+    return 16 * p.ncorner * G * 8 + (p.ncorner + p.nbelem) * G * 8 * rs + 2 * 10**9
+
+
+def angles_per_block(p, G, precision, budget):
+    """All angles in one block if their psi fits in `budget` bytes, else the
+    widest block (A/2, A/4, ...) that does."""
+    n = p.nangles
+    while n > 1 and psi_bytes(p, G, precision, n) > budget:
+        n = (n + 1) // 2
+    return 0 if n == p.nangles else n
+
+
+def measure_sweep(p, G, precision, st, S, warmup, steps, apb):
+    import umt_amd as U
+    from umt_amd import _lib
+
+    lib = _lib.load()
     ctx = U.SweepContext(p, G, precision=precision)
-    if APB[precision]:
-        ctx.set_angles_per_block(APB[precision])
+    if apb:
+        ctx.set_angles_per_block(apb)
+    ctx.set_graphs(GRAPHS[0])
     ctx.set_sigt(st)
     ctx.set_source(S)
     best_k, tune_ms = ctx.autotune(2)
@@ -115,134 +224,104 @@ def measure(p, G, precision, st, S, warmup, steps, lib):
     ctx.sweep_n(steps)
     lib.gpu_devicesynchronize()
     dt = time.perf_counter() - t0
+    # HIP events on the context's stream around every level launch.
     ms_sweep_ev, ms_levels, launches = ctx.time_sweeps(1, 3)
-    n_in = ctx.incoming_faces()
+    out = dict(dt=dt, ms_levels=ms_levels, launches=launches, n_in=ctx.incoming_faces(),
+               ms_sweep_events=ms_sweep_ev, groups_per_lane=best_k,
+               autotune_ms_per_sweep={str(k): v for k, v in tune_ms.items()},
+               level_stats=ctx.level_stats())
     ctx.close()
-    return dict(dt=dt, ms_levels=ms_levels, launches=launches, n_in=n_in,
-                ms_sweep_events=ms_sweep_ev, groups_per_lane=best_k,
-                autotune_ms_per_sweep={str(k): v for k, v in tune_ms.items()})
-
-
-def rz_bytes_per_unknown(precision, n_in, G, start_frac):
-    """Algorithmic HBM bytes per unknown of the RZ level kernel: source read,
-    sigma_t shared by 4 corners, psi write, n_in upwind psi reads, half-angle
-    flux read (not for starting directions) and write, and the per-corner
-    geometry (A_fp, A_ez, RadiusFP, RadiusEZ, Area, Volume in the sweep
-    precision, cFP as two int32) shared by the G groups."""
-    r = 8 if precision == 64 else 4
-    geo = 15 * r + 8
-    return r + r / 4 + r + r * n_in + r * (1 - start_frac) + r + geo / G
-
-
-def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds, log):
-    """The 2D cylindrical sweep: FP64 and FP32 throughput, the level kernel
-    against the HBM roofline, and the RZ oracle timed on this host's cores."""
-    import umt_amd as U
-
-    p = U.RzProblem(nr, nz, npolar, nazimuthal, r0=0.0, r1=1.0, z0=0.0, z1=1.0)
-    q, g, m = p.quadrature(), p.geometry(), p.mesh()
-    swept = q["finishing"] == 0
-    unknowns = p.ncorner * int(swept.sum()) * G
-    start_frac = float(q["starting"].sum()) / float(swept.sum())
-    # incoming interior FP faces per (corner, swept direction)
-    inner = m["cFP"] <= p.ncorner
-    cnt = 0
-    for a in np.where(swept)[0]:
-        d = np.einsum("cfk,k->cf", g["A_fp"], q["omega"][a])
-        cnt += int(np.count_nonzero((d < 0) & inner))
-    n_in = cnt / (p.ncorner * int(swept.sum()))
-    rng = np.random.default_rng(SEED)
-    sig = np.exp(rng.uniform(np.log(1e-2), np.log(1e3), (p.nzones, G)))
-    S = rng.uniform(0.0, 1.0, (p.ncorner, G)) / (2 * np.pi)
-    out = {"config": {"workload": (f"UCB RZ sweep, {nr}x{nz} quad zones on [0,1]x[0,1] "
-                                   f"({p.nzones} zones, "
-                                   f"{p.ncorner} corners), RZ product quadrature {npolar}x"
-                                   f"{nazimuthal} ({p.nangles} directions in {p.nlevels} xi "
-                                   f"levels, {int(swept.sum())} swept), {G} groups, sigma_t "
-                                   f"logU[1e-2,1e3], vacuum"),
-                      "unknowns_per_sweep": unknowns, "level_launches": p.sweep_levels},
-           "unit": "unknowns/s (corners x swept directions x groups per second)"}
-    for prec in (64, 32):
-        ctx = U.RzContext(p, G, precision=prec)
-        ctx.set_graphs(GRAPHS[0])
-        ctx.set_sigt(sig)
-        ctx.set_source(S)
-        K, ms_k = ctx.autotune(2)
-        for _ in range(warmup):
-            ctx.sweep()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            ctx.sweep()
-        dt = time.perf_counter() - t0
-        ms_ev, ms_lv = ctx.time_sweeps(1, 3)
-        ctx.close()
-        B = rz_bytes_per_unknown(prec, n_in, G, start_frac)
-        achieved = B * unknowns / (ms_lv * 1e-3) / 1e9
-        key = "fp64" if prec == 64 else "fp32"
-        kname = f"void umt::rz_sweep_level<{'double' if prec == 64 else 'float'}, {K}>(umt::RzLevelArgs<{'double' if prec == 64 else 'float'}>)"
-        prof = load_traffic("rz_sweep_level_512x512_traffic.json")
-        traffic = None
-        if prof and (nr, nz, G) == (512, 512, 16) and kname in prof.get("kernels", {}):
-            traffic = prof["kernels"][kname]["hbm_bytes_per_launch"]
-        out[key] = {"value": unknowns * steps / dt, "ms_per_step": dt * 1e3 / steps,
-                    "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                                 "traffic": traffic,
-                                 "kernel": f"rz_sweep_level<{'double' if prec == 64 else 'float'},K>",
-                                 "level_kernels_ms_per_sweep": ms_lv,
-                                 "algorithmic_bytes_per_unknown": B, "n_in": n_in,
-                                 "groups_per_lane": K,
-                                 "autotune_ms_per_sweep": {str(k): v for k, v in ms_k.items()}}}
-        log(f"RZ FP{prec} sweep: {unknowns * steps / dt:.3e} unknowns/s, level kernels "
-            f"{ms_lv:.2f} ms ({achieved:.0f} GB/s), groups per lane {K} (autotune {ms_k})")
-    if cpu_seconds > 0:
-        from oracle import rz as R
-
-        Gc = 4
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        threads = max(1, min(threads, 16, p.nlevels))
-        s = p.schedule()
-        Sc, sc = np.ascontiguousarray(S[:, :Gc]), np.ascontiguousarray(sig[:, :Gc])
-        R.lib(fast=True)
-        reps, t_total = 0, 0.0
-        while t_total < cpu_seconds or reps == 0:
-            t0 = time.perf_counter()
-            R.sweep_all(m, g, q, s, Sc, sc, nthreads=threads, fast=True)
-            t_total += time.perf_counter() - t0
-            reps += 1
-        out["cpu_baseline"] = {
-            "value": p.ncorner * int(swept.sum()) * Gc * reps / t_total, "unit": "unknowns/s",
-            "cores": threads, "kind": "port",
-            "sample": (f"oracle C RZ sweep (liboracle_fast.so, OpenMP over xi levels) of the "
-                       f"same mesh and directions with {Gc} groups, {reps} sweeps in "
-                       f"{t_total:.1f} s on {threads} threads of '{cpu_model()}'")}
     return out
 
 
-APB = {64: 0, 32: 0}  # angles per block of the FP64 / FP32 contexts (0: all)
-GRAPHS = [True]       # replay captured sweeps (off for profiler runs)
+def roofline_block(precision, r, unknowns, G, A, tiles, apb):
+    ls = r["level_stats"]
+    d = ls["items"] / ls["zone_visits"]
+    B = design_bytes_per_unknown(precision, r["n_in"], G, d)
+    launches = r["launches"]
+    mean_launch_s = r["ms_levels"] * 1e-3 / launches
+    per_launch = unknowns / launches
+    achieved = B * per_launch / mean_launch_s / 1e9
+    prof = measured_traffic(precision, tiles, G, A, apb)
+    traffic = prof["hbm_bytes_per_launch"] if prof else None
+    rs = 8 if precision == 64 else 4
+    # Whole sweep: the level kernels plus the phi reduction's psi re-read and
+    # phi write, over the sweep's HIP-event time.
+    sweep_B = B + rs + 8.0 / A
+    sweep_ms = r["ms_sweep_events"]
+    blk = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "kernel": f"ucb_sweep_level<{'double' if precision == 64 else 'float'},{r['groups_per_lane']},false>",
+        "byte_model": ("psi write + n_in upwind psi reads + (S + sigma_t + corner geometry) once per "
+                       "(level, zone) visit shared by d directions + (zone, angle) items"),
+        "algorithmic_bytes_per_unknown": B, "n_in": r["n_in"], "directions_per_zone_visit": d,
+        "unknowns_per_launch": per_launch, "launches_per_sweep": launches,
+        "mean_launch_us": mean_launch_s * 1e6, "level_kernels_ms_per_sweep": r["ms_levels"],
+        "groups_per_lane": r["groups_per_lane"], "angles_per_block": apb or A,
+        "autotune_ms_per_sweep": r["autotune_ms_per_sweep"],
+        "reference_dataflow_bytes_per_unknown": reference_bytes_per_unknown(precision, r["n_in"], G),
+        "sweep": {"bytes_per_unknown": sweep_B, "ms_per_sweep_events": sweep_ms,
+                  "frac": sweep_B * unknowns / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    if prof:
+        blk["measured_hbm_GBps"] = traffic / mean_launch_s / 1e9
+        blk["measured_hbm_frac"] = blk["measured_hbm_GBps"] / HBM_PEAK_GBS
+        blk["measured_bytes_per_unknown"] = traffic / per_launch
+        blk["traffic_profile"] = prof.get("file", "")
+    return blk
 
 
-def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True, history=False):
+def sweep_section(p, G, st, S, args, tiles, label):
+    """FP64 and FP32 sweep throughput with their roofline blocks."""
+    import umt_amd as U
+
+    unknowns = p.unknowns(G)
+    free, total = U.sweep.device_memory()
+    out = {}
+    for prec in ((64,) if args.no_fp32 else (64, 32)):
+        budget = free - context_extra_bytes(p, G, prec) - 4 * 10**9
+        apb = angles_per_block(p, G, prec, budget)
+        r = measure_sweep(p, G, prec, st, S, args.warmup, args.steps, apb)
+        v = unknowns * args.steps / r["dt"]
+        key = "fp64" if prec == 64 else "fp32"
+        out[key] = {"value": v, "unit": "unknowns/s", "ms_per_step": r["dt"] * 1e3 / args.steps,
+                    "dt": r["dt"],
+                    "roofline": roofline_block(prec, r, unknowns, G, p.nangles, tiles, apb)}
+        rf = out[key]["roofline"]
+        log(f"{label} FP{prec} sweep: {v:.3e} unknowns/s, {r['dt'] * 1e3 / args.steps:.2f} ms/sweep, "
+            f"level kernels {r['ms_levels']:.2f} ms ({rf['launches_per_sweep']} launches, "
+            f"{rf['mean_launch_us']:.1f} us mean), design bytes {rf['algorithmic_bytes_per_unknown']:.2f} "
+            f"B/unknown -> frac {rf['frac']:.3f}, measured frac {rf.get('measured_hbm_frac')}")
+    out["device_memory_bytes"] = {"free_at_start": free, "total": total}
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Time to convergence
+
+
+def solve(p, G, xs, accel, mixed, tol, max_iters, apb, net=None, warm_iters=2):
     """Source iteration to `tol` from phi = 0: FP64 sweeps (mixed False) or
     FP32 sweeps to 1e-6 then FP64 sweeps (mixed True), with acceleration
-    accel ("none", "gta" or "learned"), graph replay on. The timed solve is
-    the second of two identical ones (the first captures the graphs)."""
+    accel ("none", "gta", "learned" or "gta+learned"), graph replay on.
+    Every context first runs a warm-up solve of warm_iters iterations (which
+    captures its graphs); the timed solve then starts again from phi = 0."""
     import umt_amd as U
 
     st, ss, qq = xs
 
     def context(precision):
         ctx = U.SweepContext(p, G, precision=precision)
+        if apb.get(precision):
+            ctx.set_angles_per_block(apb[precision])
         ctx.set_sigt(st)
         ctx.set_scattering(ss, qq)
-        if APB[precision]:
-            ctx.set_angles_per_block(APB[precision])
         ctx.autotune(1)
-        if accel in ("gta", "gta+learned"):
+        if accel.startswith("gta"):
             ctx.autotune_grey(2)
         ctx.set_graphs(GRAPHS[0])
-        if accel in ("learned", "gta+learned"):
+        if accel.endswith("learned"):
             ctx.set_learner(net, every=2)
         ctx.set_acceleration(accel)
         return ctx
@@ -250,26 +329,29 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, net=None, repeat=True, history
     exact = context(64)
     fast = context(32) if mixed else None
     zero = np.zeros((p.ncorner, G))
-    out = None
-    for attempt in range(2 if repeat else 1):
-        for c in (exact, fast):
-            if c is not None:
-                c.set_phi(zero)
-        t0 = time.perf_counter()
-        if mixed:
-            ok, it32, it64, res, hist = U.solve_mixed(fast, exact, tol=tol, switch_tol=1e-6,
-                                                      max_iters=max_iters)
-        else:
-            ok, it64, res, hist = exact.source_iteration(tol=tol, max_iters=max_iters)
-            it32 = 0
-        dt = time.perf_counter() - t0
-        out = dict(converged=ok, iterations_fp32=it32, iterations_fp64=it64, residual=res,
-                   seconds=dt)
-        if history:
-            out["history"] = [float(h) for h in hist]
-    if accel in ("gta", "gta+learned"):
+    for c in (exact, fast):
+        if c is not None:
+            c.set_phi(zero)
+            c.source_iteration(tol=tol, max_iters=warm_iters)
+            c.set_phi(zero)
+            if hasattr(c, "reset_stats"):
+                c.reset_stats()
+    g0 = exact.gta_stats() if accel.startswith("gta") else (0, 0)
+    t0 = time.perf_counter()
+    if mixed:
+        ok, it32, it64, res, hist = U.solve_mixed(fast, exact, tol=tol, switch_tol=1e-6,
+                                                  max_iters=max_iters)
+    else:
+        ok, it64, res, hist = exact.source_iteration(tol=tol, max_iters=max_iters)
+        it32 = 0
+    dt = time.perf_counter() - t0
+    out = dict(converged=ok, iterations_fp32=it32, iterations_fp64=it64, residual=res, seconds=dt)
+    if accel.startswith("gta"):
         gi, gs = exact.gta_stats()
-        out["grey_iterations_fp64_total"] = gi
+        out["grey_iterations_fp64"] = gi - g0[0]
+        out["grey_sweeps_fp64"] = gs - g0[1]
+        if fast is not None:
+            out["grey_sweeps_fp32"] = fast.gta_stats()[1]
     phi = exact.phi()
     for c in (exact, fast):
         if c is not None:
@@ -281,7 +363,127 @@ def rel_error(phi, ref):
     return float(np.max(np.abs(phi - ref) / np.maximum(np.abs(ref), 1e-300)))
 
 
-def dropin_rate(p, G, st, S, n, log):
+ALL_METHODS = ["fp64_plain", "fp64_gta", "fp64_gta2", "fp32_then_fp64_gta", "fp32_then_fp64_gta2",
+               "fp64_learned", "fp32_then_fp64_learned", "fp64_gta_learned",
+               "fp32_then_fp64_gta_learned", "fp32_then_fp64_gta2_learned"]
+# name: (acceleration, mixed, network). gta is GTA kernel 1 (grey and
+# stretched-TSA sweeps), gta2 kernel 2 (transfer matrices and zone LU).
+METHOD = {
+    "fp64_plain": ("none", False, None), "fp64_gta": ("gta", False, None),
+    "fp64_gta2": ("gta2", False, None),
+    "fp32_then_fp64_gta": ("gta", True, None), "fp32_then_fp64_gta2": ("gta2", True, None),
+    "fp64_learned": ("learned", False, "plain"),
+    "fp32_then_fp64_learned": ("learned", True, "plain"),
+    "fp64_gta_learned": ("gta+learned", False, "gta"),
+    "fp32_then_fp64_gta_learned": ("gta+learned", True, "gta"),
+    "fp32_then_fp64_gta2_learned": ("gta2+learned", True, "gta2"),
+}
+
+
+def time_to_converge(p, G, xs, tol, max_iters, methods, learn_tiles, c_range):
+    """Plain FP64 source iteration (the reference method on the GPU), run to
+    convergence, against grey-transport acceleration and the learned
+    correction, FP64 and FP32->FP64. Errors are against the converged plain
+    FP64 answer."""
+    import umt_amd as U
+
+    free, _ = U.sweep.device_memory()
+    # FP64 psi of the mixed solves shares the device with the FP32 context.
+    # FP32 psi of every angle when that leaves room for half the angles' FP64 psi.
+    budget32 = free - context_extra_bytes(p, G, 32) - context_extra_bytes(p, G, 64) - 8 * 10**9
+    apb32 = angles_per_block(p, G, 32, budget32 - psi_bytes(p, G, 64, (p.nangles + 1) // 2))
+    left = budget32 - psi_bytes(p, G, 32, apb32 or p.nangles)
+    apb_mixed = {32: apb32, 64: angles_per_block(p, G, 64, left)}
+    apb_plain = {64: angles_per_block(p, G, 64, free - context_extra_bytes(p, G, 64) - 4 * 10**9)}
+    res = {"angles_per_block": {"fp64_alone": apb_plain[64] or p.nangles,
+                                "mixed_fp64": apb_mixed[64] or p.nangles,
+                                "mixed_fp32": apb_mixed[32] or p.nangles},
+           "timing": "each solve timed after a 2-iteration warm-up solve of the same contexts "
+                     "(graph capture), from phi = 0"}
+    base, phi64 = solve(p, G, xs, "none", False, tol, max_iters, apb_plain)
+    res["fp64_plain"] = base
+    log(f"fp64_plain: {base}")
+    nets = {}
+    need = {METHOD[m][2] for m in methods if METHOD[m][2]}
+    if learn_tiles:
+        small = U.TiledProblem(learn_tiles, learn_tiles, learn_tiles, p.npolar, p.nazimuthal)
+        for key in ("plain", "gta", "gta2"):
+            if key not in need:
+                continue
+            on = "none" if key == "plain" else key
+            t0 = time.perf_counter()
+            net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
+                                            c=c_range, q=(0.0, 1.0), rounds=3, batch=8192,
+                                            steps=1500, collect_stride=4 if on == "none" else 1,
+                                            max_iters=max_iters, base=on, graphs=GRAPHS[0])
+            res[f"learned_training_{key}"] = {
+                "seconds": time.perf_counter() - t0,
+                "problem": f"{learn_tiles}^3 tiles, {p.nangles} angles, {G} groups, seeds 1-3 "
+                           f"(cross sections drawn like the workload's), iterates of "
+                           f"{'plain' if on == 'none' else on + '-accelerated'} source iteration",
+                "losses": info["losses"], "samples": info["samples"]}
+            log(f"training on {key} iterates: {res[f'learned_training_{key}']}")
+            nets[key] = net
+    for name in methods:
+        if name == "fp64_plain":
+            continue
+        accel, mixed, netkey = METHOD[name]
+        if netkey and netkey not in nets:
+            continue
+        r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, apb_mixed if mixed else apb_plain,
+                       net=nets.get(netkey))
+        r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
+        if base["converged"]:
+            r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
+        res[name] = r
+        log(f"{name}: {r}")
+    # What the network gains over the same method without it.
+    for a, b in (("fp32_then_fp64_gta_learned", "fp32_then_fp64_gta"),
+                 ("fp32_then_fp64_gta2_learned", "fp32_then_fp64_gta2"),
+                 ("fp64_gta_learned", "fp64_gta")):
+        if a in res and b in res:
+            res[f"learned_gain_{b}"] = res[b]["seconds"] / res[a]["seconds"]
+    for net in nets.values():
+        net.close()
+    return res
+
+
+def learned_config4(tol):
+    """The learned correction on its own workload: 4x4x4 tiles, 2x2 product
+    quadrature (32 angles), 2 groups, sigma_t logU[0.1, 10], c U[0.95, 0.999],
+    q = 1, seed 20261206; trained on seeds 1-4 of the same distribution."""
+    import umt_amd as U
+
+    cfg = dict(sigt=(0.1, 10.0), c=(0.95, 0.999), q=(1.0, 1.0))
+    p = U.TiledProblem(4, 4, 4, 2, 2)
+    G = 2
+    t0 = time.perf_counter()
+    net, info = U.train_on_iterates(p, G, seeds=[1, 2, 3, 4], rounds=4, epochs=40,
+                                    graphs=GRAPHS[0], **cfg)
+    train_s = time.perf_counter() - t0
+    xs = U.draw_cross_sections(20261206, p.nzones, G, **cfg)
+    out = {"workload": "4x4x4 tiles (1536 zones), 32 angles, 2 groups, sigma_t logU[0.1,10], "
+                       "c U[0.95,0.999], q=1, seed 20261206",
+           "training_seconds": train_s, "training_losses": info["losses"]}
+    base, phi64 = solve(p, G, xs, "none", False, tol, 2000, {})
+    out["fp64_plain"] = base
+    for name, accel, mixed in (("fp64_learned", "learned", False),
+                               ("fp32_then_fp64_learned", "learned", True),
+                               ("fp64_gta", "gta", False)):
+        r, phi = solve(p, G, xs, accel, mixed, tol, 2000, {}, net=net)
+        r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
+        r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
+        out[name] = r
+    log(f"learned config 4: {out}")
+    net.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Other sections
+
+
+def dropin_rate(p, G, st, S, n):
     """The per-angle drop-in gpu_sweepucbxyz with host arrays: every call
     copies its inputs to the device and its outputs back, as the reference
     CUDA path does, so the rate is PCIe-bound. Not the benchmark's value."""
@@ -305,104 +507,11 @@ def dropin_rate(p, G, st, S, n, log):
     return r
 
 
-def time_to_converge(p, G, xs, tol, max_iters, learn_tiles, log, c_range=(0.0, 0.95),
-                     gta_only=False):
-    """Plain FP64 source iteration (the reference method on the GPU) against
-    GTA and the learned correction, FP64 and FP32->FP64."""
-    import umt_amd as U
-
-    res = {}
-    base, phi64 = solve(p, G, xs, "none", False, tol, max_iters, repeat=False, history=True)
-    hist = base.pop("history")
-    if not base["converged"] and len(hist) > 20:
-        # Not converged within max_iters: extrapolate the iterations (and the
-        # time) from the contraction over the last 20 iterations.
-        rate = (hist[-1] / hist[-21]) ** (1.0 / 20)
-        if 0.0 < rate < 1.0:
-            more = np.log(tol / hist[-1]) / np.log(rate)
-            base["estimated_iterations"] = float(len(hist) + more)
-            base["estimated_seconds"] = base["seconds"] * base["estimated_iterations"] / len(hist)
-            base["contraction_per_iteration"] = float(rate)
-    res["fp64_plain"] = base
-    log(f"fp64 plain: {base}")
-    base_s = base.get("estimated_seconds", base["seconds"])
-    cases = [("fp64_gta", "gta", False, None), ("fp32_then_fp64_gta", "gta", True, None)]
-    nets = []
-    if learn_tiles:
-        small = U.TiledProblem(learn_tiles, learn_tiles, learn_tiles, p.npolar, p.nazimuthal)
-        for key, on in ((("gta", "gta"),) if gta_only else (("plain", "none"), ("gta", "gta"))):
-            t0 = time.perf_counter()
-            net, info = U.train_on_iterates(small, G, seeds=[1, 2, 3], sigt=(1e-2, 1e3),
-                                            c=c_range, q=(0.0, 1.0), rounds=3, batch=8192,
-                                            steps=1500, collect_stride=4 if on == "none" else 1,
-                                            max_iters=max_iters, base=on, graphs=GRAPHS[0])
-            res[f"learned_training_{key}"] = {
-                "seconds": time.perf_counter() - t0,
-                "problem": f"{learn_tiles}^3 tiles, {p.nangles} angles, {G} groups, seeds 1-3 "
-                           f"(cross sections drawn like the workload's), iterates of "
-                           f"{'plain' if on == 'none' else 'GTA-accelerated'} source iteration",
-                "losses": info["losses"], "samples": info["samples"]}
-            log(f"training on {key} iterates: {res[f'learned_training_{key}']}")
-            nets.append(net)
-        if gta_only:
-            nets.insert(0, None)
-        if not gta_only:
-            cases += [("fp64_learned", "learned", False, nets[0]),
-                      ("fp32_then_fp64_learned", "learned", True, nets[0])]
-        cases += [("fp64_gta_learned", "gta+learned", False, nets[1]),
-                  ("fp32_then_fp64_gta_learned", "gta+learned", True, nets[1])]
-    ref_phi, ref_name = (phi64, "fp64_plain") if base["converged"] else (None, "fp64_gta")
-    for name, accel, mixed, net in cases:
-        r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, net=net)
-        r["speedup_vs_fp64_plain"] = base_s / r["seconds"]
-        if ref_phi is None and name == "fp64_gta" and r["converged"]:
-            ref_phi = phi  # plain iteration did not converge: FP64 GTA is the reference answer
-        if ref_phi is not None and name != ref_name:
-            r[f"max_rel_phi_error_vs_{ref_name}"] = rel_error(phi, ref_phi)
-        res[name] = r
-        log(f"{name}: {r}")
-    for net in nets:
-        if net is not None:
-            net.close()
-    return res
-
-
-def learned_config4(tol, log):
-    """The learned correction on its own workload: 4x4x4 tiles, 2x2 product
-    quadrature (32 angles), 2 groups, sigma_t logU[0.1, 10], c U[0.95, 0.999],
-    q = 1, seed 20261206; trained on seeds 1-4 of the same distribution."""
-    import umt_amd as U
-
-    cfg = dict(sigt=(0.1, 10.0), c=(0.95, 0.999), q=(1.0, 1.0))
-    p = U.TiledProblem(4, 4, 4, 2, 2)
-    G = 2
-    t0 = time.perf_counter()
-    net, info = U.train_on_iterates(p, G, seeds=[1, 2, 3, 4], rounds=4, epochs=40,
-                                    graphs=GRAPHS[0], **cfg)
-    train_s = time.perf_counter() - t0
-    xs = U.draw_cross_sections(20261206, p.nzones, G, **cfg)
-    out = {"workload": "4x4x4 tiles (1536 zones), 32 angles, 2 groups, sigma_t logU[0.1,10], "
-                       "c U[0.95,0.999], q=1, seed 20261206",
-           "training_seconds": train_s, "training_losses": info["losses"]}
-    base, phi64 = solve(p, G, xs, "none", False, tol, 2000)
-    out["fp64_plain"] = base
-    for name, accel, mixed in (("fp64_learned", "learned", False),
-                               ("fp32_then_fp64_learned", "learned", True),
-                               ("fp64_gta", "gta", False)):
-        r, phi = solve(p, G, xs, accel, mixed, tol, 2000, net=net)
-        r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
-        r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
-        out[name] = r
-    log(f"learned config 4: {out}")
-    net.close()
-    return out
-
-
-def batched_domains(n_domains, seed, log):
+def batched_domains(n_domains, seed, precisions=(32, 64)):
     """Uneven domains in one batch: n_domains tiled meshes of n^3 tiles,
     n ~ U{4..16}, quadrature 2x2, 3x3 or 4x4 per octant (32, 72, 128 angles),
-    G ~ U{8..64}, per-domain seeds, cross sections drawn like the workload's;
-    FP32 sweeps. Reports the batched sweep (autotuned groups per lane and
+    G ~ U{8..64}, per-domain seeds, cross sections drawn like the single
+    domain's. Per precision: the batched sweep (autotuned groups per lane and
     angles per block) against the same domains swept one context at a time."""
     import umt_amd as U
 
@@ -419,50 +528,139 @@ def batched_domains(n_domains, seed, log):
     groups = [g for _, _, g in spec]
     xs = [U.draw_cross_sections(seed + 1 + i, p.nzones, g, (1e-2, 1e3), (0.0, 0.95), (0.0, 1.0))
           for i, (p, g) in enumerate(zip(probs, groups))]
-    log(f"batch: {n_domains} problems built in {build_s:.1f} s")
-    # One context at a time (the same sweeps without batching).
-    seq_ms, seq_launches = 0.0, 0
-    for i, (p, g, (st, ss, qq)) in enumerate(zip(probs, groups, xs)):
-        if i % 16 == 15:
-            log(f"batch: swept {i + 1} of {n_domains} domains one at a time")
-        ctx = U.SweepContext(p, g, precision=32)
-        ctx.set_sigt(st)
-        ctx.set_source(np.repeat(qq, 8, axis=0) / (4 * np.pi))
-        k = 4 if g % 4 == 0 else (2 if g % 2 == 0 else 1)
-        ctx.set_groups_per_lane(k)
-        ctx.set_graphs(GRAPHS[0])
-        ctx.sweep_n(1)
-        t1 = time.perf_counter()
-        ctx.sweep_n(3)
-        seq_ms += (time.perf_counter() - t1) * 1e3 / 3
-        seq_launches += ctx.levels
-        ctx.close()
-    b = U.SweepBatch(probs, groups, precision=32)
-    b.set_graphs(GRAPHS[0])
-    for i, (st, ss, qq) in enumerate(xs):
-        b.set_cross_sections(i, st, ss, qq)
-    b.set_layout(4, 32)
-    K, n, ms_tuned = b.autotune(reps=2)
-    ms = b.time_iterations(1, 5)
-    d = b.dims()
     out = {
         "workload": (f"{n_domains} domains, n^3 tiles with n ~ U{{4..16}}, 32/72/128 angles, "
-                     f"G ~ U{{8..64}}, sigma_t logU[1e-2,1e3], c U[0,0.95], q U[0,1], FP32 sweeps, "
-                     f"seed {seed}"),
+                     f"G ~ U{{8..64}}, sigma_t logU[1e-2,1e3], c U[0,0.95], q U[0,1], seed {seed}"),
         "domains": n_domains, "zones_total": int(sum(p.nzones for p in probs)),
-        "unknowns_per_sweep": d["unknowns"],
-        "batched": {"ms_per_iteration": ms, "unknowns_per_s": d["unknowns"] / (ms * 1e-3),
-                    "launches_per_sweep": d["launches"], "groups_per_lane": K,
-                    "angles_per_block": n, "angle_blocks": d["blocks"]},
-        "one_context_at_a_time": {"ms_per_sweep_sum": seq_ms,
-                                  "unknowns_per_s": d["unknowns"] / (seq_ms * 1e-3),
-                                  "level_launches_sum": seq_launches},
-        "speedup_batched": seq_ms / ms,
         "problem_build_s": build_s,
     }
-    log(f"batched domains: {out}")
-    b.close()
+    for prec in precisions:
+        seq_ms, seq_launches = 0.0, 0
+        for i, (p, g, (st, ss, qq)) in enumerate(zip(probs, groups, xs)):
+            ctx = U.SweepContext(p, g, precision=prec)
+            ctx.set_sigt(st)
+            ctx.set_source(np.repeat(qq, 8, axis=0) / (4 * np.pi))
+            k = 4 if g % 4 == 0 else (2 if g % 2 == 0 else 1)
+            ctx.set_groups_per_lane(k)
+            ctx.set_graphs(GRAPHS[0])
+            ctx.sweep_n(1)
+            t1 = time.perf_counter()
+            ctx.sweep_n(3)
+            seq_ms += (time.perf_counter() - t1) * 1e3 / 3
+            seq_launches += ctx.levels
+            ctx.close()
+        b = U.SweepBatch(probs, groups, precision=prec)
+        b.set_graphs(GRAPHS[0])
+        for i, (st, ss, qq) in enumerate(xs):
+            b.set_cross_sections(i, st, ss, qq)
+        b.set_layout(4, 32)
+        K, n, ms_tuned = b.autotune(reps=2)
+        ms = b.time_iterations(1, 5)
+        d = b.dims()
+        key = "fp64" if prec == 64 else "fp32"
+        out["unknowns_per_sweep"] = d["unknowns"]
+        out[key] = {
+            "batched": {"ms_per_iteration": ms, "unknowns_per_s": d["unknowns"] / (ms * 1e-3),
+                        "launches_per_sweep": d["launches"], "groups_per_lane": K,
+                        "angles_per_block": n, "angle_blocks": d["blocks"]},
+            "one_context_at_a_time": {"ms_per_sweep_sum": seq_ms,
+                                      "unknowns_per_s": d["unknowns"] / (seq_ms * 1e-3),
+                                      "level_launches_sum": seq_launches},
+            "speedup_batched": seq_ms / ms,
+        }
+        log(f"batched domains FP{prec}: {out[key]}")
+        b.close()
     return out
+
+
+def rz_bytes_per_unknown(precision, n_in, G, start_frac):
+    """Bytes per unknown of the RZ level kernel: source read, sigma_t
+    shared by 4 corners, psi write, n_in upwind psi reads, half-angle flux
+    read (not for starting directions) and write, and the per-corner
+    geometry (A_fp, A_ez, RadiusFP, RadiusEZ, Area, Volume in the sweep
+    precision, cFP as two int32) shared by the G groups."""
+    r = 8 if precision == 64 else 4
+    geo = 15 * r + 8
+    return r + r / 4 + r + r * n_in + r * (1 - start_frac) + r + geo / G
+
+
+def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds):
+    """The 2D cylindrical sweep: FP64 and FP32 throughput, the level kernel
+    against the HBM roofline, and the RZ oracle timed on this host's cores."""
+    import umt_amd as U
+
+    p = U.RzProblem(nr, nz, npolar, nazimuthal, r0=0.0, r1=1.0, z0=0.0, z1=1.0)
+    q, g, m = p.quadrature(), p.geometry(), p.mesh()
+    swept = q["finishing"] == 0
+    unknowns = p.ncorner * int(swept.sum()) * G
+    start_frac = float(q["starting"].sum()) / float(swept.sum())
+    inner = m["cFP"] <= p.ncorner
+    cnt = 0
+    for a in np.where(swept)[0]:
+        d = np.einsum("cfk,k->cf", g["A_fp"], q["omega"][a])
+        cnt += int(np.count_nonzero((d < 0) & inner))
+    n_in = cnt / (p.ncorner * int(swept.sum()))
+    rng = np.random.default_rng(20261204)
+    sig = np.exp(rng.uniform(np.log(1e-2), np.log(1e3), (p.nzones, G)))
+    S = rng.uniform(0.0, 1.0, (p.ncorner, G)) / (2 * np.pi)
+    out = {"config": {"workload": (f"UCB RZ sweep, {nr}x{nz} quad zones on [0,1]x[0,1] "
+                                   f"({p.nzones} zones, {p.ncorner} corners), RZ product quadrature "
+                                   f"{npolar}x{nazimuthal} ({p.nangles} directions in {p.nlevels} xi "
+                                   f"levels, {int(swept.sum())} swept), {G} groups, sigma_t "
+                                   f"logU[1e-2,1e3], vacuum"),
+                      "unknowns_per_sweep": unknowns, "level_launches": p.sweep_levels},
+           "unit": "unknowns/s (corners x swept directions x groups per second)"}
+    for prec in (64, 32):
+        ctx = U.RzContext(p, G, precision=prec)
+        ctx.set_graphs(GRAPHS[0])
+        ctx.set_sigt(sig)
+        ctx.set_source(S)
+        K, ms_k = ctx.autotune(2)
+        for _ in range(warmup):
+            ctx.sweep()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.sweep()
+        dt = time.perf_counter() - t0
+        ms_ev, ms_lv = ctx.time_sweeps(1, 3)
+        ctx.close()
+        B = rz_bytes_per_unknown(prec, n_in, G, start_frac)
+        achieved = B * unknowns / (ms_lv * 1e-3) / 1e9
+        key = "fp64" if prec == 64 else "fp32"
+        out[key] = {"value": unknowns * steps / dt, "ms_per_step": dt * 1e3 / steps,
+                    "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                 "kernel": f"rz_sweep_level<{'double' if prec == 64 else 'float'},K>",
+                                 "level_kernels_ms_per_sweep": ms_lv,
+                                 "algorithmic_bytes_per_unknown": B, "n_in": n_in,
+                                 "groups_per_lane": K,
+                                 "autotune_ms_per_sweep": {str(k): v for k, v in ms_k.items()}}}
+        log(f"RZ FP{prec} sweep: {unknowns * steps / dt:.3e} unknowns/s, level kernels "
+            f"{ms_lv:.2f} ms ({achieved:.0f} GB/s), groups per lane {K} (autotune {ms_k})")
+    if cpu_seconds > 0:
+        from oracle import rz as R
+
+        Gc = 4
+        threads = max(1, min(cpu_threads(), p.nlevels))
+        s = p.schedule()
+        Sc, sc = np.ascontiguousarray(S[:, :Gc]), np.ascontiguousarray(sig[:, :Gc])
+        R.lib(fast=True)
+        reps, t_total = 0, 0.0
+        while t_total < cpu_seconds or reps == 0:
+            t0 = time.perf_counter()
+            R.sweep_all(m, g, q, s, Sc, sc, nthreads=threads, fast=True)
+            t_total += time.perf_counter() - t0
+            reps += 1
+        out["cpu_baseline"] = {
+            "value": p.ncorner * int(swept.sum()) * Gc * reps / t_total, "unit": "unknowns/s",
+            "cores": threads, "kind": "port",
+            "sample": (f"oracle C RZ sweep (liboracle_fast.so, OpenMP over xi levels, at most one "
+                       f"thread per level) of the same mesh and directions with {Gc} groups, {reps} "
+                       f"sweeps in {t_total:.1f} s on {threads} threads")}
+    return out
+
+
+# ---------------------------------------------------------------------------
 
 
 def main():
@@ -470,17 +668,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--tiles", type=int, nargs=3, default=[20, 20, 20])
-    ap.add_argument("--quad", type=int, nargs=2, default=[4, 4], help="npolar nazimuthal")
-    ap.add_argument("--groups", type=int, default=16)
-    ap.add_argument("--c", type=float, nargs=2, default=[0.0, 0.95],
-                    help="scattering ratio range of the workload")
-    ap.add_argument("--seed", type=int, default=SEED)
-    ap.add_argument("--angles-per-block", type=int, nargs=2, default=[0, 0],
-                    metavar=("FP64", "FP32"), help="sweep angle blocks (0: all angles at once)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large",
+                    help="headline workload: large (32^3 tiles, 32 groups) or single (20^3, 16)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the 20^3-tile section when the headline is the large workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-converge", action="store_true", help="skip time-to-convergence")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 sweep")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 sweeps")
+    ap.add_argument("--methods", default="",
+                    help="comma-separated time-to-convergence methods of the headline workload "
+                         "(default: fp64_plain, fp32_then_fp64_gta, fp32_then_fp64_gta_learned "
+                         "for large; all for single)")
     ap.add_argument("--dropin-angles", type=int, default=4,
                     help="angles timed through the per-angle drop-in with host arrays (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -490,9 +688,6 @@ def main():
                     help="uneven domains in the batched-sweep section (0: skip it)")
     ap.add_argument("--learn-tiles", type=int, default=4,
                     help="train the learned correction on n^3-tile problems (0: skip it)")
-    ap.add_argument("--learned-with-gta-only", action="store_true",
-                    help="run the learned correction only on top of GTA (for workloads where "
-                         "iteration without GTA does not converge within --max-iters)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every sweep eagerly instead of replaying captured graphs")
     ap.add_argument("--rz", type=int, nargs=2, default=[512, 512],
@@ -511,17 +706,12 @@ def main():
     import umt_amd as U
     from umt_amd import _lib
 
-    def log(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-    lib = _lib.load()
+    _lib.load()
+    GRAPHS[0] = not args.no_graphs
     if args.only_rz:
-        GRAPHS[0] = not args.no_graphs
-        print(json.dumps({"rz_sweep": rz_sweep(args.rz[0], args.rz[1], 4, 4, args.groups,
-                                               args.steps, args.warmup,
-                                               0 if args.no_cpu else 8.0, log)}))
+        print(json.dumps({"rz_sweep": rz_sweep(args.rz[0], args.rz[1], 4, 4, 16, args.steps,
+                                               args.warmup, 0 if args.no_cpu else 8.0)}))
         return
-
     if args.calibrate:
         n = 1 << 27
         ms = U.sweep.hbm_copy_probe(n, 5)
@@ -529,113 +719,113 @@ def main():
                           "ms_per_copy": ms, "GBps": 2 * n * 8 / ms / 1e6}))
         return
 
-    nx, ny, nz = args.tiles
-    npol, nazi = args.quad
-    G = args.groups
+    sections_s = {}
+    wl = WORKLOADS[args.workload]
+    tiles, (npol, nazi), G, c_range, seed = wl["tiles"], wl["quad"], wl["groups"], wl["c"], wl["seed"]
     t0 = time.perf_counter()
-    p = U.TiledProblem(nx, ny, nz, npol, nazi)
-    setup_s = time.perf_counter() - t0
-    c_range = tuple(args.c)
-    APB[64], APB[32] = args.angles_per_block
-    GRAPHS[0] = not args.no_graphs
-    st, ss, qq = U.draw_cross_sections(args.seed, p.nzones, G, (1e-2, 1e3), c_range, (0.0, 1.0))
+    p = U.TiledProblem(*tiles, npol, nazi)
+    st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (1e-2, 1e3), c_range, (0.0, 1.0))
     S = np.repeat(qq, 8, axis=0) / (4.0 * np.pi)
+    setup_s = time.perf_counter() - t0
     unknowns = p.unknowns(G)
+    log(f"headline problem built in {setup_s:.1f} s: {p.nzones} zones, {p.nangles} angles, {G} groups")
 
-    log(f"problem built in {setup_s:.1f} s: {p.nzones} zones, {p.nangles} angles, {G} groups")
-    r64 = measure(p, G, 64, st, S, args.warmup, args.steps, lib)
-    value = unknowns * args.steps / r64["dt"]
-    log(f"FP64 sweep: {value:.3e} unknowns/s, {r64['dt'] * 1e3 / args.steps:.2f} ms per sweep")
-    B64 = bytes_per_unknown(64, r64["n_in"], G)
-    ach64 = B64 * unknowns / (r64["ms_levels"] * 1e-3) / 1e9
-    # Measured HBM traffic is on file for the default workload only.
-    profiled = (nx, ny, nz, G, p.nangles) == (20, 20, 20, 16, 128)
-    tr = load_traffic("ucb_sweep_level_fp64_20x20x20_traffic.json") if profiled else None
-    traffic = tr.get("hbm_bytes_per_launch") if tr else None
-    roofline = {
-        "bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": ach64 / HBM_PEAK_GBS, "traffic": traffic,
-        "kernel": "ucb_sweep_level<double,K,false>",
-        "launches_per_sweep": r64["launches"],
-        "mean_launch_ms": r64["ms_levels"] / r64["launches"],
-        "algorithmic_bytes_per_unknown": B64, "n_in": r64["n_in"],
-        "unknowns_per_launch": unknowns / r64["launches"],
-        "groups_per_lane": r64["groups_per_lane"],
-        "autotune_ms_per_sweep": r64["autotune_ms_per_sweep"],
-    }
+    t0 = time.perf_counter()
+    sw = sweep_section(p, G, st, S, args, tiles, args.workload)
+    sections_s["sweeps"] = time.perf_counter() - t0
+    r64 = sw["fp64"]
+    tstr = "x".join(str(v) for v in tiles)
     out = {
         "metric": "sweep unknowns/s (zone-corner x angle x group); wall time to converged phi at 1e-8",
-        "value": value,
+        "value": r64["value"],
         "unit": "unknowns/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r64["dt"] * 1e3 / args.steps,
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded tiled hex mesh and cross sections)",
         "config": {
-            "workload": (f"UCB xyz sweep, {nx}x{ny}x{nz} tiles ({p.nzones} zones, {p.ncorner} "
-                         f"corners), {npol}x{nazi} product quadrature per octant ({p.nangles} "
-                         f"angles), {G} groups, sigma_t logU[1e-2,1e3], c U[{c_range[0]},{c_range[1]}], "
-                         f"q U[0,1], seed {args.seed}, "
-                         f"vacuum"),
-            "tiles": [nx, ny, nz], "zones": p.nzones, "corners": p.ncorner,
-            "angles": p.nangles, "groups": G, "unknowns_per_sweep": unknowns,
-            "parallelism": "1 GPU",
+            "workload": (f"UCB xyz sweep, {tstr} tiles ({p.nzones} zones, {p.ncorner} corners), "
+                         f"{npol}x{nazi} product quadrature per octant ({p.nangles} angles), {G} "
+                         f"groups, sigma_t logU[1e-2,1e3], c U[{c_range[0]},{c_range[1]}], q U[0,1], "
+                         f"seed {seed}, vacuum"),
+            "tiles": list(tiles), "zones": p.nzones, "corners": p.ncorner, "angles": p.nangles,
+            "groups": G, "unknowns_per_sweep": unknowns, "parallelism": "1 GPU",
         },
-        "roofline": roofline,
+        "roofline": r64["roofline"],
         "setup_s": setup_s,
     }
-
-    if not args.no_fp32:
-        r32 = measure(p, G, 32, st, S, args.warmup, args.steps, lib)
-        log(f"FP32 sweep: {unknowns * args.steps / r32['dt']:.3e} unknowns/s")
-        B32 = bytes_per_unknown(32, r32["n_in"], G)
-        ach32 = B32 * unknowns / (r32["ms_levels"] * 1e-3) / 1e9
-        tr32 = load_traffic("ucb_sweep_level_fp32_20x20x20_traffic.json") if profiled else None
-        traffic32 = tr32.get("hbm_bytes_per_launch") if tr32 else None
-        ms32 = r32["ms_levels"] / r32["launches"]
-        out["fp32_sweep"] = {
-            "value": unknowns * args.steps / r32["dt"], "unit": "unknowns/s",
-            "ms_per_step": r32["dt"] * 1e3 / args.steps,
-            # frac can exceed 1: the algorithmic bytes re-read geometry and S
-            # for every angle, and L2/MALL serve most of those re-reads.
-            "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic32,
-                         "measured_hbm_frac": (traffic32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS
-                                               if traffic32 else None),
-                         "kernel": "ucb_sweep_level<float,K,false>",
-                         "mean_launch_ms": ms32,
-                         "algorithmic_bytes_per_unknown": B32,
-                         "groups_per_lane": r32["groups_per_lane"]},
-        }
-
-    if args.dropin_angles > 0:
-        out["dropin_pcie"] = dropin_rate(p, G, st, S, args.dropin_angles, log)
+    if "fp32" in sw:
+        out["fp32_sweep"] = sw["fp32"]
+    out["device_memory_bytes"] = sw["device_memory_bytes"]
 
     if not args.no_converge:
-        out["time_to_converge"] = time_to_converge(
-            p, G, (st, ss, qq), args.tol, args.max_iters, args.learn_tiles, log, c_range,
-            args.learned_with_gta_only)
-
-        if args.learn_tiles and not args.learned_with_gta_only:
-            out["learned_config4"] = learned_config4(args.tol, log)
-
-    if args.batch_domains:
-        out["batched_domains"] = batched_domains(args.batch_domains, 20261205, log)
-
-    if args.rz[0] > 0:
-        out["rz_sweep"] = rz_sweep(args.rz[0], args.rz[1], 4, 4, G, args.steps, args.warmup,
-                                   0 if args.no_cpu else 8.0, log)
+        t0 = time.perf_counter()
+        if args.methods:
+            methods = [m.strip() for m in args.methods.split(",") if m.strip()]
+        elif args.workload == "large":
+            methods = ["fp64_plain", "fp32_then_fp64_gta", "fp32_then_fp64_gta2",
+                       "fp32_then_fp64_gta2_learned"]
+        else:
+            methods = ALL_METHODS
+        out["time_to_converge"] = time_to_converge(p, G, (st, ss, qq), args.tol, args.max_iters,
+                                                   methods, args.learn_tiles, c_range)
+        sections_s["time_to_converge"] = time.perf_counter() - t0
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, G, st, S, args.cpu_seconds)
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(p, G, st, S, args.cpu_seconds, host_info())
+        sections_s["cpu_baseline"] = time.perf_counter() - t0
     else:
         out["cpu_baseline"] = None
+    del p, st, ss, qq, S
 
+    if args.workload == "large" and not args.no_secondary:
+        t0 = time.perf_counter()
+        w2 = WORKLOADS["single"]
+        p2 = U.TiledProblem(*w2["tiles"], *w2["quad"])
+        G2 = w2["groups"]
+        st2, ss2, qq2 = U.draw_cross_sections(w2["seed"], p2.nzones, G2, (1e-2, 1e3), w2["c"],
+                                              (0.0, 1.0))
+        S2 = np.repeat(qq2, 8, axis=0) / (4.0 * np.pi)
+        sec = {"workload": (f"UCB xyz sweep, 20x20x20 tiles ({p2.nzones} zones, {p2.ncorner} "
+                            f"corners), 4x4 product quadrature per octant ({p2.nangles} angles), "
+                            f"{G2} groups, sigma_t logU[1e-2,1e3], c U[0,0.95], q U[0,1], seed "
+                            f"{w2['seed']}, vacuum"),
+               "unknowns_per_sweep": p2.unknowns(G2)}
+        sec.update(sweep_section(p2, G2, st2, S2, args, w2["tiles"], "single"))
+        if args.dropin_angles > 0:
+            sec["dropin_pcie"] = dropin_rate(p2, G2, st2, S2, args.dropin_angles)
+        if not args.no_converge:
+            sec["time_to_converge"] = time_to_converge(p2, G2, (st2, ss2, qq2), args.tol,
+                                                       args.max_iters, ALL_METHODS,
+                                                       args.learn_tiles, w2["c"])
+        out["single_domain_20x20x20"] = sec
+        sections_s["single_domain_20x20x20"] = time.perf_counter() - t0
+        del p2
+
+    if not args.no_converge and args.learn_tiles:
+        t0 = time.perf_counter()
+        out["learned_config4"] = learned_config4(args.tol)
+        sections_s["learned_config4"] = time.perf_counter() - t0
+
+    if args.batch_domains:
+        t0 = time.perf_counter()
+        out["batched_domains"] = batched_domains(args.batch_domains, 20261205,
+                                                 (32,) if args.no_fp32 else (32, 64))
+        sections_s["batched_domains"] = time.perf_counter() - t0
+
+    if args.rz[0] > 0:
+        t0 = time.perf_counter()
+        out["rz_sweep"] = rz_sweep(args.rz[0], args.rz[1], 4, 4, 16, args.steps, args.warmup,
+                                   0 if args.no_cpu else 8.0)
+        sections_s["rz_sweep"] = time.perf_counter() - t0
+
+    out["section_seconds"] = sections_s
     print(json.dumps(out))
 
 

@@ -114,3 +114,78 @@ def test_batch_fp32_and_autotuner(domains):
         ref, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st)
         assert rel_err(b.phi(i), ref) < 2e-5, i
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# Batches drawn like the benchmark's 64 uneven domains: n^3 tiles, 2x2, 3x3
+# or 4x4 product quadrature per octant (32, 72 or 128 angles), G ~ U{8..64}
+# (most group counts need padding to a multiple of 4), cross sections
+# sigma_t logU[1e-2, 1e3], c U[0, 0.95], q U[0, 1]. Groups are independent
+# in a sweep, so the oracle runs each domain on three of its groups (first,
+# middle, last: the last sits next to the padding) with every angle.
+
+
+def _draw(n_domains, seed, n_range):
+    rng = np.random.default_rng(seed)
+    spec = [(int(rng.integers(n_range[0], n_range[1] + 1)), int(rng.choice([2, 3, 4])),
+             int(rng.integers(8, 65))) for _ in range(n_domains)]
+    out = []
+    for i, (n, a, G) in enumerate(spec):
+        p = U.TiledProblem(n, n, n, a, a)
+        out.append((p, G, U.draw_cross_sections(seed + 1 + i, p.nzones, G, **XS)))
+    return out
+
+
+def _check_against_oracle(b, doms, which, tol):
+    for i in which:
+        p, G, (st, ss, qq) = doms[i]
+        sel = sorted({0, G // 2, G - 1})
+        S = np.repeat(qq, 8, axis=0) / (4 * np.pi)
+        ref, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(),
+                             np.ascontiguousarray(S[:, sel]), np.ascontiguousarray(st[:, sel]),
+                             nthreads=THREADS, fast=True)
+        err = rel_err(b.phi(i)[:, sel], ref)
+        assert err < tol, (i, p.nzones, p.nangles, G, err)
+
+
+THREADS = 16
+
+
+@pytest.mark.parametrize("precision,layout", [(64, (4, 32)), (64, (2, 128)), (32, "auto")])
+def test_config3_distribution_matches_oracle(precision, layout):
+    """16 domains with n in 4..8 tiles, every domain against the oracle."""
+    doms = _draw(16, 20261215, (4, 8))
+    print(f"case config-3 distribution FP{precision} layout {layout}: "
+          f"{[(d[0].nzones, d[0].nangles, d[1]) for d in doms]}", flush=True)
+    b = make_batch(doms, precision=precision)
+    if layout == "auto":
+        K, n, ms = b.autotune(reps=1)
+        print(f"autotuned K={K} angles per block {n}", flush=True)
+    else:
+        b.set_layout(*layout)
+    b.sweep()
+    _check_against_oracle(b, doms, range(len(doms)), 1e-12 if precision == 64 else 2e-5)
+    b.close()
+
+
+def test_config3_full_batch_fp32():
+    """The benchmark's 64-domain batch itself (n in 4..16 tiles, seed
+    20261205: 1.98e6 zones, 4.2e10 unknowns per sweep), FP32 with the benchmark's first layout
+    (4 groups per lane, 32 angles per block): one sweep from phi = 0, eight
+    domains (the largest, the smallest, and six more spread over the list)
+    against the oracle."""
+    rng = np.random.default_rng(20261205)
+    spec = [(int(rng.integers(4, 17)), int(rng.choice([2, 3, 4])), int(rng.integers(8, 65)))
+            for _ in range(64)]
+    doms = []
+    for i, (n, a, G) in enumerate(spec):
+        p = U.TiledProblem(n, n, n, a, a)
+        doms.append((p, G, U.draw_cross_sections(20261205 + 1 + i, p.nzones, G, **XS)))
+    print(f"case config-3 full batch: {sum(d[0].nzones for d in doms)} zones", flush=True)
+    b = make_batch(doms, precision=32)
+    b.set_layout(4, 32)
+    b.sweep()
+    sizes = [d[0].nzones * d[0].nangles for d in doms]
+    which = sorted({int(np.argmax(sizes)), int(np.argmin(sizes)), 5, 17, 29, 41, 53, 63})
+    _check_against_oracle(b, doms, which, 2e-5)
+    b.close()
