@@ -190,11 +190,18 @@ def psi_bytes(p, G, precision, n_angles):
     return (p.ncorner + p.nbelem) * G * n_angles * (8 if precision == 64 else 4)
 
 
-def context_extra_bytes(p, G, precision):
+def context_extra_bytes(p, G, precision, accel="none"):
     """Device memory of a context besides its S_N psi: fields, GTA vectors
-    and the S2 psi (8 directions)."""
+    and the S2 psi (8 directions); with GTA kernel 2 its transfer matrices
+    (8 FP64 values per corner and group), with the learned correction its
+    per-(zone, group) buffers (8 FP64 values and 16 FP32 features)."""
     rs = 8 if precision == 64 else 4
-    return 16 * p.ncorner * G * 8 + (p.ncorner + p.nbelem) * G * 8 * rs + 2 * 10**9
+    extra = 16 * p.ncorner * G * 8 + (p.ncorner + p.nbelem) * G * 8 * rs + 2 * 10**9
+    if "gta2" in accel:
+        extra += 8 * p.ncorner * G * 8
+    if "learned" in accel:
+        extra += p.nzones * G * (8 * 8 + 16 * 4)
+    return extra
 
 
 def angles_per_block(p, G, precision, budget):
@@ -366,6 +373,12 @@ def rel_error(phi, ref):
 ALL_METHODS = ["fp64_plain", "fp64_gta", "fp64_gta2", "fp32_then_fp64_gta", "fp32_then_fp64_gta2",
                "fp64_learned", "fp32_then_fp64_learned", "fp64_gta_learned",
                "fp32_then_fp64_gta_learned", "fp32_then_fp64_gta2_learned"]
+# The large workload's default: plain FP64 (run to convergence), and
+# FP32-then-FP64 with GTA kernel 2 with and without the learned correction.
+LARGE_METHODS = ["fp64_plain", "fp32_then_fp64_gta2", "fp32_then_fp64_gta2_learned"]
+# The 20^3-tile workload's default.
+SINGLE_METHODS = ["fp64_plain", "fp64_gta2", "fp32_then_fp64_gta", "fp32_then_fp64_gta2",
+                  "fp32_then_fp64_learned", "fp32_then_fp64_gta2_learned"]
 # name: (acceleration, mixed, network). gta is GTA kernel 1 (grey and
 # stretched-TSA sweeps), gta2 kernel 2 (transfer matrices and zone LU).
 METHOD = {
@@ -388,21 +401,35 @@ def time_to_converge(p, G, xs, tol, max_iters, methods, learn_tiles, c_range):
     import umt_amd as U
 
     free, _ = U.sweep.device_memory()
-    # FP64 psi of the mixed solves shares the device with the FP32 context.
-    # FP32 psi of every angle when that leaves room for half the angles' FP64 psi.
-    budget32 = free - context_extra_bytes(p, G, 32) - context_extra_bytes(p, G, 64) - 8 * 10**9
-    apb32 = angles_per_block(p, G, 32, budget32 - psi_bytes(p, G, 64, (p.nangles + 1) // 2))
-    left = budget32 - psi_bytes(p, G, 32, apb32 or p.nangles)
-    apb_mixed = {32: apb32, 64: angles_per_block(p, G, 64, left)}
-    apb_plain = {64: angles_per_block(p, G, 64, free - context_extra_bytes(p, G, 64) - 4 * 10**9)}
-    res = {"angles_per_block": {"fp64_alone": apb_plain[64] or p.nangles,
-                                "mixed_fp64": apb_mixed[64] or p.nangles,
-                                "mixed_fp32": apb_mixed[32] or p.nangles},
-           "timing": "each solve timed after a 2-iteration warm-up solve of the same contexts "
-                     "(graph capture), from phi = 0"}
-    base, phi64 = solve(p, G, xs, "none", False, tol, max_iters, apb_plain)
-    res["fp64_plain"] = base
-    log(f"fp64_plain: {base}")
+
+    def blocks(accel):
+        """Angles per block of the plain and the mixed solves with `accel`.
+        FP64 psi of the mixed solves shares the device with the FP32 context:
+        FP32 psi of every angle when that leaves room for half the angles'
+        FP64 psi."""
+        x32, x64 = context_extra_bytes(p, G, 32, accel), context_extra_bytes(p, G, 64, accel)
+        budget32 = free - x32 - x64 - 8 * 10**9
+        apb32 = angles_per_block(p, G, 32, budget32 - psi_bytes(p, G, 64, (p.nangles + 1) // 2))
+        left = budget32 - psi_bytes(p, G, 32, apb32 or p.nangles)
+        return ({64: angles_per_block(p, G, 64, free - x64 - 4 * 10**9)},
+                {32: apb32, 64: angles_per_block(p, G, 64, left)})
+
+    res = {"angles_per_block": {}, "timing": "each solve timed after a 2-iteration warm-up solve "
+                                             "of the same contexts (graph capture), from phi = 0"}
+
+    def note_blocks(name, accel, mixed):
+        plain, mix = blocks(accel)
+        res["angles_per_block"][name] = (
+            {"fp32": mix[32] or p.nangles, "fp64": mix[64] or p.nangles} if mixed
+            else {"fp64": plain[64] or p.nangles})
+        return mix if mixed else plain
+
+    base, phi64 = None, None
+    if "fp64_plain" in methods:
+        base, phi64 = solve(p, G, xs, "none", False, tol, max_iters,
+                            note_blocks("fp64_plain", "none", False))
+        res["fp64_plain"] = base
+        log(f"fp64_plain: {base}")
     nets = {}
     need = {METHOD[m][2] for m in methods if METHOD[m][2]}
     if learn_tiles:
@@ -430,11 +457,12 @@ def time_to_converge(p, G, xs, tol, max_iters, methods, learn_tiles, c_range):
         accel, mixed, netkey = METHOD[name]
         if netkey and netkey not in nets:
             continue
-        r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, apb_mixed if mixed else apb_plain,
+        r, phi = solve(p, G, xs, accel, mixed, tol, max_iters, note_blocks(name, accel, mixed),
                        net=nets.get(netkey))
-        r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
-        if base["converged"]:
-            r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
+        if base is not None:
+            r["speedup_vs_fp64_plain"] = base["seconds"] / r["seconds"]
+            if base["converged"]:
+                r["max_rel_phi_error_vs_fp64_plain"] = rel_error(phi, phi64)
         res[name] = r
         log(f"{name}: {r}")
     # What the network gains over the same method without it.
@@ -511,8 +539,10 @@ def batched_domains(n_domains, seed, precisions=(32, 64)):
     """Uneven domains in one batch: n_domains tiled meshes of n^3 tiles,
     n ~ U{4..16}, quadrature 2x2, 3x3 or 4x4 per octant (32, 72, 128 angles),
     G ~ U{8..64}, per-domain seeds, cross sections drawn like the single
-    domain's. Per precision: the batched sweep (autotuned groups per lane and
-    angles per block) against the same domains swept one context at a time."""
+    domain's. The first precision: the batched sweep (autotuned groups per
+    lane and angles per block) against the same domains swept one context at
+    a time; the others: the batched sweep with the first one's groups per
+    lane and blocks of 32 angles."""
     import umt_amd as U
 
     rng = np.random.default_rng(seed)
@@ -534,9 +564,11 @@ def batched_domains(n_domains, seed, precisions=(32, 64)):
         "domains": n_domains, "zones_total": int(sum(p.nzones for p in probs)),
         "problem_build_s": build_s,
     }
+    K32 = None
     for prec in precisions:
         seq_ms, seq_launches = 0.0, 0
-        for i, (p, g, (st, ss, qq)) in enumerate(zip(probs, groups, xs)):
+        # One context at a time is timed for the first precision only.
+        for i, (p, g, (st, ss, qq)) in enumerate(zip(probs, groups, xs) if K32 is None else []):
             ctx = U.SweepContext(p, g, precision=prec)
             ctx.set_sigt(st)
             ctx.set_source(np.repeat(qq, 8, axis=0) / (4 * np.pi))
@@ -553,8 +585,15 @@ def batched_domains(n_domains, seed, precisions=(32, 64)):
         b.set_graphs(GRAPHS[0])
         for i, (st, ss, qq) in enumerate(xs):
             b.set_cross_sections(i, st, ss, qq)
-        b.set_layout(4, 32)
-        K, n, ms_tuned = b.autotune(reps=2)
+        if K32 is None:
+            b.set_layout(4, 32)
+            K, n, ms_tuned = b.autotune(reps=2)
+            K32 = K
+        else:
+            # The layout the autotuner chose for the first precision, with
+            # angle blocks of 32 (the FP64 psi of every angle does not fit).
+            K, n = K32, 32
+            b.set_layout(K, n)
         ms = b.time_iterations(1, 5)
         d = b.dims()
         key = "fp64" if prec == 64 else "fp32"
@@ -563,11 +602,12 @@ def batched_domains(n_domains, seed, precisions=(32, 64)):
             "batched": {"ms_per_iteration": ms, "unknowns_per_s": d["unknowns"] / (ms * 1e-3),
                         "launches_per_sweep": d["launches"], "groups_per_lane": K,
                         "angles_per_block": n, "angle_blocks": d["blocks"]},
-            "one_context_at_a_time": {"ms_per_sweep_sum": seq_ms,
-                                      "unknowns_per_s": d["unknowns"] / (seq_ms * 1e-3),
-                                      "level_launches_sum": seq_launches},
-            "speedup_batched": seq_ms / ms,
         }
+        if seq_ms > 0:
+            out[key]["one_context_at_a_time"] = {"ms_per_sweep_sum": seq_ms,
+                                                 "unknowns_per_s": d["unknowns"] / (seq_ms * 1e-3),
+                                                 "level_launches_sum": seq_launches}
+            out[key]["speedup_batched"] = seq_ms / ms
         log(f"batched domains FP{prec}: {out[key]}")
         b.close()
     return out
@@ -584,7 +624,7 @@ def rz_bytes_per_unknown(precision, n_in, G, start_frac):
     return r + r / 4 + r + r * n_in + r * (1 - start_frac) + r + geo / G
 
 
-def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds):
+def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds, tiles=False):
     """The 2D cylindrical sweep: FP64 and FP32 throughput, the level kernel
     against the HBM roofline, and the RZ oracle timed on this host's cores."""
     import umt_amd as U
@@ -613,6 +653,8 @@ def rz_sweep(nr, nz, npolar, nazimuthal, G, steps, warmup, cpu_seconds):
     for prec in (64, 32):
         ctx = U.RzContext(p, G, precision=prec)
         ctx.set_graphs(GRAPHS[0])
+        if tiles:
+            ctx.set_tiles(True)
         ctx.set_sigt(sig)
         ctx.set_source(S)
         K, ms_k = ctx.autotune(2)
@@ -694,6 +736,8 @@ def main():
                     help="RZ sweep section: nr nz quad zones (0 0: skip it)")
     ap.add_argument("--only-rz", action="store_true",
                     help="run only the RZ sweep section and print its JSON object")
+    ap.add_argument("--rz-tiles", action="store_true",
+                    help="RZ sweeps with the tile levels (fewer, larger launches) instead of zone levels")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the HBM copy probe (for rocprofv3 --pmc calibration)")
     args = ap.parse_args()
@@ -710,7 +754,8 @@ def main():
     GRAPHS[0] = not args.no_graphs
     if args.only_rz:
         print(json.dumps({"rz_sweep": rz_sweep(args.rz[0], args.rz[1], 4, 4, 16, args.steps,
-                                               args.warmup, 0 if args.no_cpu else 8.0)}))
+                                               args.warmup, 0 if args.no_cpu else 8.0,
+                                               args.rz_tiles)}))
         return
     if args.calibrate:
         n = 1 << 27
@@ -768,8 +813,7 @@ def main():
         if args.methods:
             methods = [m.strip() for m in args.methods.split(",") if m.strip()]
         elif args.workload == "large":
-            methods = ["fp64_plain", "fp32_then_fp64_gta", "fp32_then_fp64_gta2",
-                       "fp32_then_fp64_gta2_learned"]
+            methods = LARGE_METHODS
         else:
             methods = ALL_METHODS
         out["time_to_converge"] = time_to_converge(p, G, (st, ss, qq), args.tol, args.max_iters,
@@ -802,7 +846,7 @@ def main():
             sec["dropin_pcie"] = dropin_rate(p2, G2, st2, S2, args.dropin_angles)
         if not args.no_converge:
             sec["time_to_converge"] = time_to_converge(p2, G2, (st2, ss2, qq2), args.tol,
-                                                       args.max_iters, ALL_METHODS,
+                                                       args.max_iters, SINGLE_METHODS,
                                                        args.learn_tiles, w2["c"])
         out["single_domain_20x20x20"] = sec
         sections_s["single_domain_20x20x20"] = time.perf_counter() - t0

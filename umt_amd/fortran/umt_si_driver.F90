@@ -185,7 +185,7 @@ program umt_si_driver
     block
       type(SetData), target :: Set
       integer(c_int) :: totalCycles
-      integer        :: a, b, g, h, ncl
+      integer        :: ia, ib, ig, ih, ncl
       allocate(Geom% numCorner(Z), Geom% cOffSet(Z), Geom% nCFacesArray(C), Geom% cFP(3,C), &
                Geom% cEZ(3,C), Geom% Volume(C), Geom% A_fp(3,3,C), Geom% A_ez(3,3,C))
       Geom% numCorner = numCorner; Geom% cOffSet = cOffSet; Geom% nCFacesArray = nCFaces
@@ -201,12 +201,12 @@ program umt_si_driver
         ASet% omega = omega; ASet% weight = weight; ASet% nextZ = nextZ; ASet% nextC = nextC
         ASet% nHyperPlanes = nHyperPlanes
         start = 0
-        do a = 1, A
-          allocate(ASet% HypPlanePtr(a)% zonesInPlane(nHyperPlanes(a)))
-          do h = 1, nHyperPlanes(a)
-            ASet% HypPlanePtr(a)% zonesInPlane(h) = zonesInPlane(start + h)
+        do ia = 1, A
+          allocate(ASet% HypPlanePtr(ia)% zonesInPlane(nHyperPlanes(ia)))
+          do ih = 1, nHyperPlanes(ia)
+            ASet% HypPlanePtr(ia)% zonesInPlane(ih) = zonesInPlane(start + ih)
           enddo
-          start = start + nHyperPlanes(a)
+          start = start + nHyperPlanes(ia)
         enddo
         ! cycle lists of the schedule (rt/rtorder.F90:62-90 offsets)
         rc = umt_problem_cycles(prob, totalCycles, c_loc(ASet% numCycles), c_null_ptr)
@@ -217,8 +217,8 @@ program umt_si_driver
         rc = umt_problem_cycles(prob, totalCycles, c_loc(ASet% numCycles), c_loc(ASet% cycleList))
         call check(rc, "umt_problem_cycles")
         ASet% cycleOffSet(1) = 0
-        do a = 2, A
-          ASet% cycleOffSet(a) = ASet% cycleOffSet(a-1) + ASet% numCycles(a-1)
+        do ia = 2, A
+          ASet% cycleOffSet(ia) = ASet% cycleOffSet(ia-1) + ASet% numCycles(ia-1)
         enddo
         allocate(GSet% STotal(G,C), GSet% Sigt(G,Z))
         do i = 1, C
@@ -229,15 +229,15 @@ program umt_si_driver
         allocate(Set% Psi(G,C,A), Set% Psi1(G,C+nb), Set% PsiB(G,nb,A), Set% Phi(G,C), &
                  Set% cyclePsi(G,ncl))
         Set% Psi = 0.0d0; Set% Psi1 = 0.0d0; Set% Phi = 0.0d0; Set% cyclePsi = 0.0d0
-        do a = 1, A
-          do b = 1, nb
-            do g = 1, G
-              Set% PsiB(g,b,a) = 0.01d0 * (1 + mod(g + 3*b + 7*a, 11))
+        do ia = 1, A
+          do ib = 1, nb
+            do ig = 1, G
+              Set% PsiB(ig,ib,ia) = 0.01d0 * (1 + mod(ig + 3*ib + 7*ia, 11))
             enddo
           enddo
         enddo
-        do a = 1, A
-          call SweepUCBxyz(Set, 1, G, a, logical(.true., kind=1))
+        do ia = 1, A
+          call SweepUCBxyz(Set, 1, G, ia, logical(.true., kind=1))
         enddo
         Phi = Set% Phi
         allocate(Psi3(G, nb*A + C*A, 1))
