@@ -11,6 +11,7 @@
 #ifndef UMT_SWEEP_H
 #define UMT_SWEEP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -45,6 +46,31 @@ void gpu_sweepucbxyz(int *Angle, int *nHyperPlanes, int *nZonesInPlane, int *nex
                      int *totalStreams, int *savePsi, int *numCycles, int *cycleOffSet,
                      double *cyclePsi, int *cycleList, int *b0, int *nBdyElem, double *PsiBMref,
                      int *Mref, int *Geom_numCorner, int *Geom_cOffSet);
+
+/* The same sweep as gpu_sweepucbxyz, same arguments and results, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in
+ * the reference. It allocates nothing: every device buffer of the call is
+ * carved from one static device array of gpu_sweepucbxyz_noalloc_max_bytes()
+ * bytes (128 MiB) that each GPU holds until the process ends; calls whose
+ * buffers would not fit are refused (status 53). It runs on the null stream
+ * and its outputs are complete when it returns. Every call on one GPU shares
+ * that array, which keeps the last call's contents (each call clears what it
+ * accumulates into). Calls that overlapped on one GPU would race on the
+ * array: the library's drop-in lock serialises them within a process (each
+ * process holds its own copy of the array). */
+void gpu_sweepucbxyz_noalloc(int *Angle, int *nHyperPlanes, int *nZonesInPlane, int *nextZ,
+                             int *nextC, double *STotal, double *tau, double *Psi, int *Groups,
+                             double *Volume, double *Sigt, int *nCFacesArray, int *ndim,
+                             int *maxcf, int *ncorner, double *A_fp, double *omega, int *cFP,
+                             double *Psi1, int *nbelem, double *A_ez, int *cEZ, int *NumAngles,
+                             double *quadwt, double *Phi, double *PsiB, int *maxCorner,
+                             int *mem0solve1, int *streamIdPtr, int *totalStreams, int *savePsi,
+                             int *numCycles, int *cycleOffSet, double *cyclePsi, int *cycleList,
+                             int *b0, int *nBdyElem, double *PsiBMref, int *Mref,
+                             int *Geom_numCorner, int *Geom_cOffSet);
+/* Bytes of the no-alloc drop-in's device scratch. */
+size_t gpu_sweepucbxyz_noalloc_max_bytes(void);
 
 /* Replaces gpu_streamsynchronize, GPU_SweepUCBxyz.cu:520-523. */
 void gpu_streamsynchronize(int *streamId);

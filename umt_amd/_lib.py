@@ -28,6 +28,11 @@ SIGNATURES = {
                                _pi, _pi, _pd, _pd, _pi, _pd, _pi, _pd, _pi, _pi, _pd, _pd, _pd,
                                _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pi, _pi, _pi, _pd, _pi,
                                _pi, _pi]),
+    "gpu_sweepucbxyz_noalloc": (None, [_pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi, _pd, _pd,
+                                       _pi, _pi, _pi, _pi, _pd, _pd, _pi, _pd, _pi, _pd, _pi, _pi,
+                                       _pd, _pd, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pi,
+                                       _pi, _pi, _pd, _pi, _pi, _pi]),
+    "gpu_sweepucbxyz_noalloc_max_bytes": (C.c_size_t, []),
     "gpu_streamsynchronize": (None, [_pi]),
     "gpu_devicesynchronize": (None, []),
     "gpu_sweepucbxyz_status": (_i, []),

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/umt_sweep.h"
@@ -2399,6 +2400,71 @@ constexpr int kMaxStreams = 80;
 std::mutex g_dropMutex;
 DropIn g_drop[kMaxStreams];
 
+// Device scratch of the no-alloc drop-in (gpu_sweepucbxyz_noalloc): every
+// device buffer of a call is carved from this array, whose address on each
+// GPU is looked up once (hipGetSymbolAddress) with that GPU current.
+constexpr size_t kNoAllocScratchBytes = 128ull << 20;
+constexpr int kMaxDevices = 64;
+void* g_scratchPtr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// The current GPU's address of `scratch` (looked up on the first call on
+// that GPU; callers hold g_dropMutex).
+int scratch_address(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(53, "device %d beyond the no-alloc table", dev);
+  if (!g_scratchPtr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(53, "no device address for the no-alloc scratch");
+    g_scratchPtr[dev] = ptr;
+  }
+  *out = g_scratchPtr[dev];
+  return 0;
+}
+
+// Bytes of the device buffers of a call, each rounded up to 256 bytes.
+size_t dropin_bytes(size_t C, size_t Z, size_t G, size_t nb, bool tau) {
+  auto r = [](size_t n) { return (n + 255) / 256 * 256; };
+  return r(9 * C * 8) * 2 + r(C * 8) + r(G * C * 8) + r(G * Z * 8) + r(3 * 8) +
+         r(G * (C + nb) * 8) + r(G * C * 8) + (tau ? r(G * C * 8) : 0) + r(3 * C * 4) +
+         r(Z * sizeof(int2));
+}
+
+// Carves the buffers of a call from the no-alloc scratch.
+int dropin_carve(DropBufs& b, size_t C, size_t Z, size_t G, size_t nb, bool tau) {
+  const size_t need = dropin_bytes(C, Z, G, nb, tau);
+  if (need > kNoAllocScratchBytes)
+    return fail(53, "gpu_sweepucbxyz_noalloc: the call needs %zu bytes of device scratch, more "
+                    "than its %zu (use gpu_sweepucbxyz)", need, kNoAllocScratchBytes);
+  void* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  char* p = (char*)base;
+  auto take = [&](auto*& dst, size_t bytes) {
+    dst = (std::remove_reference_t<decltype(dst)>)(void*)p;
+    p += (bytes + 255) / 256 * 256;
+  };
+  take(b.Afp, 9 * C * 8);
+  take(b.Aez, 9 * C * 8);
+  take(b.vol, C * 8);
+  take(b.S, G * C * 8);
+  take(b.sigt, G * Z * 8);
+  take(b.omega, 3 * 8);
+  take(b.psi1, G * (C + nb) * 8);
+  take(b.phi, G * C * 8);
+  if (tau) take(b.tauPsi, G * C * 8);
+  else b.tauPsi = nullptr;
+  take(b.cFP, 3 * C * 4);
+  take(b.items, Z * sizeof(int2));
+  return 0;
+}
+
 int dropin_alloc(DropIn& d, size_t C, size_t Z, size_t G, size_t nb) {
   if (C <= d.capC && Z <= d.capZ && G <= d.capG && nb <= d.capNb) return 0;
   dfree(d.Afp); dfree(d.Aez); dfree(d.vol); dfree(d.S); dfree(d.sigt); dfree(d.omega);
@@ -2425,7 +2491,7 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
                  const int* cEZ, double quadwt, double* Phi, double* PsiB, int maxCorner,
                  int streamId, int savePsi, int numCycles, int cycleOffSet, double* cyclePsi,
                  const int* cycleList, int nBdyElem, int Mref, const int* numCorner,
-                 const int* cOffSet) {
+                 const int* cOffSet, bool noAlloc) {
   if (ndim != 3) return fail(40, "only 3D (ndim = 3) is supported, got %d", ndim);
   if (maxcf != 3 || maxCorner != 8) return fail(41, "maxcf = %d, maxCorner = %d (need 3 and 8)", maxcf, maxCorner);
   if (numCycles < 0 || cycleOffSet < 0 || (numCycles > 0 && (!cyclePsi || !cycleList)))
@@ -2456,13 +2522,20 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   }
   const int nLevels = (int)planeOffset.size() - 1;
 
-  // Device buffers: the stream's own allocation (grown as needed).
+  // Device buffers: the stream's own allocation (grown as needed), or for
+  // the no-alloc entry point the static scratch on the null stream.
   const bool useTau = tau != 0.0;
-  DropIn& di = g_drop[streamId % kMaxStreams];
-  if (!di.stream) HIP_TRY(hipStreamCreate(&di.stream));
-  if (int rc = dropin_alloc(di, C, Z, G, nb)) return rc;
-  DropBufs d = di;
-  hipStream_t s = di.stream;
+  DropBufs d;
+  hipStream_t s = nullptr;
+  if (noAlloc) {
+    if (int rc = dropin_carve(d, C, Z, G, nb, useTau)) return rc;
+  } else {
+    DropIn& di = g_drop[streamId % kMaxStreams];
+    if (!di.stream) HIP_TRY(hipStreamCreate(&di.stream));
+    if (int rc = dropin_alloc(di, C, Z, G, nb)) return rc;
+    d = di;
+    s = di.stream;
+  }
   const size_t GC = (size_t)G * C;
   const std::vector<double> afpDev = umt::zone_blocked(A_fp, 9, (size_t)C);
   const std::vector<double> aezDev = umt::zone_blocked(A_ez, 9, (size_t)C);
@@ -2536,9 +2609,32 @@ void gpu_sweepucbxyz(int* Angle, int* nHyperPlanes, int* nZonesInPlane, int* nex
                                  omega, cFP, Psi1, *nbelem, A_ez, cEZ, *quadwt, Phi, PsiB,
                                  *maxCorner, *streamIdPtr, *savePsi, *numCycles, *cycleOffSet,
                                  cyclePsi, cycleList, *nBdyElem, *Mref, Geom_numCorner,
-                                 Geom_cOffSet);
+                                 Geom_cOffSet, false);
   if (g_dropin_status) fprintf(stderr, "gpu_sweepucbxyz: %s\n", g_err.c_str());
 }
+
+void gpu_sweepucbxyz_noalloc(int* Angle, int* nHyperPlanes, int* nZonesInPlane, int* nextZ,
+                             int* nextC, double* STotal, double* tau, double* Psi, int* Groups,
+                             double* Volume, double* Sigt, int* nCFacesArray, int* ndim,
+                             int* maxcf, int* ncorner, double* A_fp, double* omega, int* cFP,
+                             double* Psi1, int* nbelem, double* A_ez, int* cEZ, int* NumAngles,
+                             double* quadwt, double* Phi, double* PsiB, int* maxCorner,
+                             int* mem0solve1, int* streamIdPtr, int* totalStreams, int* savePsi,
+                             int* numCycles, int* cycleOffSet, double* cyclePsi, int* cycleList,
+                             int* b0, int* nBdyElem, double* PsiBMref, int* Mref,
+                             int* Geom_numCorner, int* Geom_cOffSet) {
+  (void)nextC; (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)b0; (void)PsiBMref;
+  std::lock_guard<std::mutex> lock(g_dropMutex);
+  g_dropin_status = dropin_sweep(*Angle, *nHyperPlanes, nZonesInPlane, nextZ, STotal, *tau, Psi,
+                                 *Groups, Volume, Sigt, nCFacesArray, *ndim, *maxcf, *ncorner, A_fp,
+                                 omega, cFP, Psi1, *nbelem, A_ez, cEZ, *quadwt, Phi, PsiB,
+                                 *maxCorner, *streamIdPtr, *savePsi, *numCycles, *cycleOffSet,
+                                 cyclePsi, cycleList, *nBdyElem, *Mref, Geom_numCorner,
+                                 Geom_cOffSet, true);
+  if (g_dropin_status) fprintf(stderr, "gpu_sweepucbxyz_noalloc: %s\n", g_err.c_str());
+}
+
+size_t gpu_sweepucbxyz_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
 
 void gpu_streamsynchronize(int* streamId) {
   DropIn& d = g_drop[*streamId % kMaxStreams];

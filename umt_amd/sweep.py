@@ -453,14 +453,16 @@ def hbm_copy_probe(n_doubles, reps=10):
 
 
 def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, Psi_angle=None,
-                 save_psi=False, stream_id=0, cycle_psi=None):
+                 save_psi=False, stream_id=0, cycle_psi=None, noalloc=False):
     """One angle through the drop-in C-ABI ``gpu_sweepucbxyz``.
 
     ``angle`` is 1-based. Psi1 (C+nbelem, G), PsiB_angle (nbelem, G) and Phi
     (C, G) are updated in place, as the Fortran caller's arrays would be.
     When the problem's schedule has cycle lists, cycle_psi (total, G) holds
     the lagged psi of every angle's cycle corners (Teton's cyclePsi) and is
-    updated in place; None uses zeros. Returns the library status (0 on success).
+    updated in place; None uses zeros. noalloc selects ``gpu_sweepucbxyz_noalloc``
+    (the same sweep with no device allocation). Returns the library status
+    (0 on success).
     """
     lib = _lib.load()
     m, g, q, s = problem.mesh(), problem.geometry(), problem.quadrature(), problem.schedule()
@@ -488,7 +490,8 @@ def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, P
     dummy_d = np.zeros(1)
     for arr in (STotal, Sigt, Psi1, PsiB_angle, Phi, Psi):
         assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"]
-    lib.gpu_sweepucbxyz(
+    fn = lib.gpu_sweepucbxyz_noalloc if noalloc else lib.gpu_sweepucbxyz
+    fn(
         C.byref(ints["Angle"]), C.byref(ints["nHP"]), _ip(zip_), _ip(nextZ), _ip(nextC),
         _dp(STotal), C.byref(tau_c), _dp(Psi), C.byref(ints["Groups"]), _dp(g["Volume"]),
         _dp(Sigt), _ip(m["nCFaces"]), C.byref(ints["ndim"]), C.byref(ints["maxcf"]),
