@@ -274,7 +274,7 @@ int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const dou
 int umt_ctx_set_acceleration(umt_ctx *ctx, int mode);
 /* Autotuner of the S2 grey sweeps' groups per lane (their levels are much
  * smaller than the S_N sweep's, so the best K can differ): times K = 1, 2,
- * 4, keeps the fastest. Needs the cross sections. No counterpart in the reference. */
+ * 4, 8 (msPerK[0..3], -1 when unsupported), keeps the fastest. Needs the cross sections. No counterpart in the reference. */
 int umt_ctx_autotune_grey(umt_ctx *ctx, int reps, int *bestK, double *msPerK);
 /* Cumulative BiCGStab iterations and S2 sweeps done by GTA on this context. */
 int umt_ctx_gta_stats(const umt_ctx *ctx, int64_t *greyIterations, int64_t *greySweeps);
@@ -305,7 +305,7 @@ int umt_ctx_set_gta_solver(umt_ctx *ctx, int kernel);
  * Doolittle factors (unit lower below the diagonal, upper on and above). */
 int umt_ctx_grey_transfer(umt_ctx *ctx, double *T, double *LU);
 
-/* Groups each lane of the sweep kernels carries (K = 1, 2 or 4, dividing
+/* Groups each lane of the sweep kernels carries (K = 1, 2, 4 or 8, dividing
  * the group count; default 1). No counterpart in the reference. */
 int umt_ctx_set_groups_per_lane(umt_ctx *ctx, int K);
 int umt_ctx_get_groups_per_lane(const umt_ctx *ctx, int *K);
@@ -329,8 +329,8 @@ int umt_ctx_get_angles_per_block(const umt_ctx *ctx, int *n, int *blocks);
 int umt_ctx_copy_phi(umt_ctx *dst, const umt_ctx *src);
 
 /* Autotuner: times reps sweeps with each supported groups-per-lane value on
- * the context's own problem and keeps the fastest. msPerK[0..2] receives the
- * mean ms per sweep for K = 1, 2, 4 (-1 when unsupported); may be NULL.
+ * the context's own problem and keeps the fastest. msPerK[0..3] receives the
+ * mean ms per sweep for K = 1, 2, 4, 8 (-1 when unsupported); may be NULL.
  * No counterpart in the reference. */
 int umt_ctx_autotune(umt_ctx *ctx, int reps, int *bestK, double *msPerK);
 

@@ -42,7 +42,8 @@ def _oracle_si(p, xs, iters):
     return phi
 
 
-@pytest.mark.parametrize("G,K,n", [(3, 1, 1000), (4, 1, 3), (4, 2, 1), (4, 4, 1000), (8, 4, 5)])
+@pytest.mark.parametrize("G,K,n", [(3, 1, 1000), (4, 1, 3), (4, 2, 1), (4, 4, 1000), (8, 4, 5),
+                                   (16, 8, 1000)])
 def test_cyclic_iterations_match_oracle(cyc, G, K, n):
     """Four source iterations (the lagged cycle psi carried from sweep to
     sweep) give the oracle's phi for every layout."""

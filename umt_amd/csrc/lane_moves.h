@@ -8,7 +8,9 @@ namespace umt {
 // the dpp_ctrl encoding: quad_perm patterns move values inside groups of
 // four lanes, row shifts and mirrors within a row of sixteen. A source
 // outside the row reads 0 (bound_ctrl), so no old value has to be set up.
-// The sweep kernels call these with every lane of the wave active.
+// The sweep kernels call these with every lane of the wave active: a lane
+// whose source lane is switched off reads 0, so a move must not sit in a
+// branch, nor be an operand of ?:, && or || (evaluated on some lanes only).
 template <int kCtrl>
 __device__ __forceinline__ int dpp_i32(int v) {
   return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, true);

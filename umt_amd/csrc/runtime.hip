@@ -1164,7 +1164,7 @@ int umt_ctx_sweep_n(umt_ctx* x, int n) {
 int umt_ctx_set_groups_per_lane(umt_ctx* x, int K) {
   if (!x) return fail(1, "null context");
   if (!umt::groups_per_lane_supported(x->G, K))
-    return fail(35, "groups per lane %d not supported for %d groups (1, 2 or 4, dividing G)", K, x->G);
+    return fail(35, "groups per lane %d not supported for %d groups (1, 2, 4 or 8, dividing G)", K, x->G);
   x->groupsPerLane = K;
   return 0;
 }
@@ -1263,10 +1263,10 @@ int umt_ctx_get_groups_per_lane(const umt_ctx* x, int* K) {
 
 int umt_ctx_autotune(umt_ctx* x, int reps, int* bestK, double* msPerK) {
   if (!x || reps < 1) return fail(1, "bad argument");
-  static const int cand[3] = {1, 2, 4};
+  static const int cand[4] = {1, 2, 4, 8};
   double best = 1e300;
   int bk = x->groupsPerLane;
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < 4; ++i) {
     if (msPerK) msPerK[i] = -1.0;
     if (!umt::groups_per_lane_supported(x->G, cand[i])) continue;
     x->groupsPerLane = cand[i];
@@ -1758,10 +1758,10 @@ int umt_ctx_autotune_grey(umt_ctx* x, int reps, int* bestK, double* msPerK) {
   const long GC = (long)x->G * x->C;
   if (x->precision == 32) HIP_TRY(umt::launch_fill_zero<float>((float*)g.S2, GC, x->stream));
   else HIP_TRY(umt::launch_fill_zero<double>((double*)g.S2, GC, x->stream));
-  static const int cand[3] = {1, 2, 4};
+  static const int cand[4] = {1, 2, 4, 8};
   double best = 1e300;
   int bk = x->s2.groupsPerLane;
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < 4; ++i) {
     if (msPerK) msPerK[i] = -1.0;
     if (!umt::groups_per_lane_supported(x->G, cand[i])) continue;
     x->s2.groupsPerLane = cand[i];

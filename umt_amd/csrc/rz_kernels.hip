@@ -189,6 +189,8 @@ __global__ __launch_bounds__(256) void rz_sweep_level(RzLevelArgs<Real> p) {
 // with a barrier between steps. The 16 quads of the wave are 16 / gbPerChunk
 // zone positions times gbPerChunk group blocks; a step with more zones than
 // positions takes several passes. Every lane runs every step's barrier.
+// live and the pass count depend on the quad only, so the lane moves inside
+// rz_corner, which stay within a quad, never read a switched-off lane.
 template <typename Real, int K>
 __global__ __launch_bounds__(64) void rz_sweep_tile(RzTileArgs<Real> q) {
   const int item = blockIdx.x;

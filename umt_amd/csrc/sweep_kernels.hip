@@ -29,8 +29,10 @@
 
 namespace umt {
 
-// Zeros read in place of the upwind psi of a face that brings none.
-__device__ __align__(32) const double kZeroGroups[4] = {0.0, 0.0, 0.0, 0.0};
+// Zeros read in place of the upwind psi of a face that brings none: one
+// group vector of the widest groups-per-lane variant (K = 8).
+constexpr int kMaxGroupsPerLane = 8;
+__device__ __align__(64) const double kZeroGroups[kMaxGroupsPerLane] = {};
 
 // The kernels assume the canonical hex topology of the tiled-mesh generator:
 // the EZ neighbour of local corner c (0-based) in corner-face slots 0..2 is
@@ -98,6 +100,37 @@ __device__ __forceinline__ Real sez_term(Real aez, Real mOpp, Real t, Real t2, R
   }
 }
 
+// The EZ coupling term of the ring edge this lane owns: the EZ face shared
+// with its slot-1 neighbour (the next lane of the quad, which sees the face
+// in slot 0). The face areas of the two corners are equal and opposite
+// (geometryUCBxyz.F90:166-178), so A = |aez| is the upwind corner's area
+// whichever corner that is. vU, tU = sigma vU, t2U, mU and wU = sigma
+// psiOpp - Qu belong to the upwind corner; dQ = Q(this) - Q(next) and the
+// sign of aez (this corner's area) carry the orientation:
+// 0.5 A (Qu - Qd) = 0.5 aez dQ exactly. The value is added to the upwind
+// corner's source and taken off the downwind one's; a face parallel to the
+// direction (aez = 0) gives 0. Computing each ring edge once, in the lane
+// that owns it, replaces the two evaluations (one per corner, one of them
+// 0) of the per-corner form sez_term, whose select-free FP64 arithmetic the
+// first expression here follows.
+template <typename Real>
+__device__ __forceinline__ Real ring_term(Real aez, Real mU, Real tU, Real t2U, Real sig, Real vU,
+                                          Real wU, Real dQ) {
+  const Real A = fabs(aez), halfA = (Real)0.5 * aez;
+  const Real poly = (Real)1.82 * t2U + A * ((Real)4 * tU + (Real)3 * A);
+  const Real gden = vU * ((Real)4 * tU * t2U + A * ((Real)6 * t2U + (Real)2 * A * ((Real)2 * tU + A)));
+  if constexpr (sizeof(Real) == 8) {
+    const Real gnum = mU * (A * A) * poly;
+    return quot<Real>(vU * gnum * wU + halfA * gden * dQ, gnum + gden * sig);
+  } else {
+    const Real gnum = (A * A) * poly;
+    const Real withOpp = quot<Real>(vU * gnum * wU + halfA * gden * dQ, gnum + gden * sig);
+    const Real noOpp = aez * quot<Real>((Real)0.5, sig) * dQ;
+    const Real x = mU > 0 ? withOpp : noOpp;
+    return A > 0 ? x : (Real)0;
+  }
+}
+
 // One lane: one corner c of one zone for one direction and K consecutive
 // groups. Geometry, dot products and the zone's EZ orientation are shared by
 // the K groups; everything group-dependent is a K-vector in registers.
@@ -107,6 +140,7 @@ __device__ __forceinline__ Real sez_term(Real aez, Real mOpp, Real t, Real t2, R
 template <typename Real, int K, bool kSingleAngle>
 __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int item, int g0,
                                              bool valid) {
+  static_assert(K <= kMaxGroupsPerLane, "the zero group vector is shorter than K");
   const int c = corner_of_lane(threadIdx.x & 7);
   const int G = p.G;
   const int2 it = p.items[item];
@@ -197,6 +231,17 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
   Real mOpp[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) mOpp[f] = afp[(f + 1) % 3] < 0 ? (Real)1 : (Real)0;
+  // The ring edge this lane owns (see ring_term): the upwind corner's
+  // volume and opposite-face flag, and the signs with which this lane takes
+  // its own ring edge (slot 1) and the previous lane's (its slot 0).
+  // The moves run on every lane before the selects: as an operand of ?:
+  // a move would run only on the lanes taking that branch, and a lane whose
+  // source lane is switched off reads 0.
+  const bool up1 = aez[1] > 0;
+  const Real vN = from_ez<1>(v), mN = from_ez<1>(mOpp[0]);
+  const Real vU = up1 ? v : vN;
+  const Real mU = up1 ? mOpp[1] : mN;
+  const Real sg1 = up1 ? (Real)1 : (Real)-1, sg0 = aez[0] > 0 ? (Real)1 : (Real)-1;
   Real src[K], rden[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -219,19 +264,22 @@ __device__ __forceinline__ void sweep_corner(const SweepLevelArgs<Real>& p, int 
       for (int f = 0; f < 3; ++f) s = s - (afp[f] < 0 ? afp[f] * psf[f].v[k] : (Real)0);
     }
     rden[k] = quot<Real>(1, outArea + sig * v);
-    const Real Qn[3] = {from_ez<0>(q), from_ez<1>(q), from_ez<2>(q)};
     const Real t = sig * v, t2 = t * t;
-    Real sezOut[3];
-#pragma unroll
-    for (int f = 0; f < 3; ++f)
-      sezOut[f] = sez_term<Real>(aez[f], mOpp[f], t, t2, sig, v, q, Qn[f], psf[(f + 1) % 3].v[k]);
-    s += sezOut[0];
-    s += sezOut[1];
-    s += sezOut[2];
-    // The neighbour in slot f sees this edge in slot 1, 0, 2 for f = 0, 1, 2.
-    s -= from_ez<0>(sezOut[1]);
-    s -= from_ez<1>(sezOut[0]);
-    s -= from_ez<2>(sezOut[2]);
+    // Ring edge: sigma psiOpp - Q of this corner on its slot-1 face
+    // (opposite FP face 2) and of the next corner on its slot-0 face
+    // (opposite FP face 1, computed there and moved here).
+    const Real w1 = sig * psf[2].v[k] - q;
+    const Real wN = from_ez<1>(sig * psf[1].v[k] - q);
+    const Real tU = sig * vU;
+    const Real e = ring_term<Real>(aez[1], mU, tU, tU * tU, sig, vU, up1 ? w1 : wN,
+                                   q - from_ez<1>(q));
+    // Vertical edge (slot 2, the mirror lane): each corner computes it when
+    // upwind and the mirror lane takes it off.
+    const Real s2 = sez_term<Real>(aez[2], mOpp[2], t, t2, sig, v, q, from_ez<2>(q), psf[0].v[k]);
+    s = fma(sg1, e, s);
+    s = fma(sg0, from_ez<0>(e), s);
+    s += s2;
+    s -= from_ez<2>(s2);
     src[k] = s;
   }
 
@@ -543,7 +591,9 @@ static hipError_t launch_k(const SweepLevelArgs<Real>& a, bool singleAngle, hipS
   return hipGetLastError();
 }
 
-int groups_per_lane_supported(int G, int K) { return (K == 1 || K == 2 || K == 4) && G % K == 0; }
+int groups_per_lane_supported(int G, int K) {
+  return (K == 1 || K == 2 || K == 4 || K == 8) && G % K == 0;
+}
 
 template <typename Real>
 hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStream_t s) {
@@ -562,6 +612,7 @@ hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStr
 template <typename Real>
 hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
   switch (a.groupsPerLane) {
+    case 8: return launch_k<Real, 8>(a, singleAngle, s);
     case 4: return launch_k<Real, 4>(a, singleAngle, s);
     case 2: return launch_k<Real, 2>(a, singleAngle, s);
     default: return launch_k<Real, 1>(a, singleAngle, s);

@@ -61,7 +61,7 @@ program umt_si_driver
   real(c_double), allocatable :: omega(:,:), weight(:)
   real(c_double), allocatable :: Sigt(:,:), sigs(:,:), q(:,:), Phi(:,:), STotal(:,:)
   real(c_double), allocatable :: Psi1(:,:), PsiB(:,:), Psi(:,:), hist(:), Psi3(:,:,:)
-  real(c_double)              :: residual, msPerK(3), tau, quadwt
+  real(c_double)              :: residual, msPerK(4), tau, quadwt
   real(c_double)              :: dummy(1)
   integer(c_int64_t)          :: greyIts, greySweeps
   integer(c_int)              :: idummy(1)

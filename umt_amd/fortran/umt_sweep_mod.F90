@@ -225,7 +225,7 @@ module umt_sweep_mod
       type(c_ptr), value :: ctx
       integer(c_int), value :: reps
       integer(c_int) :: bestK
-      real(c_double) :: msPerK(3)
+      real(c_double) :: msPerK(4)
     end function umt_ctx_autotune
 
     integer(c_int) function umt_ctx_create_from_problem(prob, groups, precision, ctx) &

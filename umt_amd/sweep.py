@@ -228,17 +228,17 @@ class SweepContext:
         """Time each supported groups-per-lane value; keep the fastest.
         Returns (best K, {K: ms per sweep})."""
         k = C.c_int(0)
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         check("umt_ctx_autotune", _lib.load().umt_ctx_autotune(self._h, reps, C.byref(k), _dp(ms)))
-        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4), ms) if t >= 0}
+        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4, 8), ms) if t >= 0}
 
     def autotune_grey(self, reps=2):
         """The same for the S2 grey sweeps of GTA. Returns (best K, {K: ms per S2 sweep})."""
         k = C.c_int(0)
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         check("umt_ctx_autotune_grey",
               _lib.load().umt_ctx_autotune_grey(self._h, reps, C.byref(k), _dp(ms)))
-        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4), ms) if t >= 0}
+        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4, 8), ms) if t >= 0}
 
     def incoming_faces(self):
         v = C.c_double(0.0)
