@@ -42,6 +42,7 @@ WORKLOADS = {
     "single": dict(tiles=(20, 20, 20), quad=(4, 4), groups=16, c=(0.0, 0.95), seed=20261204),
 }
 GRAPHS = [True]  # replay captured sweeps (off for profiler runs)
+FP32_GREY = {}   # grey controls of the FP32 contexts of mixed solves (empty: defaults)
 
 
 def log(msg):
@@ -327,6 +328,8 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, apb, net=None, warm_iters=2):
         ctx.autotune(1)
         if accel.startswith("gta"):
             ctx.autotune_grey(2)
+        if precision == 32 and accel.startswith("gta") and FP32_GREY:
+            ctx.set_grey_controls(**FP32_GREY)
         ctx.set_graphs(GRAPHS[0])
         if accel.endswith("learned"):
             ctx.set_learner(net, every=2)
@@ -416,6 +419,8 @@ def time_to_converge(p, G, xs, tol, max_iters, methods, learn_tiles, c_range):
 
     res = {"angles_per_block": {}, "timing": "each solve timed after a 2-iteration warm-up solve "
                                              "of the same contexts (graph capture), from phi = 0"}
+    if FP32_GREY:
+        res["fp32_phase_grey_controls"] = dict(FP32_GREY)
 
     def note_blocks(name, accel, mixed):
         plain, mix = blocks(accel)
@@ -725,6 +730,10 @@ def main():
                     help="angles timed through the per-angle drop-in with host arrays (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--fp32-grey-reltol", type=float, default=None,
+                    help="grey tolerance of the FP32 phase of mixed solves (default: the reference's 1e-6)")
+    ap.add_argument("--fp32-grey-maxits", type=int, default=None,
+                    help="grey iteration maximum of the FP32 phase of mixed solves (default 21)")
     ap.add_argument("--max-iters", type=int, default=3000)
     ap.add_argument("--batch-domains", type=int, default=64,
                     help="uneven domains in the batched-sweep section (0: skip it)")
@@ -752,6 +761,10 @@ def main():
 
     _lib.load()
     GRAPHS[0] = not args.no_graphs
+    if args.fp32_grey_reltol is not None:
+        FP32_GREY["rel_tol"] = args.fp32_grey_reltol
+    if args.fp32_grey_maxits is not None:
+        FP32_GREY["max_its"] = args.fp32_grey_maxits
     if args.only_rz:
         print(json.dumps({"rz_sweep": rz_sweep(args.rz[0], args.rz[1], 4, 4, 16, args.steps,
                                                args.warmup, 0 if args.no_cpu else 8.0,

@@ -297,6 +297,22 @@ int umt_ctx_grey_solve(umt_ctx *ctx, const double *gsrc, const double *phiHalf, 
  * (snac/UpdateScalarIntensity.F90:11-170) after one grey sweep
  * (rt/GreySweep.F90:12-47), the factors made once per set of opacities. */
 int umt_ctx_set_gta_solver(umt_ctx *ctx, int kernel);
+/* Grey iteration controls: relative tolerance (grey_reltol, default 1e-6)
+ * and maximum grey iterations (grey_max_sweeps, default 21). Replace
+ * teton_adjust_grey_reltol and teton_adjust_grey_maxits
+ * (aux/AdjustGreyRelTol.F90:11, aux/AdjustGreyMaxIts.F90:11). Status 57
+ * for a tolerance <= 0 or fewer than one iteration. */
+int umt_ctx_set_grey_controls(umt_ctx *ctx, double relTol, int maxIts);
+int umt_ctx_get_grey_controls(const umt_ctx *ctx, double *relTol, int *maxIts,
+                              int *enforceHardIterMax);
+/* enforceHardIterMax != 0: the grey iteration stops at its maximum whatever
+ * its error (rt/GTASolver.F90:373-375). Replaces the first argument of
+ * teton_setgtaoptions (aux/setGTAOptions.F90:15-34); the reference's default,
+ * kept here, is off for kernel 1 and on for kernel 2
+ * (mods/GreyAcceleration_mod.F90:122,169), set again by umt_ctx_set_gta_solver.
+ * forceExtraOuter, the second argument, is not built: it needs the grey
+ * control's convergence state, which GTASolver.F90 never sets. */
+int umt_ctx_set_gta_options(umt_ctx *ctx, int enforceHardIterMax);
 /* Kernel 2's matrices for checking: T(g, c, s) (Fortran order, G x C x 8)
  * is the scalar flux at corner c (1..C) from a unit per-steradian source at
  * local corner s (1..8) of the same zone, summed over the grey directions

@@ -68,7 +68,13 @@ class GreySolver:
     def zavg(self, x):
         return (x * self.V[:, None]).reshape(-1, 8, self.G).sum(1) / self.VZ[:, None]
 
-    def solve(self, gsrc, phi_half, eps_grey, max_sweeps=21, eps_point=1.0e-6):
+    # GTA%enforceHardGTAIterMax: off for the old solver (GreyAcceleration_mod.F90:122),
+    # on for the new one (:169); teton_setgtaoptions changes it (setGTAOptions.F90:15-34).
+    hard_max_default = False
+
+    def solve(self, gsrc, phi_half, eps_grey, max_sweeps=21, eps_point=1.0e-6, hard_max=None):
+        if hard_max is None:
+            hard_max = self.hard_max_default
         G = self.G
         F = np.zeros_like(gsrc)
         pz_old = np.zeros((self.VZ.shape[0], G))
@@ -114,6 +120,8 @@ class GreySolver:
             l2 = np.sqrt(abs(e2 / p2)) if p2 != 0.0 else 0.0
             err_grey = max(point, l2)
             pz_old = pz
+            if hard_max and n_grey >= max_sweeps:  # GTASolver.F90:373-375
+                break
             if (err_grey < eps_point or n_grey >= max_sweeps) and err_grey < eps_grey:
                 break
             if n_grey >= 100 * max_sweeps:
@@ -414,6 +422,8 @@ class NewGreySolver(GreySolver):
     zone's direct solve (I - TT wtiso sigma_s) P = PhiInc [+ TT wtiso gsrc]
     with LU factors made by the with-source call (GreySweepNEW)."""
 
+    hard_max_default = True
+
     def __init__(self, *args, **kw):
         super().__init__(*args, **kw)
         self.sigs_g = self.sigs_c[::8]
@@ -438,6 +448,6 @@ class NewGreySolver(GreySolver):
             self.LU = scalar_intensity_decompose(self.TT, self.sigs_g)
         return _corner_vec(scalar_intensity_solve(self.LU, rhs))
 
-    def solve(self, gsrc, phi_half, eps_grey, max_sweeps=21, eps_point=1.0e-6):
+    def solve(self, gsrc, phi_half, eps_grey, max_sweeps=21, eps_point=1.0e-6, hard_max=None):
         self.build()  # rt/GTASolver.F90:158-178: matrices built for every solve
-        return super().solve(gsrc, phi_half, eps_grey, max_sweeps, eps_point)
+        return super().solve(gsrc, phi_half, eps_grey, max_sweeps, eps_point, hard_max)

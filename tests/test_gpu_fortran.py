@@ -63,19 +63,21 @@ def test_fortran_dropin_angle_matches_oracle(tmp_path):
     assert rel_err(phi.reshape(p.ncorner, G), ref) < 1e-12
 
 
-def test_fortran_gta_source_iteration_matches_python_host(tmp_path):
-    """GTA-accelerated source iteration driven from Fortran gives the Python
-    host's answer on the same problem, and plain iteration's fixed point."""
+@pytest.mark.parametrize("mode", ["gta", "gta2"])
+def test_fortran_gta_source_iteration_matches_python_host(tmp_path, mode):
+    """GTA-accelerated source iteration (kernel 1, or kernel 2 through
+    umt_ctx_set_gta_solver) driven from Fortran gives the Python host's
+    answer on the same problem, and plain iteration's fixed point."""
     seed, G = 20261205, 2
-    args = ["gta", 2, 2, 2, 2, 2, G, 64, 200, 1e-10, seed, 1e-2, 1e3, 0.9, 0.999, 0.0, 1.0, 1]
-    phi, out = _run(args, "phi_gta.bin", tmp_path)
+    args = [mode, 2, 2, 2, 2, 2, G, 64, 200, 1e-10, seed, 1e-2, 1e3, 0.9, 0.999, 0.0, 1.0, 1]
+    phi, out = _run(args, f"phi_{mode}.bin", tmp_path)
     assert "gta iterations=" in out
     p = U.TiledProblem(2, 2, 2, 2, 2)
     st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (1e-2, 1e3), (0.9, 0.999), (0.0, 1.0))
     ctx = U.SweepContext(p, G)
     ctx.set_sigt(st)
     ctx.set_scattering(ss, qq)
-    ctx.set_acceleration("gta")
+    ctx.set_acceleration(mode)
     ok, it, _, _ = ctx.source_iteration(tol=1e-10, max_iters=200)
     assert ok and f"gta iterations={it:6d}" in out
     assert rel_err(phi.reshape(p.ncorner, G), ctx.phi()) < 1e-12

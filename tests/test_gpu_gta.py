@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import umt_amd as U
+from umt_amd._lib import UmtError
 from oracle import gta as OG
 from oracle import oracle as O
 
@@ -290,3 +291,35 @@ def test_transfer_matrix_gta_with_fp32_sweeps(thick):
     assert err < 1e-6
     fast.close()
     exact.close()
+
+
+def test_grey_controls_and_the_hard_iteration_cap(thick_small):
+    """Kernel 2 stops the grey iteration at its maximum whatever the error
+    (the reference's new solver sets GTA%enforceHardGTAIterMax,
+    GreyAcceleration_mod.F90:169; exit at GTASolver.F90:373-375), with the
+    oracle's iterate at the same controls; without the cap the same solve
+    runs past the maximum."""
+    p, G, st, ss, qq, q2, s2 = thick_small
+    ctx = _ctx2(p, G, st, ss, qq)
+    assert ctx.grey_controls() == (1e-6, 21, True)
+    ctx.set_grey_controls(rel_tol=1e-300, max_its=5)
+    rng = np.random.default_rng(14)
+    half = rng.uniform(0.5, 2.0, (p.ncorner, G))
+    gsrc = np.repeat(ss, 8, axis=0) * rng.normal(0.0, 0.1, (p.ncorner, G))
+    print("case hard grey cap, 5 grey iterations", flush=True)
+    F, n = ctx.grey_solve(gsrc, half, 1e-300)
+    solver = OG.NewGreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    Fr, nr = solver.solve(gsrc, half, 1e-300, max_sweeps=5, eps_point=1e-300)
+    assert nr == 5 and n == 2, (n, nr)
+    assert np.max(np.abs(F - Fr)) / np.max(np.abs(Fr)) < 1e-9
+    ctx.set_gta_options(False)
+    ctx.set_grey_controls(rel_tol=1e-13)
+    _, n_free = ctx.grey_solve(gsrc, half, 1e-13)
+    _, nr_free = solver.solve(gsrc, half, 1e-13, max_sweeps=5, eps_point=1e-13, hard_max=False)
+    assert n_free == (nr_free - 1) // 2 and n_free > 2, (n_free, nr_free)
+    ctx.set_gta_solver(1)
+    assert ctx.grey_controls()[2] is False
+    with pytest.raises(UmtError) as e:
+        ctx.set_grey_controls(rel_tol=0.0)
+    assert e.value.code == 57
+    ctx.close()

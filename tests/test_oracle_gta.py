@@ -164,7 +164,7 @@ def test_new_solver_reaches_the_grey_solution(tile):
     old = OG.GreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
     new = OG.NewGreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
     Fo, _ = old.solve(gsrc, half, 1e-11)
-    Fn, _ = new.solve(gsrc, half, 1e-11)
+    Fn, _ = new.solve(gsrc, half, 1e-11, hard_max=False)
     sig_g, scat_g, _, _ = OG.grey_opacities(st, ss)
     sc = np.repeat(scat_g, 8, axis=0)
     F = np.zeros_like(gsrc)
@@ -178,3 +178,24 @@ def test_new_solver_reaches_the_grey_solution(tile):
     assert np.max(np.abs(Fn - F)) < 1e-9 * np.abs(F).max()
     assert np.max(np.abs(Fo - F)) < 1e-9 * np.abs(F).max()
     assert 2 * new.sweeps < old.sweeps
+
+
+def test_new_solver_stops_at_the_hard_grey_iteration_cap(tile):
+    """The new solver enforces the grey iteration maximum as a hard cap
+    (GTA%enforceHardGTAIterMax, GreyAcceleration_mod.F90:169; the exit at
+    GTASolver.F90:373-375): with a tolerance it cannot reach it stops at
+    nGreyIter = max_sweeps exactly. Without the cap the same solve runs on
+    past max_sweeps, and the old solver has no cap by default."""
+    p, q2, s2, st, ss = tile
+    G = st.shape[1]
+    rng = np.random.default_rng(5)
+    gsrc = rng.uniform(0.0, 1.0, (p.ncorner, G))
+    half = rng.uniform(1.0, 2.0, (p.ncorner, G))
+    new = OG.NewGreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    _, n_cap = new.solve(gsrc, half, 1e-300, max_sweeps=5, eps_point=1e-300)
+    assert n_cap == 5
+    _, n_more = new.solve(gsrc, half, 1e-13, max_sweeps=5, eps_point=1e-13, hard_max=False)
+    assert n_more > 5
+    old = OG.GreySolver(p.mesh(), p.geometry(), q2, s2, st, ss)
+    _, n_old = old.solve(gsrc, half, 1e-13, max_sweeps=5, eps_point=1e-13)
+    assert n_old > 5

@@ -86,6 +86,9 @@ SIGNATURES = {
     "umt_ctx_autotune_grey": (_i, [_vp, _i, _pi, _pd]),
     "umt_ctx_grey_solve": (_i, [_vp, _pd, _pd, _d, _pd, _pi64]),
     "umt_ctx_set_gta_solver": (_i, [_vp, _i]),
+    "umt_ctx_set_grey_controls": (_i, [_vp, _d, _i]),
+    "umt_ctx_get_grey_controls": (_i, [_vp, _pd, _pi, _pi]),
+    "umt_ctx_set_gta_options": (_i, [_vp, _i]),
     "umt_ctx_set_incident_flux": (_i, [_vp, _pd]),
     "umt_ctx_get_boundary_flux": (_i, [_vp, _pd]),
     "umt_ctx_grey_transfer": (_i, [_vp, _pd, _pd]),

@@ -149,6 +149,10 @@ struct GtaState {
   int2* localItems = nullptr; // kernel 2: every (zone, grey direction), zone by zone
   int maxSweeps = 21;         // grey_max_sweeps (mods/default_iter_controls_mod.F90)
   double epsPoint = 1.0e-6;   // grey_reltol
+  // GTA%enforceHardGTAIterMax: stop the grey iteration at maxSweeps whatever
+  // its error; off for kernel 1 (mods/GreyAcceleration_mod.F90:122), on for
+  // kernel 2 (:169).
+  bool hardIterMax = false;
   double epsGreyInit = 1.0e-5;  // rt/rtmainsn.F90:129
   long long greyIterations = 0, greySweeps = 0;
 };
@@ -1421,6 +1425,30 @@ int umt_ctx_set_gta_solver(umt_ctx* x, int kernel) {
   if (!x) return fail(1, "null context");
   if (kernel != 1 && kernel != 2) return fail(55, "GTA kernel must be 1 (grey + stretched TSA) or 2 (transfer matrices)");
   x->gta.solver = kernel;
+  x->gta.hardIterMax = kernel == 2;  // each kernel's default; umt_ctx_set_gta_options overrides
+  return 0;
+}
+
+int umt_ctx_set_grey_controls(umt_ctx* x, double relTol, int maxIts) {
+  if (!x) return fail(1, "null context");
+  if (!(relTol > 0.0) || maxIts < 1)
+    return fail(57, "grey controls need a relative tolerance > 0 and at least one iteration");
+  x->gta.epsPoint = relTol;
+  x->gta.maxSweeps = maxIts;
+  return 0;
+}
+
+int umt_ctx_get_grey_controls(const umt_ctx* x, double* relTol, int* maxIts, int* enforceHardIterMax) {
+  if (!x) return fail(1, "null context");
+  if (relTol) *relTol = x->gta.epsPoint;
+  if (maxIts) *maxIts = x->gta.maxSweeps;
+  if (enforceHardIterMax) *enforceHardIterMax = x->gta.hardIterMax ? 1 : 0;
+  return 0;
+}
+
+int umt_ctx_set_gta_options(umt_ctx* x, int enforceHardIterMax) {
+  if (!x) return fail(1, "null context");
+  x->gta.hardIterMax = enforceHardIterMax != 0;
   return 0;
 }
 
@@ -1690,6 +1718,7 @@ int gta_solve(umt_ctx* x, const double* phiHalf, double epsGrey) {
       return fail(39, "grey solver produced a NaN at grey iteration %d", nGreyIter);
     const double relL2 = e[2] != 0.0 ? std::sqrt(std::fabs(e[1] / e[2])) : 0.0;
     const double errGrey = std::max(e[0], relL2);
+    if (g.hardIterMax && nGreyIter >= g.maxSweeps) break;  // rt/GTASolver.F90:373-375
     if ((errGrey < g.epsPoint || nGreyIter >= g.maxSweeps) && errGrey < epsGrey) break;
     if (nGreyIter >= 100 * g.maxSweeps)
       return fail(40, "grey solver not converging after %d grey iterations (error %.3e)", nGreyIter, errGrey);

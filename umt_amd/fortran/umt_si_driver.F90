@@ -8,6 +8,7 @@
 !                   semantics for tau = 0, within-group scattering);
 !    mode "gta":    the same with grey-transport acceleration, the context
 !                   built from the problem handle (S2 schedule included);
+!    mode "gta2":   the same with GTA kernel 2 (transfer matrices);
 !    mode "dropin": sweeps one angle through gpu_sweepucbxyz with the
 !                   argument list of src/teton/gpu/SweepUCBxyzToGPU.F90:198-240;
 !    mode "mesh":   as "si", on a hex mesh read from the file named by the
@@ -134,7 +135,7 @@ program umt_si_driver
     write(*, '(a,i6,a,es12.5,a,i2,a,es22.15)') "si iterations=", iters, " residual=", residual, &
          " groupsPerLane=", bestK, " sumPhi=", sum(Phi)
 
-  elseif (trim(mode) == "gta") then
+  elseif (trim(mode) == "gta" .or. trim(mode) == "gta2") then
 
     ! Context from the problem (it brings the S2 grey-acceleration schedule),
     ! GTA-accelerated source iteration with graph replay.
@@ -143,6 +144,7 @@ program umt_si_driver
     call check(umt_ctx_set_sigt(ctx, Sigt), "umt_ctx_set_sigt")
     call check(umt_ctx_set_scattering(ctx, sigs, q), "umt_ctx_set_scattering")
     call check(umt_ctx_set_acceleration(ctx, 1), "umt_ctx_set_acceleration")
+    if (trim(mode) == "gta2") call check(umt_ctx_set_gta_solver(ctx, 2), "umt_ctx_set_gta_solver")
     call check(umt_ctx_set_graphs(ctx, 1), "umt_ctx_set_graphs")
     allocate(hist(max(maxIters, 1)))
     rc = umt_ctx_source_iteration(ctx, tol, maxIters, iters, residual, hist)

@@ -41,7 +41,8 @@ module umt_sweep_mod
             umt_batch_set_cross_sections, umt_batch_reset, umt_batch_sweep,   &
             umt_batch_get_phi, umt_batch_source_iteration,                    &
             umt_batch_set_graphs, umt_batch_set_layout,                       &
-            umt_batch_time_iterations, umt_batch_autotune
+            umt_batch_time_iterations, umt_batch_autotune,                    &
+            umt_ctx_set_gta_solver, umt_ctx_set_grey_controls, umt_ctx_set_gta_options
 
   interface
 
@@ -263,6 +264,33 @@ module umt_sweep_mod
       type(c_ptr), value :: ctx
       integer(c_int), value :: mode
     end function umt_ctx_set_acceleration
+
+!   GTA kernel 1 (grey + stretched-TSA sweeps) or 2 (transfer matrices, the
+!   reference's Size%useNewGTASolver); resets the hard grey-iteration maximum
+!   to the kernel's default (off for 1, on for 2)
+    integer(c_int) function umt_ctx_set_gta_solver(ctx, kernel) &
+        bind(c, name="umt_ctx_set_gta_solver")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kernel
+    end function umt_ctx_set_gta_solver
+
+!   grey_reltol and grey_max_sweeps (teton_adjust_grey_reltol / _maxits)
+    integer(c_int) function umt_ctx_set_grey_controls(ctx, relTol, maxIts) &
+        bind(c, name="umt_ctx_set_grey_controls")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: relTol
+      integer(c_int), value :: maxIts
+    end function umt_ctx_set_grey_controls
+
+!   enforceHardGTAIterMax (teton_setgtaoptions)
+    integer(c_int) function umt_ctx_set_gta_options(ctx, enforceHardIterMax) &
+        bind(c, name="umt_ctx_set_gta_options")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enforceHardIterMax
+    end function umt_ctx_set_gta_options
 
     integer(c_int) function umt_ctx_gta_stats(ctx, greyIterations, greySweeps) &
         bind(c, name="umt_ctx_gta_stats")

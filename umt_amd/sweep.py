@@ -354,6 +354,26 @@ class SweepContext:
         """GTA kernel 1 (grey + stretched-TSA sweeps) or 2 (transfer matrices)."""
         check("umt_ctx_set_gta_solver", _lib.load().umt_ctx_set_gta_solver(self._h, int(kernel)))
 
+    def set_grey_controls(self, rel_tol=None, max_its=None):
+        """Grey iteration tolerance and maximum (teton_adjust_grey_reltol,
+        teton_adjust_grey_maxits); None keeps the current value."""
+        t, m, _ = self.grey_controls()
+        check("umt_ctx_set_grey_controls", _lib.load().umt_ctx_set_grey_controls(
+            self._h, float(t if rel_tol is None else rel_tol), int(m if max_its is None else max_its)))
+
+    def grey_controls(self):
+        """(relative tolerance, maximum grey iterations, hard maximum on)."""
+        t, m, h = C.c_double(0.0), C.c_int(0), C.c_int(0)
+        check("umt_ctx_get_grey_controls", _lib.load().umt_ctx_get_grey_controls(
+            self._h, C.byref(t), C.byref(m), C.byref(h)))
+        return t.value, m.value, bool(h.value)
+
+    def set_gta_options(self, enforce_hard_iter_max):
+        """Stop the grey iteration at its maximum whatever its error
+        (teton_setgtaoptions; default on for kernel 2, off for kernel 1)."""
+        check("umt_ctx_set_gta_options",
+              _lib.load().umt_ctx_set_gta_options(self._h, int(bool(enforce_hard_iter_max))))
+
     def grey_transfer(self):
         """Kernel 2's matrices as (Z, G, 8, 8) arrays: T[z, g, s, c] (flux at
         corner c from a unit source at corner s) and the LU factors
