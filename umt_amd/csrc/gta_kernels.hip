@@ -102,6 +102,68 @@ __global__ void gta_wdot_partial(const double* __restrict__ x, const double* __r
   }
 }
 
+// Both of BiCGStab's omega inner products in one pass: per-group sums of
+// x y sigma_s V (partial[0 .. nblocks G)) and x x sigma_s V (partial[nblocks G
+// .. 2 nblocks G)), each reduced like gta_wdot_partial's.
+__global__ void gta_wdot2_partial(const double* __restrict__ x, const double* __restrict__ y,
+                                  const double* __restrict__ sigs, const double* __restrict__ vol,
+                                  int G, long n, double* __restrict__ partial) {
+  extern __shared__ double red[];
+  double axy = 0.0, axx = 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long c = i / G;
+    const int g = (int)(i - c * G);
+    const double w = sigs[(c >> 3) * G + g] * vol[c];
+    const double xi = x[i];
+    axy += xi * y[i] * w;
+    axx += xi * xi * w;
+  }
+  red[threadIdx.x] = axy;
+  red[blockDim.x + threadIdx.x] = axx;
+  __syncthreads();
+  if ((int)threadIdx.x < G) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int t = threadIdx.x; t < (int)blockDim.x; t += G) {
+      s1 += red[t];
+      s2 += red[blockDim.x + t];
+    }
+    partial[(long)blockIdx.x * G + threadIdx.x] = s1;
+    partial[((long)gridDim.x + blockIdx.x) * G + threadIdx.x] = s2;
+  }
+}
+
+// BiCGStab's closing updates of one step in one pass, per group g with
+// coef[3g] = alpha, coef[3g+1] = omega: F += alpha d + omega r, then
+// r -= omega As, with the per-group sums of the new r sigma_s V reduced
+// like gta_wdot_partial's (the next step's r.r-hat product).
+__global__ void gta_bicg_update(double* __restrict__ F, const double* __restrict__ d,
+                                double* __restrict__ r, const double* __restrict__ As,
+                                const double* __restrict__ coef, const double* __restrict__ sigs,
+                                const double* __restrict__ vol, int G, long n,
+                                double* __restrict__ partial) {
+  extern __shared__ double red[];
+  double acc = 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long c = i / G;
+    const int g = (int)(i - c * G);
+    const double alpha = coef[3 * g], omega = coef[3 * g + 1];
+    const double ri = r[i];
+    F[i] = F[i] + alpha * d[i] + omega * ri;
+    const double rn = ri - omega * As[i];
+    r[i] = rn;
+    acc += rn * (sigs[(c >> 3) * G + g] * vol[c]);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if ((int)threadIdx.x < G) {
+    double s = 0.0;
+    for (int t = threadIdx.x; t < (int)blockDim.x; t += G) s += red[t];
+    partial[(long)blockIdx.x * G + threadIdx.x] = s;
+  }
+}
+
 // One block per group: each thread sums a fixed stride of the block
 // partials, then a fixed-shape tree in LDS, so the result has the same bits
 // on every run.
@@ -272,10 +334,13 @@ __global__ __launch_bounds__(128) void gta_zone_lu(double* __restrict__ T,
 // T sigma_s, equals the reference's (I - B)^-1 (PhiInc [+ T wtiso gsrc])
 // because the sweep is linear: PhiInc = phi - T wtiso (sigma_s P [+ gsrc]).
 // Forward and back substitution as ScalarIntensitySolve
-// (UpdateScalarIntensity.F90:136-167). out may alias phi.
+// (UpdateScalarIntensity.F90:136-167). out may alias phi. residual != 0
+// writes P - M(P) = -(I - B)^-1 (phi - P) instead, the operator's residual
+// form that BiCGStab applies (v - M v).
 __global__ __launch_bounds__(256) void gta_zone_solve(const double* __restrict__ LU,
                                                       const double* P, const double* phi, int G,
-                                                      long nzg, long CG, double* out) {
+                                                      long nzg, long CG, double* out,
+                                                      int residual) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nzg) return;
   const long z = t / G;
@@ -304,7 +369,7 @@ __global__ __launch_bounds__(256) void gta_zone_solve(const double* __restrict__
     x[k] = (x[k] - r) / base[k * CG + (long)k * G];
   }
 #pragma unroll
-  for (int c = 0; c < 8; ++c) out[k0 + (long)c * G] = p[c] + x[c];
+  for (int c = 0; c < 8; ++c) out[k0 + (long)c * G] = residual ? -x[c] : p[c] + x[c];
 }
 
 // ----------------------------------------------------------------------------
@@ -344,6 +409,27 @@ hipError_t launch_gta_wdot(const double* x, const double* y, const double* sigs,
   return hipGetLastError();
 }
 
+hipError_t launch_gta_wdot2(const double* x, const double* y, const double* sigs, const double* vol,
+                            int G, long n, int nblocks, double* partial, double* out, hipStream_t s) {
+  const int bs = gta_dot_block(G);
+  hipLaunchKernelGGL(gta_wdot2_partial, dim3(nblocks), dim3(bs), 2 * bs * sizeof(double), s, x, y,
+                     sigs, vol, G, n, partial);
+  hipLaunchKernelGGL(gta_reduce_partials, dim3(G), dim3(256), 0, s, partial, nblocks, G, out);
+  hipLaunchKernelGGL(gta_reduce_partials, dim3(G), dim3(256), 0, s, partial + (long)nblocks * G,
+                     nblocks, G, out + G);
+  return hipGetLastError();
+}
+
+hipError_t launch_gta_bicg_update(double* F, const double* d, double* r, const double* As,
+                                  const double* coef, const double* sigs, const double* vol, int G,
+                                  long n, int nblocks, double* partial, double* out, hipStream_t s) {
+  const int bs = gta_dot_block(G);
+  hipLaunchKernelGGL(gta_bicg_update, dim3(nblocks), dim3(bs), bs * sizeof(double), s, F, d, r, As,
+                     coef, sigs, vol, G, n, partial);
+  hipLaunchKernelGGL(gta_reduce_partials, dim3(G), dim3(256), 0, s, partial, nblocks, G, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const double* vol,
                                  const double* volZ, int G, long nzg, double* pzOld, int nblocks,
                                  double* partial, double* out, hipStream_t s) {
@@ -366,9 +452,9 @@ hipError_t launch_gta_zone_lu(double* T, const double* sigs, int G, long nzg, lo
 }
 
 hipError_t launch_gta_zone_solve(const double* LU, const double* P, const double* phi, int G,
-                                 long nzg, long CG, double* out, hipStream_t s) {
+                                 long nzg, long CG, double* out, bool residual, hipStream_t s) {
   hipLaunchKernelGGL(gta_zone_solve, dim3(blocks_for(nzg, 256)), dim3(256), 0, s, LU, P, phi, G,
-                     nzg, CG, out);
+                     nzg, CG, out, residual ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -131,11 +131,19 @@ template <typename Real>
 hipError_t launch_gta_unit_source(int s, int G, long n, Real* S, hipStream_t st);
 hipError_t launch_gta_zone_lu(double* T, const double* sigs, int G, long nzg, long CG,
                               hipStream_t s);
+// residual: write P - M(P) instead of M(P).
 hipError_t launch_gta_zone_solve(const double* LU, const double* P, const double* phi, int G,
-                                 long nzg, long CG, double* out, hipStream_t s);
+                                 long nzg, long CG, double* out, bool residual, hipStream_t s);
 int gta_dot_block(int G);
 hipError_t launch_gta_wdot(const double* x, const double* y, const double* sigs, const double* vol,
                            int G, long n, int nblocks, double* partial, double* out, hipStream_t s);
+// out[0..G) = sum x y sigma_s V, out[G..2G) = sum x x sigma_s V (partial: 2 nblocks G).
+hipError_t launch_gta_wdot2(const double* x, const double* y, const double* sigs, const double* vol,
+                            int G, long n, int nblocks, double* partial, double* out, hipStream_t s);
+// F += alpha d + omega r; r -= omega As (coef[3g], coef[3g+1]); out = sum r sigma_s V.
+hipError_t launch_gta_bicg_update(double* F, const double* d, double* r, const double* As,
+                                  const double* coef, const double* sigs, const double* vol, int G,
+                                  long n, int nblocks, double* partial, double* out, hipStream_t s);
 hipError_t launch_gta_zone_error(const double* F, const double* phiHalf, const double* vol,
                                  const double* volZ, int G, long nzg, double* pzOld, int nblocks,
                                  double* partial, double* out, hipStream_t s);

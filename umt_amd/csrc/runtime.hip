@@ -1402,7 +1402,7 @@ int umt_ctx_enable_gta(umt_ctx* x, int nangles, const double* omega, const doubl
       (rc = dalloc(&g.d, GC)) || (rc = dalloc(&g.Ad, GC)) || (rc = dalloc(&g.As, GC)) ||
       (rc = dalloc(&g.P1, GC)) || (rc = dalloc(&g.gsrc, GC)) || (rc = dalloc(&g.phiHalf, GC)) ||
       (rc = dalloc(&g.pzOld, GZ)) ||
-      (rc = dalloc(&g.partial, (size_t)std::max(g.dotBlocks * x->G, 3 * g.errBlocks))) ||
+      (rc = dalloc(&g.partial, (size_t)std::max(2 * g.dotBlocks * x->G, 3 * g.errBlocks))) ||
       (rc = dalloc(&g.dots, 4 * (size_t)x->G)) || (rc = dalloc(&g.coef, 3 * (size_t)x->G)) ||
       (rc = dalloc(&g.err, 3)))
     return rc;
@@ -1557,8 +1557,10 @@ int gta_source(umt_ctx* x, int kind, const double* P, const double* other) {
 
 // out = M(P): grey S2 sweep of wtiso (sigma_s P [+ gsrc]) followed by the
 // stretched-TSA sweep of wtiso sigma_s eps (out - P) (rt/GreySweep.F90:60-96).
-// P == nullptr stands for zero. out must differ from P.
-int grey_sweep(umt_ctx* x, const double* P, bool withSource, double* out) {
+// P == nullptr stands for zero. out must differ from P. With kernel 2,
+// residual writes P - M(P) instead (the solve's last step gives it
+// directly; see grey_residual).
+int grey_sweep(umt_ctx* x, const double* P, bool withSource, double* out, bool residual = false) {
   GtaState& g = x->gta;
   const size_t GZ = (size_t)x->G * x->Z;
   if (g.solver == 2) {
@@ -1569,10 +1571,12 @@ int grey_sweep(umt_ctx* x, const double* P, bool withSource, double* out) {
                                  : gta_source<double>(x, 1, P, withSource ? g.gsrc : nullptr);
     if (rc2) return rc2;
     if ((rc2 = sweep_set(x, x->s2, g.S2, g.sigt2, out, false, false))) return rc2;
-    HIP_TRY(umt::launch_gta_zone_solve(g.lu, P, out, x->G, (long)GZ, (long)x->C * x->G, out, x->stream));
+    HIP_TRY(umt::launch_gta_zone_solve(g.lu, P, out, x->G, (long)GZ, (long)x->C * x->G, out,
+                                       residual, x->stream));
     g.greySweeps += 1;
     return 0;
   }
+  if (residual) return fail(101, "residual form of the grey operator is kernel 2's");
   const char* st = (const char*)g.sigt2;
   const void* sigGrey = st;
   const void* sigStretch = st + GZ * x->rsize();
@@ -1608,6 +1612,51 @@ int wdot(umt_ctx* x, const double* a, const double* b, std::vector<double>& out)
   HIP_TRY(hipMemcpyAsync(out.data(), g.dots, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
   HIP_TRY(hipStreamSynchronize(x->stream));
   return 0;
+}
+
+// wn = (x y), wd = (x x) with the sigma_s V weights, in one pass.
+int wdot2(umt_ctx* x, const double* a, const double* b, std::vector<double>& wn,
+          std::vector<double>& wd) {
+  GtaState& g = x->gta;
+  const size_t GZ = (size_t)x->G * x->Z;
+  HIP_TRY(umt::launch_gta_wdot2(a, b, g.eps + GZ, x->vol64, x->G, (long)x->G * x->C, g.dotBlocks,
+                                g.partial, g.dots, x->stream));
+  std::vector<double> both(2 * (size_t)x->G);
+  HIP_TRY(hipMemcpyAsync(both.data(), g.dots, both.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));
+  wn.assign(both.begin(), both.begin() + x->G);
+  wd.assign(both.begin() + x->G, both.end());
+  return 0;
+}
+
+// F += alpha d + omega r; r -= omega As; rr = (r, 1) with the sigma_s V
+// weights, in one pass.
+int bicg_update(umt_ctx* x, const std::vector<double>& alpha, const std::vector<double>& omega,
+                std::vector<double>& rr) {
+  GtaState& g = x->gta;
+  const size_t GZ = (size_t)x->G * x->Z;
+  std::vector<double> cf(3 * (size_t)x->G, 0.0);
+  for (int k = 0; k < x->G; ++k) { cf[3 * k] = alpha[k]; cf[3 * k + 1] = omega[k]; }
+  HIP_TRY(hipMemcpyAsync(g.coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice,
+                         x->stream));
+  HIP_TRY(umt::launch_gta_bicg_update(g.F, g.d, g.r, g.As, g.coef, g.eps + GZ, x->vol64, x->G,
+                                      (long)x->G * x->C, g.dotBlocks, g.partial, g.dots, x->stream));
+  rr.resize(x->G);
+  HIP_TRY(hipMemcpyAsync(rr.data(), g.dots, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIP_TRY(hipStreamSynchronize(x->stream));  // also before cf (host memory) goes out of scope
+  return 0;
+}
+
+// out = P - M(P), BiCGStab's operator application (rt/GTASolver.F90:240-245,
+// 268-273): kernel 2's zone solve writes it in the same pass, kernel 1 takes
+// M(P) off P with a linear combination.
+int grey_residual(umt_ctx* x, const double* P, double* out) {
+  if (x->gta.solver == 2) return grey_sweep(x, P, false, out, true);
+  if (int rc = grey_sweep(x, P, false, out)) return rc;
+  std::vector<double> cf(3 * (size_t)x->G);
+  for (int k = 0; k < x->G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -1.0; cf[3 * k + 2] = 0.0; }
+  return lincomb(x, P, out, nullptr, out, cf);
 }
 
 // BiCGStab on the grey corrections (rt/GTASolver.F90:185-402), the G
@@ -1648,12 +1697,7 @@ int gta_solve(umt_ctx* x, const double* phiHalf, double epsGrey) {
     nGreyIter += 2;
     ++g.greyIterations;
     // Ad = d - M d
-    if ((rc = grey_sweep(x, g.d, false, g.Ad))) return rc;
-    {
-      std::vector<double> cf(3 * G);
-      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -1.0; cf[3 * k + 2] = 0.0; }
-      if ((rc = lincomb(x, g.d, g.Ad, nullptr, g.Ad, cf))) return rc;
-    }
+    if ((rc = grey_residual(x, g.d, g.Ad))) return rc;
     if ((rc = wdot(x, g.Ad, nullptr, dAd))) return rc;
     std::vector<double> alpha(G, 0.0);
     for (int k = 0; k < G; ++k) {
@@ -1667,13 +1711,8 @@ int gta_solve(umt_ctx* x, const double* phiHalf, double epsGrey) {
       if ((rc = lincomb(x, g.r, g.Ad, nullptr, g.r, cf))) return rc;
     }
     // As = r - M r
-    if ((rc = grey_sweep(x, g.r, false, g.As))) return rc;
-    {
-      std::vector<double> cf(3 * G);
-      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -1.0; cf[3 * k + 2] = 0.0; }
-      if ((rc = lincomb(x, g.r, g.As, nullptr, g.As, cf))) return rc;
-    }
-    if ((rc = wdot(x, g.As, g.r, wn)) || (rc = wdot(x, g.As, g.As, wd))) return rc;
+    if ((rc = grey_residual(x, g.r, g.As))) return rc;
+    if ((rc = wdot2(x, g.As, g.r, wn, wd))) return rc;
     std::vector<double> omega(G, 0.0);
     std::vector<char> stopAfter(G, 0);
     for (int k = 0; k < G; ++k) {
@@ -1681,19 +1720,8 @@ int gta_solve(umt_ctx* x, const double* phiHalf, double epsGrey) {
       if (std::fabs(wd[k]) < tiny || std::fabs(wn[k]) < tiny) stopAfter[k] = 1;  // F += alpha d only
       else omega[k] = wn[k] / wd[k];
     }
-    // F = F + alpha d + omega r
-    {
-      std::vector<double> cf(3 * G);
-      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = alpha[k]; cf[3 * k + 2] = omega[k]; }
-      if ((rc = lincomb(x, g.F, g.d, g.r, g.F, cf))) return rc;
-    }
-    // r = r - omega As
-    {
-      std::vector<double> cf(3 * G);
-      for (int k = 0; k < G; ++k) { cf[3 * k] = 1.0; cf[3 * k + 1] = -omega[k]; cf[3 * k + 2] = 0.0; }
-      if ((rc = lincomb(x, g.r, g.As, nullptr, g.r, cf))) return rc;
-    }
-    if ((rc = wdot(x, g.r, nullptr, rr))) return rc;
+    // F = F + alpha d + omega r, then r = r - omega As, and r's product
+    if ((rc = bicg_update(x, alpha, omega, rr))) return rc;
     // d = r + beta (d - omega Ad)
     {
       std::vector<double> cf(3 * G);
