@@ -42,7 +42,12 @@ WORKLOADS = {
     "single": dict(tiles=(20, 20, 20), quad=(4, 4), groups=16, c=(0.0, 0.95), seed=20261204),
 }
 GRAPHS = [True]  # replay captured sweeps (off for profiler runs)
-FP32_GREY = {}   # grey controls of the FP32 contexts of mixed solves (empty: defaults)
+FP32_GREY = {}   # grey controls of the FP32 phase of mixed solves (empty: solve_mixed's)
+
+
+def fp32_grey_max_its():
+    import umt_amd as U
+    return FP32_GREY.get("max_its", U.sweep.FP32_PHASE_GREY_MAX_ITS)
 
 
 def log(msg):
@@ -328,8 +333,8 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, apb, net=None, warm_iters=2):
         ctx.autotune(1)
         if accel.startswith("gta"):
             ctx.autotune_grey(2)
-        if precision == 32 and accel.startswith("gta") and FP32_GREY:
-            ctx.set_grey_controls(**FP32_GREY)
+        if precision == 32 and accel.startswith("gta") and "rel_tol" in FP32_GREY:
+            ctx.set_grey_controls(rel_tol=FP32_GREY["rel_tol"])
         ctx.set_graphs(GRAPHS[0])
         if accel.endswith("learned"):
             ctx.set_learner(net, every=2)
@@ -350,7 +355,8 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, apb, net=None, warm_iters=2):
     t0 = time.perf_counter()
     if mixed:
         ok, it32, it64, res, hist = U.solve_mixed(fast, exact, tol=tol, switch_tol=1e-6,
-                                                  max_iters=max_iters)
+                                                  max_iters=max_iters,
+                                                  fast_grey_max_its=fp32_grey_max_its())
     else:
         ok, it64, res, hist = exact.source_iteration(tol=tol, max_iters=max_iters)
         it32 = 0
@@ -419,8 +425,9 @@ def time_to_converge(p, G, xs, tol, max_iters, methods, learn_tiles, c_range):
 
     res = {"angles_per_block": {}, "timing": "each solve timed after a 2-iteration warm-up solve "
                                              "of the same contexts (graph capture), from phi = 0"}
-    if FP32_GREY:
-        res["fp32_phase_grey_controls"] = dict(FP32_GREY)
+    res["fp32_phase_grey_max_its"] = fp32_grey_max_its()
+    if "rel_tol" in FP32_GREY:
+        res["fp32_phase_grey_rel_tol"] = FP32_GREY["rel_tol"]
 
     def note_blocks(name, accel, mixed):
         plain, mix = blocks(accel)
@@ -730,10 +737,12 @@ def main():
                     help="angles timed through the per-angle drop-in with host arrays (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=None,
+                    help="cross-section seed of the headline workload (default: the workload's own)")
     ap.add_argument("--fp32-grey-reltol", type=float, default=None,
                     help="grey tolerance of the FP32 phase of mixed solves (default: the reference's 1e-6)")
     ap.add_argument("--fp32-grey-maxits", type=int, default=None,
-                    help="grey iteration maximum of the FP32 phase of mixed solves (default 21)")
+                    help="grey iteration maximum of the FP32 phase of mixed solves (default: solve_mixed's, 9)")
     ap.add_argument("--max-iters", type=int, default=3000)
     ap.add_argument("--batch-domains", type=int, default=64,
                     help="uneven domains in the batched-sweep section (0: skip it)")
@@ -780,6 +789,8 @@ def main():
     sections_s = {}
     wl = WORKLOADS[args.workload]
     tiles, (npol, nazi), G, c_range, seed = wl["tiles"], wl["quad"], wl["groups"], wl["c"], wl["seed"]
+    if args.seed is not None:
+        seed = args.seed
     t0 = time.perf_counter()
     p = U.TiledProblem(*tiles, npol, nazi)
     st, ss, qq = U.draw_cross_sections(seed, p.nzones, G, (1e-2, 1e3), c_range, (0.0, 1.0))

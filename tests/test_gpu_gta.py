@@ -278,12 +278,19 @@ def test_transfer_matrix_gta_converges_with_fewer_grey_sweeps(thick):
     assert runs[2][1][1] * 2 < runs[1][1][1]
 
 
-def test_transfer_matrix_gta_with_fp32_sweeps(thick):
+@pytest.mark.parametrize("fp32_grey_max_its", [U.sweep.FP32_PHASE_GREY_MAX_ITS, None])
+def test_transfer_matrix_gta_with_fp32_sweeps(thick, fp32_grey_max_its):
+    """FP32 then FP64 sweeps with GTA kernel 2 reach plain iteration's fixed
+    point, with the FP32 phase's grey cap (the default) and with the
+    reference's controls; the cap leaves the FP32 context's own controls as
+    they were."""
     p, G, st, ss, qq, q2, s2 = thick
     fast = _ctx2(p, G, st, ss, qq, precision=32)
     exact = _ctx2(p, G, st, ss, qq, precision=64)
-    ok, it32, it64, res, _ = U.solve_mixed(fast, exact, tol=1e-8, switch_tol=1e-6, max_iters=200)
+    ok, it32, it64, res, _ = U.solve_mixed(fast, exact, tol=1e-8, switch_tol=1e-6, max_iters=200,
+                                           fast_grey_max_its=fp32_grey_max_its)
     assert ok and res < 1e-8
+    assert fast.grey_controls() == (1e-6, 21, True)
     ref, _, _, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), p.schedule(),
                                       st, ss, qq, 1e-13, 20000, nthreads=4)
     err = rel_err(fast.phi(), ref)

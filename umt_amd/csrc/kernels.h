@@ -33,7 +33,7 @@ struct SweepLevelArgs {
                             // with cyclic corner dependencies comes as ~zone
   int nItems;
   int G;                    // groups
-  int groupsPerLane;        // K: groups one lane carries (1, 2 or 4; divides G)
+  int groupsPerLane;        // K: groups one lane carries (1, 2, 4 or 8; divides G)
   int C;                    // corners; cFP values above C are boundary elements
   const Real* Afp;          // zone-blocked: A_fp(d, f, 8 z + c) at (9 z + 3 f + d) 8 + c
   const Real* Aez;          // zone-blocked like Afp

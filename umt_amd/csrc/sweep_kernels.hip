@@ -13,6 +13,9 @@
 // The corner arithmetic follows snac/SweepUCBxyz.F90:119-306: volume terms,
 // incoming FP-face fluxes, the EZ coupling term sez with fouralpha = 1.82,
 // and the per-corner solve psi_c = src_c / (sum of outgoing areas + sigma V_c).
+// Of a zone's 12 EZ edges, each of the 8 on the two quad rings has its term
+// computed once, by the lane that owns it (ring_term); each of the 4
+// vertical edges by its upwind corner, as the reference does (sez_term).
 // The within-zone corner solve does not need the nextC order: corners are
 // relaxed in a fixed order, each pulling the EZ couplings of its upstream
 // neighbours, until every corner has been computed from final upstream
@@ -64,7 +67,7 @@ __device__ __forceinline__ T from_ez(T x) {
 }
 
 // The EZ coupling term of EZ face f of a corner (SweepUCBxyz.F90:217-252),
-// fouralpha = 1.82, t = sigma v:
+// used for the vertical edges (slot 2), fouralpha = 1.82, t = sigma v:
 //   gnum = A^2 (1.82 t^2 + A (4 t + 3 A)), gden = v (4 t^3 + A (6 t^2 + 2 A (2 t + A))),
 //   sez = (v gnum (sigma psiOpp - Qu) + A/2 gden (Qu - Qd)) / (gnum + gden sigma)
 // when the face is outgoing (A = Omega.A_ez > 0) and the opposite FP face

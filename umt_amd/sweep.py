@@ -527,15 +527,32 @@ def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, P
     return lib.gpu_sweepucbxyz_status()
 
 
-def solve_mixed(fast, exact, tol=1e-8, switch_tol=1e-6, max_iters=1000, stall_window=8):
+# Grey iterations per grey solve in the FP32 phase of solve_mixed: that
+# phase needs its correction only to the FP32 switch tolerance. With the
+# reference's 21, GTA kernel 2 solves on 32^3 tiles took 14.6 and 14.5 s
+# (two seeds) against 11.9 and 11.6 s with 9, and 0.71 against 0.66 s on
+# 20^3 tiles, same outer iterations; 7, 5 and 3 were slower.
+FP32_PHASE_GREY_MAX_ITS = 9
+
+
+def solve_mixed(fast, exact, tol=1e-8, switch_tol=1e-6, max_iters=1000, stall_window=8,
+                fast_grey_max_its=FP32_PHASE_GREY_MAX_ITS):
     """Source iteration with FP32 sweeps (context `fast`) until the residual
     falls below switch_tol (or stops improving at the FP32 rounding floor),
     then FP64 sweeps (context `exact`, same problem and cross sections) from
     that phi to tol; the FP64 residual decides convergence. Both contexts keep
-    their own acceleration. fast's phi is set to the result. Returns
+    their own acceleration; the FP32 phase's grey solves stop after
+    fast_grey_max_its grey iterations (None: fast's own controls, restored
+    afterwards either way). fast's phi is set to the result. Returns
     (converged, FP32 iterations, FP64 iterations, residual, history)."""
     fast.set_stall_window(stall_window)
-    _, it1, _, h1 = fast.source_iteration(switch_tol, max_iters)
+    controls = fast.grey_controls()
+    if fast_grey_max_its is not None:
+        fast.set_grey_controls(max_its=fast_grey_max_its)
+    try:
+        _, it1, _, h1 = fast.source_iteration(switch_tol, max_iters)
+    finally:
+        fast.set_grey_controls(rel_tol=controls[0], max_its=controls[1])
     exact.copy_phi_from(fast)
     ok, it2, res, h2 = exact.source_iteration(tol, max(1, max_iters - it1))
     fast.copy_phi_from(exact)
