@@ -253,7 +253,16 @@ def test_trained_network_cuts_iterations(cfg4, trained):
     V, VZ = p.geometry()["Volume"], p.geometry()["VolumeZone"]
     a, b = OL.zone_average(ctx.phi(), V, VZ), OL.zone_average(plain, V, VZ)
     assert np.max(np.abs(a - b) / b) < 1e-6
+    # The bound is the training's doing: the network with its initial
+    # (untrained) weights does not meet it.
+    untrained = U.Learner(0)
+    ctx.set_phi(np.zeros_like(plain))
+    ctx.set_learner(untrained, every=2)
+    conv2, it2, _, _ = ctx.source_iteration(tol=1e-8, max_iters=1000)
+    say(f"FP64: untrained network {it2} iterations")
+    assert conv2 and it2 > 0.7 * it0
     ctx.close()
+    untrained.close()
 
 
 def test_fp32_learned_then_fp64_matches_plain_fp64(cfg4, trained):

@@ -57,6 +57,32 @@ def test_two_cycles_match_oracle(prob, precision, blocks):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_two_cycles_with_eight_groups_per_lane(precision):
+    """The tau Psi source read K = 8 groups at a time (8 groups, one lane
+    block per corner), against the oracle's time step."""
+    p = U.TiledProblem(2, 1, 1, 2, 2)
+    G = 8
+    st, ss, qq = U.draw_cross_sections(43, p.nzones, G, (0.1, 10.0), (0.0, 0.95), (0.5, 1.0))
+    rng = np.random.default_rng(5)
+    psi0 = rng.uniform(0.0, 0.2, (p.nangles, p.ncorner, G))
+    print(f"case K=8 precision={precision}", flush=True)
+    m, g, q, s = p.mesh(), p.geometry(), p.quadrature(), p.schedule()
+    ctx = ctx_for(p, G, st, ss, qq, psi0, precision)
+    ctx.set_groups_per_lane(8)
+    psi_ref, phi_ref = psi0, None
+    tol = 1e-12 if precision == 64 else 2e-5
+    for cycle in range(2):
+        ctx.source_iteration(tol=0.0, max_iters=4)
+        ctx.save_psi()
+        phi_ref, psi_ref, _, _, _ = O.time_step(m, g, q, s, st, ss, qq, TAU, psi_ref, 0.0, 4,
+                                                phi0=phi_ref)
+        assert rel_err(ctx.phi(), phi_ref) < tol, cycle
+        err = np.max(np.abs(ctx.psi_prev() - psi_ref)) / np.max(np.abs(psi_ref))
+        assert err < tol, (cycle, err)
+    ctx.close()
+
+
 def test_gta_time_step_reaches_the_plain_answer(prob):
     p, G, st, ss, qq, psi0 = prob
     plain = ctx_for(p, G, st, ss, qq, psi0)
