@@ -321,8 +321,12 @@ int umt_ctx_set_gta_options(umt_ctx *ctx, int enforceHardIterMax);
  * Doolittle factors (unit lower below the diagonal, upper on and above). */
 int umt_ctx_grey_transfer(umt_ctx *ctx, double *T, double *LU);
 
-/* Groups each lane of the sweep kernels carries (K = 1, 2, 4 or 8, dividing
- * the group count; default 1). No counterpart in the reference. */
+/* Layout of the all-angle sweep kernels (default 1). K = 1, 2, 4 or 8
+ * (dividing the group count): one lane per corner of a zone, carrying K
+ * groups. K = UMT_ZONE_LAYOUT (-1, group count a multiple of 16): one lane
+ * per group of a zone, carrying its eight corners. The per-angle drop-in
+ * always runs the first. No counterpart in the reference. */
+#define UMT_ZONE_LAYOUT (-1)
 int umt_ctx_set_groups_per_lane(umt_ctx *ctx, int K);
 int umt_ctx_get_groups_per_lane(const umt_ctx *ctx, int *K);
 
@@ -344,11 +348,15 @@ int umt_ctx_get_angles_per_block(const umt_ctx *ctx, int *n, int *blocks);
  * the lagged cycle-list psi when both have the same cycle lists. */
 int umt_ctx_copy_phi(umt_ctx *dst, const umt_ctx *src);
 
-/* Autotuner: times reps sweeps with each supported groups-per-lane value on
- * the context's own problem and keeps the fastest. msPerK[0..3] receives the
+/* Autotuner: times reps sweeps with each supported layout (groups per lane
+ * 1, 2, 4, 8 and the zone layout) on the context's own problem and keeps
+ * the fastest. msPerK[0..3] receives the
  * mean ms per sweep for K = 1, 2, 4, 8 (-1 when unsupported); may be NULL.
  * No counterpart in the reference. */
 int umt_ctx_autotune(umt_ctx *ctx, int reps, int *bestK, double *msPerK);
+/* The same, reporting msPerLayout[0..4] for K = 1, 2, 4, 8 and
+ * UMT_ZONE_LAYOUT (-1 when unsupported). No counterpart in the reference. */
+int umt_ctx_autotune_layouts(umt_ctx *ctx, int reps, int *bestK, double *msPerLayout);
 
 /* n sweeps back to back on the context's stream, one synchronisation at the end. */
 int umt_ctx_sweep_n(umt_ctx *ctx, int n);

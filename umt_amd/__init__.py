@@ -5,12 +5,12 @@ the C-ABI of include/umt_sweep.h and include/umt_rz.h); this package is the
 Python host side.
 """
 from ._lib import LIB_PATH, UmtError, load  # noqa: F401
-from .sweep import (SweepContext, TiledProblem, draw_cross_sections, solve_mixed,  # noqa: F401
-                    sweep_ucbxyz)
+from .sweep import (ZONE_LAYOUT, SweepContext, TiledProblem, draw_cross_sections,  # noqa: F401
+                    solve_mixed, sweep_ucbxyz)
 from .learn import Learner, train_on_iterates  # noqa: F401
 from .batch import SweepBatch  # noqa: F401
 from .rz import RzContext, RzProblem  # noqa: F401
 
-__all__ = ["SweepContext", "TiledProblem", "draw_cross_sections", "sweep_ucbxyz", "load",
+__all__ = ["ZONE_LAYOUT", "SweepContext", "TiledProblem", "draw_cross_sections", "sweep_ucbxyz", "load",
            "UmtError", "LIB_PATH", "Learner", "train_on_iterates", "solve_mixed", "SweepBatch",
            "RzProblem", "RzContext"]

@@ -68,6 +68,7 @@ SIGNATURES = {
     "umt_ctx_set_groups_per_lane": (_i, [_vp, _i]),
     "umt_ctx_get_groups_per_lane": (_i, [_vp, _pi]),
     "umt_ctx_autotune": (_i, [_vp, _i, _pi, _pd]),
+    "umt_ctx_autotune_layouts": (_i, [_vp, _i, _pi, _pd]),
     "umt_problem_gta": (_i, [_vp, _pi, _pd, _pd, _pi, _pi, _pi]),
     "umt_ctx_enable_gta": (_i, [_vp, _i, _pd, _pd, _pi, _pi, _pi]),
     "umt_ctx_set_acceleration": (_i, [_vp, _i]),

@@ -99,6 +99,15 @@ template <typename Real>
 hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStream_t s);
 // Nonzero when the sweep kernels can carry K groups per lane for G groups.
 int groups_per_lane_supported(int G, int K);
+// Layout code of the xyz zone layout (ucb_sweep_level_zone): all eight
+// corners of an item on one lane, one group per lane, W = 64, 32 or 16
+// lanes (the largest dividing G) per item. The corner layout's codes are its
+// groups per lane, K = 1, 2, 4, 8.
+constexpr int kZoneLayout = -1;
+// Lanes per item of the zone layout for G groups (0: G is not a multiple of 16).
+int zone_lanes_for(int G);
+// Nonzero when an xyz all-angle sweep can run layout code K for G groups.
+int sweep_layout_supported(int G, int K);
 template <typename Real>
 hipError_t launch_phi_from_psi(const Real* psi, const double* w, int A, long stride, long n,
                                double* phi, hipStream_t s);

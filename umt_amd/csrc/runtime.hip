@@ -1167,8 +1167,9 @@ int umt_ctx_sweep_n(umt_ctx* x, int n) {
 
 int umt_ctx_set_groups_per_lane(umt_ctx* x, int K) {
   if (!x) return fail(1, "null context");
-  if (!umt::groups_per_lane_supported(x->G, K))
-    return fail(35, "groups per lane %d not supported for %d groups (1, 2, 4 or 8, dividing G)", K, x->G);
+  if (!umt::sweep_layout_supported(x->G, K))
+    return fail(35, "sweep layout %d not supported for %d groups (groups per lane 1, 2, 4 or 8 "
+                    "dividing G, or -1: zone layout, G a multiple of 16)", K, x->G);
   x->groupsPerLane = K;
   return 0;
 }
@@ -1265,14 +1266,14 @@ int umt_ctx_get_groups_per_lane(const umt_ctx* x, int* K) {
   return 0;
 }
 
-int umt_ctx_autotune(umt_ctx* x, int reps, int* bestK, double* msPerK) {
+int umt_ctx_autotune_layouts(umt_ctx* x, int reps, int* bestK, double* msPerLayout) {
   if (!x || reps < 1) return fail(1, "bad argument");
-  static const int cand[4] = {1, 2, 4, 8};
+  static const int cand[5] = {1, 2, 4, 8, umt::kZoneLayout};
   double best = 1e300;
   int bk = x->groupsPerLane;
-  for (int i = 0; i < 4; ++i) {
-    if (msPerK) msPerK[i] = -1.0;
-    if (!umt::groups_per_lane_supported(x->G, cand[i])) continue;
+  for (int i = 0; i < 5; ++i) {
+    if (msPerLayout) msPerLayout[i] = -1.0;
+    if (!umt::sweep_layout_supported(x->G, cand[i])) continue;
     x->groupsPerLane = cand[i];
     if (int rc = sweep(x, false)) return rc;  // warm-up
     HIP_TRY(hipEventRecord(x->ev0, x->stream));
@@ -1283,7 +1284,7 @@ int umt_ctx_autotune(umt_ctx* x, int reps, int* bestK, double* msPerK) {
     float t = 0.f;
     HIP_TRY(hipEventElapsedTime(&t, x->ev0, x->ev1));
     const double ms = t / reps;
-    if (msPerK) msPerK[i] = ms;
+    if (msPerLayout) msPerLayout[i] = ms;
     if (ms < best) {
       best = ms;
       bk = cand[i];
@@ -1291,6 +1292,14 @@ int umt_ctx_autotune(umt_ctx* x, int reps, int* bestK, double* msPerK) {
   }
   x->groupsPerLane = bk;
   if (bestK) *bestK = bk;
+  return 0;
+}
+
+int umt_ctx_autotune(umt_ctx* x, int reps, int* bestK, double* msPerK) {
+  double ms[5];
+  if (int rc = umt_ctx_autotune_layouts(x, reps, bestK, ms)) return rc;
+  if (msPerK)
+    for (int i = 0; i < 4; ++i) msPerK[i] = ms[i];
   return 0;
 }
 

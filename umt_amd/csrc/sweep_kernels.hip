@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 #include "device_math.h"
 #include "kernels.h"
@@ -32,10 +34,15 @@
 
 namespace umt {
 
-// Zeros read in place of the upwind psi of a face that brings none: one
-// group vector of the widest groups-per-lane variant (K = 8).
+// Zeros read in place of the upwind psi of a face that brings none (at
+// least one group vector of the widest groups-per-lane variant). Not const: a
+// const array sits in the constant address space, and a select between it
+// and a psi address makes the upwind loads flat loads, whose waits drain the
+// scalar and LDS counter too; in the global address space they stay global
+// loads. Nothing writes it.
 constexpr int kMaxGroupsPerLane = 8;
-__device__ __align__(64) const double kZeroGroups[kMaxGroupsPerLane] = {};
+// 64 values: the zone layout reads element `lane` of it (lane < 64).
+__device__ __align__(64) double kZeroGroups[64] = {};
 
 // The kernels assume the canonical hex topology of the tiled-mesh generator:
 // the EZ neighbour of local corner c (0-based) in corner-face slots 0..2 is
@@ -352,6 +359,373 @@ __global__ __launch_bounds__(256) void ucb_sweep_level(SweepLevelArgs<Real> p) {
   sweep_corner<Real, K, kSingleAngle>(p, item, g0, valid);
 }
 
+// ---------------------------------------------------------------------------
+// Zone layout: one lane owns all eight corners of one (zone, angle) item for
+// one group, and W consecutive lanes (a slot: W = 16, 32 or 64, dividing G)
+// hold W consecutive groups of the same item, 64 / W slots per wave. Every
+// corner-to-corner coupling of the zone is then inside one lane's registers:
+// no lane moves, each of the zone's 12 EZ edge terms evaluated once (the 4
+// vertical ones only at their upwind corner, behind a branch that is uniform
+// over the slot), and the corner solve as in-register Gauss-Seidel passes in
+// corner order until every corner was computed from final upstream values.
+// Every quantity is the corner layout's expression in the same order (the
+// two agree to a few ulps: the compiler fuses multiply-adds differently in
+// places). Everything that depends on the item but not on the group
+// (Omega.A of the 48 corner faces, the signs, coefficients and upwind
+// operands derived from them, volumes, upwind psi offsets, the number of
+// Gauss-Seidel passes) is computed once per slot by its first 8 lanes, one
+// corner each, and read by all W lanes from LDS (one address per slot:
+// broadcast). All-angle sweeps only (the per-angle drop-in keeps the corner
+// layout).
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>): a loop
+// whose index is a compile-time constant in its body from the start, so
+// register arrays indexed by it (and by constant functions of it) are split
+// into registers before any other optimisation sees them.
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// EZ neighbour of corner c in corner-face slot f, the canonical hex table
+// {3,1,4} {0,2,5} {1,3,6} {2,0,7} {5,7,0} {6,4,1} {7,5,2} {4,6,3} as
+// arithmetic, so that with c and f known at compile time it folds to a
+// constant (register arrays indexed by it stay in registers).
+__device__ __forceinline__ constexpr int kEZ(int c, int f) {
+  return f == 2 ? (c ^ 4)
+                : (c < 4 ? ((c + (f == 0 ? 3 : 1)) & 3) : 4 + ((c + (f == 0 ? 1 : 3)) & 3));
+}
+
+// A compiler-only memory fence: loads after it are not merged with loads
+// before it, so values in LDS are read again where they are used instead of
+// being held in registers in between.
+__device__ __forceinline__ void reread_lds() { asm volatile("" ::: "memory"); }
+// Keeps the scheduler from interleaving the arithmetic of consecutive
+// corners or edges: interleaved, their temporaries are live together and
+// take more registers than the latency hiding is worth.
+__device__ __forceinline__ void one_at_a_time() { __builtin_amdgcn_sched_barrier(0); }
+// x, computed here and opaque to the compiler from here on: it cannot sink
+// the computation of x to a later use (which would keep its operands live
+// across the phases in between), nor turn a select of two such values read
+// from LDS into a read through a selected address.
+template <typename T>
+__device__ __forceinline__ T pinned(T x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// One slot's item as its lanes read it: every quantity that depends on the
+// item but not on the group, computed once by the slot's corner lanes in the
+// form the group lanes use it.
+template <typename Real>
+struct ZoneRec {
+  Real afpIn[8][3];   // Omega.A_fp of incoming FP faces, 0 on the others
+  Real kup[8][3];     // |Omega.A_ez| of upstream EZ faces, 0 on the others (all 0
+                      // for a cyclic zone, which takes no Gauss-Seidel passes)
+  Real kOnce[8][3];   // the same for a cyclic zone's one solve from lagged values
+  Real vol[8];
+  Real outArea[8];    // sum of the outgoing FP and EZ areas of each corner
+  Real ringA[8];      // ring edge owned by c (to its slot-1 neighbour): c's Omega.A_ez
+  Real ringV[8];      //   the upwind corner's volume
+  Real ringM[8];      //   1 where the upwind corner's FP face opposite the edge is incoming
+  Real sg0[8];        // +1 / -1: c upwind / not on its slot-0 EZ face
+  Real sg1[8];        // the same on its slot-1 EZ face
+  Real vertA[4];      // vertical edge (i, i + 4): the upwind corner's Omega.A_ez (<= 0: none)
+  Real vertV[4];      //   its volume
+  Real vertM[4];      //   1 where its FP face 0 (opposite the edge) is incoming
+  Real vertS[8];      // +1 where c adds the vertical edge's term, -1 where it takes it off
+  long off[8][3];     // element offset from psiA + lane of the value each FP face brings
+                      // in: the upwind psi of an incoming face that reads one, else a
+                      // zero of kZeroGroups
+  int cfp[8][3];      // corner or boundary element across each FP face, 0-based
+  int flags[8];       // bit 0: c upwind on its own ring edge; bit 1: the bottom corner of
+                      // vertical edge c (c < 4) is its upwind one; bits 2 + f: psi of c
+                      // goes to the boundary element across outgoing FP face f
+};
+
+template <typename Real, int W>
+__global__ __launch_bounds__(256) void ucb_sweep_level_zone(SweepLevelArgs<Real> p) {
+  static_assert(W == 16 || W == 32 || W == 64, "a slot is 16, 32 or 64 lanes");
+  constexpr int kSlots = 256 / W;
+  __shared__ ZoneRec<Real> rec[kSlots];
+  const int lane = threadIdx.x % W;  // group within the slot's block of W groups
+  ZoneRec<Real>& R = rec[threadIdx.x / W];
+  const unsigned task = (blockIdx.x * 256u + threadIdx.x) / W;
+  const unsigned blocksPerItem = (unsigned)(p.G / W);
+  int item = (int)fast_div(task, p.perItem);
+  const int g = W * (int)(task - (unsigned)item * blocksPerItem) + lane;
+  // Lanes past the level's last item work on item 0 and store nothing.
+  const bool valid = item < p.nItems;
+  if (!valid) item = 0;
+  const int2 it = p.items[item];
+  // A zone whose corners depend on each other in a cycle comes as ~zone.
+  const bool cyclic = it.x < 0;
+  const int z = cyclic ? ~it.x : it.x, a = it.y;
+  const int G = p.G;
+  Real* __restrict__ psiA = p.psi + (long)(a - p.angle0) * p.psiAngleStride;
+
+  // The group's cross section and source need only the item: their loads
+  // are in flight while the prologue runs.
+  const Real sig = p.sigt[(long)z * G + g];
+  Real Q[8];
+  static_for<8>([&](auto C) {
+    constexpr int c = C;
+    Q[c] = p.S[(8L * z + c) * G + g];
+  });
+  if (p.tauPsi) {
+    // Time-dependent source tau Psi (SweepUCBxyz.F90:121): the previous
+    // cycle's psi of angle a, stored like psi with every angle.
+    const Real* tpA = p.tauPsi + (long)a * p.psiAngleStride;
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      Q[c] += p.tau * tpA[(8L * z + c) * G + g];
+    });
+  }
+
+  // Prologue: lane l of a slot computes corner c = l mod 8 (the corner
+  // layout's per-lane geometry; lanes 8 and up repeat the first eight, which
+  // keeps every lane active for the lane reads), and lanes 0..7 store it.
+  const int c = lane & 7;
+  const int slotBase = (int)(threadIdx.x % 64) - lane;  // first lane of the slot in its wave
+  unsigned need = 0;  // upstream EZ neighbours of corner c (lanes 0..7)
+  {
+    const Real ox = (Real)p.omega[3 * a + 0];
+    const Real oy = (Real)p.omega[3 * a + 1];
+    const Real oz = (Real)p.omega[3 * a + 2];
+    const Real* Afz = p.Afp + 72L * z + c;
+    const Real* Aez = p.Aez + 72L * z + c;
+    Real afp[3], aez[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      afp[f] = dot3(ox, oy, oz, Afz + 24 * f, 8);
+      aez[f] = dot3(ox, oy, oz, Aez + 24 * f, 8);
+    }
+    if (p.parTol > 0) {
+      // Directions lying in a mesh face leave rounding noise of either sign
+      // in Omega.A; such faces are parallel here as in the schedule.
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        const Real* af = Afz + 24 * f;
+        const Real* ae = Aez + 24 * f;
+        if (fabs(afp[f]) <= p.parTol * (fabs(af[0]) + fabs(af[8]) + fabs(af[16]))) afp[f] = 0;
+        if (fabs(aez[f]) <= p.parTol * (fabs(ae[0]) + fabs(ae[8]) + fabs(ae[16]))) aez[f] = 0;
+      }
+    }
+    Real outArea = 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (afp[f] > 0) outArea += afp[f];
+      if (aez[f] > 0) outArea += aez[f];
+    }
+    const Real v = p.vol[8L * z + c];
+    int cf[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) cf[f] = p.cFP[24L * z + 8 * f + c] - 1;
+    // Operands from the other corner of this corner's ring edge (slot-1
+    // neighbour) and vertical edge (c ^ 4), read from their lanes.
+    const int n1 = c < 4 ? (c + 1) & 3 : 4 + ((c + 3) & 3);
+    const Real vN = __shfl(v, slotBase + n1, 64);
+    const bool inN1 = __shfl(afp[1] < 0 ? 1 : 0, slotBase + n1, 64) != 0;
+    const Real vV = __shfl(v, slotBase + (c ^ 4), 64);
+    const Real aV = __shfl(aez[2], slotBase + (c ^ 4), 64);
+    const bool inV0 = __shfl(afp[0] < 0 ? 1 : 0, slotBase + (c ^ 4), 64) != 0;
+    const long zeroOff = ((long)(uintptr_t)kZeroGroups - (long)(uintptr_t)psiA) / (long)sizeof(Real);
+    const int g0 = g - lane;  // first group of the slot's block
+    if (lane < 8) {
+      int flags = 0;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        R.afpIn[c][f] = afp[f] < 0 ? afp[f] : (Real)0;
+        const Real k = aez[f] < 0 ? -aez[f] : (Real)0;
+        R.kOnce[c][f] = k;
+        R.kup[c][f] = cyclic ? (Real)0 : k;
+        R.cfp[c][f] = cf[f];
+        // The corner layout's read rule for an incoming FP face.
+        const bool rd = afp[f] < 0 && (p.boundaryIn || cf[f] < p.C) && !p.noInflow;
+        R.off[c][f] = rd ? (long)cf[f] * G + g0 : zeroOff;
+        if (afp[f] > 0 && cf[f] >= p.C) flags |= 4 << f;
+        if (aez[f] < 0 && !cyclic) need |= 1u << (f == 2 ? (c ^ 4) : (f == 1 ? n1 : (c < 4 ? (c + 3) & 3 : 4 + ((c + 1) & 3))));
+      }
+      R.vol[c] = v;
+      R.outArea[c] = outArea;
+      const bool up1 = aez[1] > 0;
+      R.ringA[c] = aez[1];
+      R.ringV[c] = up1 ? v : vN;
+      R.ringM[c] = (up1 ? afp[2] < 0 : inN1) ? (Real)1 : (Real)0;
+      if (up1) flags |= 1;
+      R.sg0[c] = aez[0] > 0 ? (Real)1 : (Real)-1;
+      R.sg1[c] = up1 ? (Real)1 : (Real)-1;
+      // Vertical edge (i, i + 4); Omega.A_ez of i + 4 is minus that of i.
+      const bool upBottom = c < 4 ? aez[2] > 0 : aez[2] < 0;
+      R.vertS[c] = (c < 4) == upBottom ? (Real)1 : (Real)-1;
+      if (c < 4) {
+        R.vertA[c] = upBottom ? aez[2] : aV;
+        R.vertV[c] = upBottom ? v : vV;
+        R.vertM[c] = (upBottom ? afp[0] < 0 : inV0) ? (Real)1 : (Real)0;
+        if (upBottom) flags |= 2;
+      }
+      R.flags[c] = flags;
+    }
+  }
+  // Gauss-Seidel passes the slots of this wave need: the depth of the
+  // deepest corner in the upstream order (after the first solve, a pass
+  // finalises every corner whose upstream corners are final).
+  int passes = 0;
+  {
+    unsigned long long done = __ballot(need == 0);
+    while (done != ~0ull && passes < 8) {
+      const unsigned fin = (unsigned)(done >> slotBase) & 0xffu;
+      done = __ballot((need & ~fin) == 0);
+      ++passes;
+    }
+  }
+  // A slot's record is written and read by the lanes of one wave, whose LDS
+  // operations complete in the order they are issued: a wave-level barrier
+  // (no code motion across it) orders the reads after the writes.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // Upwind psi of the incoming FP faces (SweepUCBxyz.F90:139-161); a face
+  // that brings none reads a zero.
+  Real psf[8][3];
+  static_for<8>([&](auto C) {
+    constexpr int c = C;
+    static_for<3>([&](auto F) {
+      constexpr int f = F;
+      psf[c][f] = psiA[R.off[c][f] + lane];
+    });
+  });
+  // The slot's record is re-read from LDS in each phase below rather than
+  // held in registers across phases.
+  reread_lds();
+
+  // Volume and incoming FP terms (the FP32 form: SweepUCBxyz.F90's order;
+  // FP64 takes the first product off V q in one fused step).
+  Real src[8];
+  static_for<8>([&](auto C) {
+    constexpr int c = C;
+    const Real v = R.vol[c], q = Q[c];
+    Real s;
+    if constexpr (sizeof(Real) == 8) {
+#pragma clang fp contract(off)
+      s = fma(v, q, -(R.afpIn[c][0] * psf[c][0]));
+      s = s - R.afpIn[c][1] * psf[c][1];
+      s = s - R.afpIn[c][2] * psf[c][2];
+    } else {
+      s = v * q;
+      static_for<3>([&](auto F) {
+        constexpr int f = F;
+        s = s - R.afpIn[c][f] * psf[c][f];
+      });
+    }
+    src[c] = pinned(s);
+    one_at_a_time();
+  });
+  reread_lds();
+  // Vertical edges (i, i + 4): the term at the upwind corner (sez_term with
+  // that corner's operands).
+  Real vt[4];
+  static_for<4>([&](auto I) {
+    constexpr int i = I, j = I + 4;
+    const bool up = (pinned(R.flags[i]) & 2) != 0;
+    const Real QU = up ? Q[i] : Q[j], QD = up ? Q[j] : Q[i];
+    const Real pU = up ? psf[i][0] : psf[j][0];
+    const Real vU = R.vertV[i];
+    const Real tU = sig * vU;
+    vt[i] = pinned(sez_term<Real>(R.vertA[i], R.vertM[i], tU, tU * tU, sig, vU, QU, QD, pU));
+    one_at_a_time();
+  });
+  reread_lds();
+  // Ring edges: corner c owns the edge to its slot-1 neighbour n
+  // (ring_term with the upwind corner's operands); the owner adds the term
+  // when upwind and takes it off when not, and so does n with its own sign.
+  Real e[8];
+  static_for<8>([&](auto C) {
+    constexpr int c = C, n = kEZ(C, 1);
+    const bool up1 = (pinned(R.flags[c]) & 1) != 0;
+    const Real w1 = sig * psf[c][2] - Q[c];
+    const Real wN = sig * psf[n][1] - Q[n];
+    const Real vU = R.ringV[c];
+    const Real tU = sig * vU;
+    e[c] = pinned(ring_term<Real>(R.ringA[c], R.ringM[c], tU, tU * tU, sig, vU, up1 ? w1 : wN,
+                                  Q[c] - Q[n]));
+    one_at_a_time();
+  });
+  reread_lds();
+  static_for<8>([&](auto C) {
+    constexpr int c = C;
+    Real s = src[c];
+    s = fma(R.sg1[c], e[c], s);
+    s = fma(R.sg0[c], e[kEZ(C, 0)], s);
+    src[c] = fma(R.vertS[c], vt[c & 3], s);
+  });
+  reread_lds();
+
+  // Corner solve psi_c = (src_c + sum over upstream EZ neighbours of
+  // |aez| psi_n) / den_c, den_c = outgoing areas + sigma V_c.
+  Real rden[8], psi[8];
+  static_for<8>([&](auto C) {
+    constexpr int c = C;
+    rden[c] = quot<Real>(1, R.outArea[c] + sig * R.vol[c]);
+    psi[c] = src[c] * rden[c];
+  });
+  if (cyclic) {
+    // A cyclic zone is solved once from the previous values of its own
+    // corners (SweepUCBxyz.F90:281-295), which psi holds: its corners are on
+    // the angle's cycle list. Its passes then leave it as it is (no
+    // upstream terms, unit reciprocal).
+    Real old[8];
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      old[c] = psiA[(8L * z + c) * G + g];
+    });
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      psi[c] = (src[c] + R.kOnce[c][0] * old[kEZ(C, 0)] + R.kOnce[c][1] * old[kEZ(C, 1)] +
+                R.kOnce[c][2] * old[kEZ(C, 2)]) * rden[c];
+    });
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      src[c] = psi[c];
+      rden[c] = 1;
+    });
+  }
+  for (int pass = 0; pass < passes; ++pass) {
+    reread_lds();
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      psi[c] = (src[c] + R.kup[c][0] * psi[kEZ(C, 0)] + R.kup[c][1] * psi[kEZ(C, 1)] +
+                R.kup[c][2] * psi[kEZ(C, 2)]) * rden[c];
+    });
+  }
+
+  reread_lds();
+  if (valid) {
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      psiA[(8L * z + c) * G + g] = psi[c];
+    });
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      const int fl = R.flags[c];
+      if (fl & 28)
+        static_for<3>([&](auto F) {
+          constexpr int f = F;
+          if (fl & (4 << f)) psiA[(long)R.cfp[c][f] * G + g] = psi[c];
+        });
+    });
+  }
+}
+
+int zone_lanes_for(int G) {
+  return G % 64 == 0 ? 64 : (G % 32 == 0 ? 32 : (G % 16 == 0 ? 16 : 0));
+}
+
 // One level of many domains in one launch. The level's items are grouped by
 // domain into segments; segment s starts at task segs[s].taskStart, where a
 // task is one (item, K groups) pair of 8 lanes. Segments are padded to whole
@@ -612,8 +986,38 @@ hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStr
   return hipGetLastError();
 }
 
+template <typename Real, int W>
+static hipError_t launch_zone(const SweepLevelArgs<Real>& a, hipStream_t s) {
+  const long threads = (long)a.nItems * a.G;  // one lane per (item, group)
+  if (threads == 0) return hipSuccess;
+  if (threads > kMaxLevelThreads) return hipErrorInvalidValue;
+  SweepLevelArgs<Real> b = a;
+  b.perItem = make_fast_div((unsigned)(a.G / W));
+  hipLaunchKernelGGL((ucb_sweep_level_zone<Real, W>), dim3(blocks_for(threads, 256)), dim3(256), 0,
+                     s, b);
+  return hipGetLastError();
+}
+
+int sweep_layout_supported(int G, int K) {
+  return K == kZoneLayout ? zone_lanes_for(G) != 0 : groups_per_lane_supported(G, K);
+}
+
 template <typename Real>
 hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s) {
+  if (a.groupsPerLane == kZoneLayout) {
+    // The zone layout serves all-angle sweeps with G a multiple of 16; the
+    // per-angle drop-in and other G run the corner layout's widest K.
+    if (!singleAngle) switch (zone_lanes_for(a.G)) {
+        case 64: return launch_zone<Real, 64>(a, s);
+        case 32: return launch_zone<Real, 32>(a, s);
+        case 16: return launch_zone<Real, 16>(a, s);
+        default: break;
+      }
+    const int K = a.G % 8 == 0 ? 8 : (a.G % 4 == 0 ? 4 : (a.G % 2 == 0 ? 2 : 1));
+    SweepLevelArgs<Real> b = a;
+    b.groupsPerLane = K;
+    return launch_sweep_level<Real>(b, singleAngle, s);
+  }
   switch (a.groupsPerLane) {
     case 8: return launch_k<Real, 8>(a, singleAngle, s);
     case 4: return launch_k<Real, 4>(a, singleAngle, s);

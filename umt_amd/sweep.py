@@ -23,6 +23,11 @@ from ._lib import check
 _pi = C.POINTER(C.c_int)
 _pd = C.POINTER(C.c_double)
 
+# Layout code of the zone layout for SweepContext.set_groups_per_lane: one
+# lane per group of a zone carrying its eight corners (include/umt_sweep.h
+# UMT_ZONE_LAYOUT); 1, 2, 4, 8 are the corner layout's groups per lane.
+ZONE_LAYOUT = -1
+
 
 def _ip(a):
     return a.ctypes.data_as(_pi) if a is not None else None
@@ -225,12 +230,13 @@ class SweepContext:
         return k.value
 
     def autotune(self, reps=2):
-        """Time each supported groups-per-lane value; keep the fastest.
-        Returns (best K, {K: ms per sweep})."""
+        """Time each supported layout (groups per lane 1, 2, 4, 8 and
+        ZONE_LAYOUT); keep the fastest. Returns (best K, {K: ms per sweep})."""
         k = C.c_int(0)
-        ms = np.zeros(4)
-        check("umt_ctx_autotune", _lib.load().umt_ctx_autotune(self._h, reps, C.byref(k), _dp(ms)))
-        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4, 8), ms) if t >= 0}
+        ms = np.zeros(5)
+        check("umt_ctx_autotune_layouts",
+              _lib.load().umt_ctx_autotune_layouts(self._h, reps, C.byref(k), _dp(ms)))
+        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4, 8, ZONE_LAYOUT), ms) if t >= 0}
 
     def autotune_grey(self, reps=2):
         """The same for the S2 grey sweeps of GTA. Returns (best K, {K: ms per S2 sweep})."""
