@@ -92,16 +92,27 @@ def load_profile(name):
         return None
 
 
-def measured_traffic(precision, tiles, G, A, apb):
+def level_kernel_name(precision, layout, G):
+    """The level kernel a sweep layout runs, as rocprofv3 names it."""
+    t = "double" if precision == 64 else "float"
+    if layout == -1:  # zone layout: one lane per group, W lanes per zone
+        W = 64 if G % 64 == 0 else (32 if G % 32 == 0 else 16)
+        return f"umt::ucb_sweep_level_zone<{t}, {W}>"
+    return f"umt::ucb_sweep_level<{t}, {layout}, false>"
+
+
+def measured_traffic(precision, tiles, G, A, apb, layout):
     """HBM bytes per launch of the level kernel from the committed
-    rocprofv3 --pmc summary for this workload and angle blocking, if any."""
+    rocprofv3 --pmc summary for this workload, angle blocking and kernel,
+    if any."""
     t = "x".join(str(v) for v in tiles)
     prof = load_profile(f"ucb_sweep_level_fp{precision}_{t}_traffic.json")
     if not prof:
         return None
     w = prof.get("workload_key", {})
     if (tuple(w.get("tiles", ())) != tuple(tiles) or w.get("groups") != G or w.get("angles") != A
-            or w.get("angles_per_block") != apb):
+            or w.get("angles_per_block") != apb
+            or prof.get("kernel") != level_kernel_name(precision, layout, G)):
         return None
     return prof
 
@@ -255,7 +266,7 @@ def roofline_block(precision, r, unknowns, G, A, tiles, apb):
     mean_launch_s = r["ms_levels"] * 1e-3 / launches
     per_launch = unknowns / launches
     achieved = B * per_launch / mean_launch_s / 1e9
-    prof = measured_traffic(precision, tiles, G, A, apb)
+    prof = measured_traffic(precision, tiles, G, A, apb, r["groups_per_lane"])
     traffic = prof["hbm_bytes_per_launch"] if prof else None
     rs = 8 if precision == 64 else 4
     # Whole sweep: the level kernels plus the phi reduction's psi re-read and
@@ -265,7 +276,7 @@ def roofline_block(precision, r, unknowns, G, A, tiles, apb):
     blk = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-        "kernel": f"ucb_sweep_level<{'double' if precision == 64 else 'float'},{r['groups_per_lane']},false>",
+        "kernel": level_kernel_name(precision, r["groups_per_lane"], G),
         "byte_model": ("psi write + n_in upwind psi reads + (S + sigma_t + corner geometry) once per "
                        "(level, zone) visit shared by d directions + (zone, angle) items"),
         "algorithmic_bytes_per_unknown": B, "n_in": r["n_in"], "directions_per_zone_visit": d,

@@ -604,8 +604,23 @@ __global__ __launch_bounds__(256) void ucb_sweep_level_zone(SweepLevelArgs<Real>
   // held in registers across phases.
   reread_lds();
 
-  // Volume and incoming FP terms (the FP32 form: SweepUCBxyz.F90's order;
-  // FP64 takes the first product off V q in one fused step).
+  // Vertical edges (i, i + 4): the term at the upwind corner (sez_term with
+  // that corner's operands). Computed first: it is the last use of face 0's
+  // upwind psi but one, and 4 terms take fewer registers than 8 values.
+  Real vt[4];
+  static_for<4>([&](auto I) {
+    constexpr int i = I, j = I + 4;
+    const bool up = (pinned(R.flags[i]) & 2) != 0;
+    const Real QU = up ? Q[i] : Q[j], QD = up ? Q[j] : Q[i];
+    const Real pU = up ? psf[i][0] : psf[j][0];
+    const Real vU = R.vertV[i];
+    const Real tU = sig * vU;
+    vt[i] = pinned(sez_term<Real>(R.vertA[i], R.vertM[i], tU, tU * tU, sig, vU, QU, QD, pU));
+    one_at_a_time();
+  });
+  reread_lds();
+  // Volume and incoming FP terms (FP64 takes the first product off V q in
+  // one fused step).
   Real src[8];
   static_for<8>([&](auto C) {
     constexpr int c = C;
@@ -627,42 +642,42 @@ __global__ __launch_bounds__(256) void ucb_sweep_level_zone(SweepLevelArgs<Real>
     one_at_a_time();
   });
   reread_lds();
-  // Vertical edges (i, i + 4): the term at the upwind corner (sez_term with
-  // that corner's operands).
-  Real vt[4];
-  static_for<4>([&](auto I) {
-    constexpr int i = I, j = I + 4;
-    const bool up = (pinned(R.flags[i]) & 2) != 0;
-    const Real QU = up ? Q[i] : Q[j], QD = up ? Q[j] : Q[i];
-    const Real pU = up ? psf[i][0] : psf[j][0];
-    const Real vU = R.vertV[i];
-    const Real tU = sig * vU;
-    vt[i] = pinned(sez_term<Real>(R.vertA[i], R.vertM[i], tU, tU * tU, sig, vU, QU, QD, pU));
-    one_at_a_time();
-  });
-  reread_lds();
   // Ring edges: corner c owns the edge to its slot-1 neighbour n
   // (ring_term with the upwind corner's operands); the owner adds the term
   // when upwind and takes it off when not, and so does n with its own sign.
-  Real e[8];
-  static_for<8>([&](auto C) {
+  // In each ring the owners go round in order, so a term is added to its
+  // owner at once and to n (whose own term comes first) one edge later:
+  // at most two terms are held.
+  auto ring = [&](auto C) {
     constexpr int c = C, n = kEZ(C, 1);
     const bool up1 = (pinned(R.flags[c]) & 1) != 0;
     const Real w1 = sig * psf[c][2] - Q[c];
     const Real wN = sig * psf[n][1] - Q[n];
     const Real vU = R.ringV[c];
     const Real tU = sig * vU;
-    e[c] = pinned(ring_term<Real>(R.ringA[c], R.ringM[c], tU, tU * tU, sig, vU, up1 ? w1 : wN,
-                                  Q[c] - Q[n]));
+    const Real e = pinned(ring_term<Real>(R.ringA[c], R.ringM[c], tU, tU * tU, sig, vU,
+                                          up1 ? w1 : wN, Q[c] - Q[n]));
+    src[c] = fma(R.sg1[c], e, src[c]);
     one_at_a_time();
+    return e;
+  };
+  static_for<2>([&](auto H) {
+    // Ring H: corners 4H .. 4H + 3 in the order each owns the edge its
+    // slot-0 neighbour takes from it (bottom 0 1 2 3, top 4 7 6 5).
+    constexpr int r0 = 4 * H, r1 = H ? 7 : 1, r2 = H ? 6 : 2, r3 = H ? 5 : 3;
+    const Real e0 = ring(std::integral_constant<int, r0>{});
+    const Real e1 = ring(std::integral_constant<int, r1>{});
+    src[r1] = fma(R.sg0[r1], e0, src[r1]);
+    const Real e2 = ring(std::integral_constant<int, r2>{});
+    src[r2] = fma(R.sg0[r2], e1, src[r2]);
+    const Real e3 = ring(std::integral_constant<int, r3>{});
+    src[r3] = fma(R.sg0[r3], e2, src[r3]);
+    src[r0] = fma(R.sg0[r0], e3, src[r0]);
   });
   reread_lds();
   static_for<8>([&](auto C) {
     constexpr int c = C;
-    Real s = src[c];
-    s = fma(R.sg1[c], e[c], s);
-    s = fma(R.sg0[c], e[kEZ(C, 0)], s);
-    src[c] = fma(R.vertS[c], vt[c & 3], s);
+    src[c] = fma(R.vertS[c], vt[c & 3], src[c]);
   });
   reread_lds();
 
