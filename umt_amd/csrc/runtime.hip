@@ -747,10 +747,12 @@ static int block_set(umt_ctx* x, SweepSet& set, int n) {
   // XCD's L2 from level to level.
   {
     constexpr int kXcd = 8;
-    // Items per 256-lane workgroup for the widest groups-per-lane layout G
-    // allows (the autotuner's usual pick); another layout only mixes ranges.
+    // Items per 256-lane workgroup of the layout the autotuner usually
+    // picks: the zone layout (one lane per group) where G allows it, else
+    // the widest groups-per-lane layout; another layout only mixes ranges.
     const int K = x->G % 4 == 0 ? 4 : (x->G % 2 == 0 ? 2 : 1);
-    const int perWg = std::max(1, 32 / std::max(1, x->G / K));
+    const int perWg = umt::zone_lanes_for(x->G) ? std::max(1, 256 / x->G)
+                                                : std::max(1, 32 / std::max(1, x->G / K));
     std::vector<int2> out;
     for (size_t h = 0; h + 1 < lo.size(); ++h) {
       std::array<int, kXcd + 1> start;
