@@ -565,7 +565,8 @@ def batched_domains(n_domains, seed, precisions=(32, 64)):
     domain's. The first precision: the batched sweep (autotuned groups per
     lane and angles per block) against the same domains swept one context at
     a time; the others: the batched sweep with the first one's groups per
-    lane and blocks of 32 angles."""
+    lane and blocks of 32 angles. Each is timed with one launch per level
+    and with one persistent launch per angle block."""
     import umt_amd as U
 
     rng = np.random.default_rng(seed)
@@ -617,14 +618,29 @@ def batched_domains(n_domains, seed, precisions=(32, 64)):
             # angle blocks of 32 (the FP64 psi of every angle does not fit).
             K, n = K32, 32
             b.set_layout(K, n)
-        ms = b.time_iterations(1, 5)
+        # One launch per level against one persistent launch per angle block
+        # (grid barrier between levels); "batched" is the faster.
+        modes = {}
+        for persistent in (False, True):
+            b.set_persistent(persistent)
+            ms_mode = b.time_iterations(1, 5)
+            dm = b.dims()
+            modes[persistent] = {"ms_per_iteration": ms_mode,
+                                 "unknowns_per_s": dm["unknowns"] / (ms_mode * 1e-3),
+                                 "launches_per_sweep": dm["launches"]}
+        modes[True]["workgroups"] = b.persistent()[1]
+        best = min(modes, key=lambda m: modes[m]["ms_per_iteration"])
+        b.set_persistent(best)
+        ms = modes[best]["ms_per_iteration"]
         d = b.dims()
         key = "fp64" if prec == 64 else "fp32"
         out["unknowns_per_sweep"] = d["unknowns"]
         out[key] = {
             "batched": {"ms_per_iteration": ms, "unknowns_per_s": d["unknowns"] / (ms * 1e-3),
                         "launches_per_sweep": d["launches"], "groups_per_lane": K,
-                        "angles_per_block": n, "angle_blocks": d["blocks"]},
+                        "angles_per_block": n, "angle_blocks": d["blocks"], "persistent": best},
+            "one_launch_per_level": modes[False],
+            "persistent_launch_per_angle_block": modes[True],
         }
         if seq_ms > 0:
             out[key]["one_context_at_a_time"] = {"ms_per_sweep_sum": seq_ms,

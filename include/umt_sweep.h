@@ -407,8 +407,8 @@ int umt_batch_create(int nDomains, const umt_problem *const *problems, const int
                      int precision, umt_batch **out);
 void umt_batch_destroy(umt_batch *batch);
 /* dims[0..5] = domains, unknowns per batched sweep (corners x angles x groups
- * summed), level launches per sweep, groups per lane, angles per block,
- * angle blocks. */
+ * summed), sweep-kernel launches per sweep (levels, or angle blocks when
+ * persistent), groups per lane, angles per block, angle blocks. */
 int umt_batch_dims(const umt_batch *batch, int64_t *dims);
 /* sigt, sigs, q (groups[d], nzones of domain d). */
 int umt_batch_set_cross_sections(umt_batch *batch, int domain, const double *sigt,
@@ -428,11 +428,24 @@ int umt_batch_source_iteration(umt_batch *batch, double tol, int maxIters, int *
 int umt_batch_set_graphs(umt_batch *batch, int on);
 /* Groups per lane K (1, 2, 4) and angles per block for the batched sweep. */
 int umt_batch_set_layout(umt_batch *batch, int groupsPerLane, int anglesPerBlock);
+/* on = 1: every level of an angle block in one persistent launch (a grid
+ * the device holds at once walks the levels with a grid barrier between
+ * them; same results as one launch per level). A workgroup waits at most
+ * maxPolls polls at a barrier (0: about a second; < 0: none, so that a
+ * sweep of two or more workgroups fails, to exercise the error path) and
+ * then gives up; the sweep then fails with an error and its results are
+ * invalid. No
+ * counterpart in the reference (its levels are a loop inside one kernel,
+ * gpu/GPU_SweepUCBxyz.cu:278-515, for one angle at a time). */
+int umt_batch_set_persistent(umt_batch *batch, int on, int maxPolls);
+/* Whether the persistent sweep is on, and its workgroups. */
+int umt_batch_get_persistent(const umt_batch *batch, int *on, int *blocks);
 /* Mean ms of one source iteration (with residuals) over reps, after warmup. */
 int umt_batch_time_iterations(umt_batch *batch, int warmup, int reps, double *msPerIteration);
 /* Autotuner: times every K in {1, 2, 4} with angles per block in {8, 16, 32,
- * 64, all} (skipping blocks whose psi would not fit in free HBM), keeps the
- * fastest and resets phi. */
+ * 64, all} (skipping blocks whose psi would not fit in free HBM) with one
+ * launch per level, then the persistent sweep at the fastest of those;
+ * keeps the fastest and resets phi. */
 int umt_batch_autotune(umt_batch *batch, int reps, int *bestK, int *bestAnglesPerBlock,
                        double *msPerIteration);
 

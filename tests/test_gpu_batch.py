@@ -189,3 +189,90 @@ def test_config3_full_batch_fp32():
     which = sorted({int(np.argmax(sizes)), int(np.argmin(sizes)), 5, 17, 29, 41, 53, 63})
     _check_against_oracle(b, doms, which, 2e-5)
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# The persistent sweep: every level of an angle block in one launch, with a
+# grid barrier between levels. The per-task work is the per-level kernel's,
+# so both must give the same bits.
+
+
+@pytest.mark.parametrize("precision,layout", [(64, (4, 1000)), (64, (2, 5)), (32, (1, 7))])
+def test_persistent_sweep_gives_the_per_level_bits(domains, precision, layout):
+    print(f"case persistent FP{precision} layout {layout}", flush=True)
+    out = {}
+    for persistent in (False, True):
+        b = make_batch(domains, precision=precision)
+        b.set_layout(*layout)
+        b.set_persistent(persistent)
+        on, blocks = b.persistent()
+        assert on == persistent and blocks >= 1
+        d = b.dims()
+        assert d["launches"] == (d["blocks"] if persistent else d["launches"])
+        b.sweep()
+        b.sweep()
+        out[persistent] = [b.phi(i) for i in range(len(domains))]
+        b.close()
+    for i in range(len(domains)):
+        np.testing.assert_array_equal(out[True][i], out[False][i])
+
+
+def test_persistent_source_iteration_with_graph_replay_on_config3_domains():
+    """Sixteen domains drawn like the benchmark's batch: graph-replayed
+    persistent source iteration takes the per-level sweep's iterations to its
+    bits, and each domain's phi matches the oracle's sweep afterwards."""
+    doms = _draw(16, 20261217, (4, 6))
+    runs = []
+    for persistent in (False, True):
+        b = make_batch(doms)
+        b.set_layout(4, 1000)
+        b.set_persistent(persistent)
+        b.set_graphs(True)
+        ok, its, res, tot = b.source_iteration(tol=1e-8, max_iters=300)
+        runs.append((ok, list(its), tot, [b.phi(i) for i in range(len(doms))], b.persistent()))
+        b.close()
+    (ok0, its0, tot0, phi0, _), (ok1, its1, tot1, phi1, (on, blocks)) = runs
+    print(f"persistent SI: {tot1} iterations on {blocks} workgroups", flush=True)
+    assert on and blocks > 1
+    assert ok0 and ok1 and its0 == its1 and tot0 == tot1
+    for a, c in zip(phi0, phi1):
+        np.testing.assert_array_equal(a, c)
+
+
+def test_persistent_barrier_that_gives_up_is_an_error():
+    """A grid barrier that waits for nobody (max_polls < 0) makes every
+    workgroup but the last to arrive give up: the sweep fails with an error
+    instead of returning wrong results, and after it the barrier is clear
+    and a persistent sweep with the default wait matches per-level launches."""
+    doms = _draw(8, 20261219, (5, 6))
+    b = make_batch(doms)
+    b.set_layout(4, 1000)
+    b.set_persistent(True, max_polls=-1)
+    assert b.persistent()[1] > 1
+    with pytest.raises(U.UmtError):
+        b.sweep()
+    b.reset()
+    b.set_persistent(True)
+    b.sweep()
+    got = [b.phi(i) for i in range(len(doms))]
+    b.close()
+    ref = make_batch(doms)
+    ref.set_layout(4, 1000)
+    ref.sweep()
+    for i in range(len(doms)):
+        np.testing.assert_array_equal(got[i], ref.phi(i))
+    ref.close()
+
+
+def test_batch_autotuner_tries_the_persistent_sweep(domains):
+    b = make_batch(domains)
+    K, n, ms = b.autotune(reps=1)
+    on, blocks = b.persistent()
+    print(f"autotuned K={K} n={n} persistent={on} ({blocks} workgroups), {ms:.3f} ms", flush=True)
+    assert blocks >= 1 and ms > 0
+    b.sweep()
+    for i, (p, G, (st, ss, qq)) in enumerate(domains):
+        S = np.repeat(qq, 8, axis=0) / (4 * np.pi)
+        ref, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st)
+        assert rel_err(b.phi(i), ref) < 1e-12, i
+    b.close()

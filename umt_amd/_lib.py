@@ -105,6 +105,8 @@ SIGNATURES = {
     "umt_batch_set_layout": (_i, [_vp, _i, _i]),
     "umt_batch_time_iterations": (_i, [_vp, _i, _i, _pd]),
     "umt_batch_autotune": (_i, [_vp, _i, _pi, _pi, _pd]),
+    "umt_batch_set_persistent": (_i, [_vp, _i, _i]),
+    "umt_batch_get_persistent": (_i, [_vp, _pi, _pi]),
     "umt_learner_num_params": (_i, []),
     "umt_learner_create": (_i, [_u64, C.POINTER(_vp)]),
     "umt_learner_destroy": (None, [_vp]),

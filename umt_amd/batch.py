@@ -88,6 +88,21 @@ class SweepBatch:
         check("umt_batch_set_layout",
               _lib.load().umt_batch_set_layout(self._h, groups_per_lane, angles_per_block))
 
+    def set_persistent(self, on, max_polls=0):
+        """on: every level of an angle block in one persistent launch, with a
+        grid barrier between levels. A workgroup polls a barrier at most
+        max_polls times (0: about a second; < 0: not at all, which makes a
+        sweep of two or more workgroups fail) before the sweep fails."""
+        check("umt_batch_set_persistent",
+              _lib.load().umt_batch_set_persistent(self._h, int(bool(on)), int(max_polls)))
+
+    def persistent(self):
+        """(on, workgroups of the persistent launch)."""
+        on, blocks = C.c_int(0), C.c_int(0)
+        check("umt_batch_get_persistent",
+              _lib.load().umt_batch_get_persistent(self._h, C.byref(on), C.byref(blocks)))
+        return bool(on.value), blocks.value
+
     def time_iterations(self, warmup=1, reps=3):
         ms = C.c_double(0.0)
         check("umt_batch_time_iterations",

@@ -40,6 +40,9 @@ const Schedule& problem_schedule(const umt_problem* p);
 namespace {
 
 constexpr int kGroupPad = 4;
+// Default bound on a workgroup's polls at a grid barrier of the persistent
+// sweep: about a second, far above any level's time.
+constexpr unsigned kBarrierPolls = 1u << 20;
 
 struct Domain {
   int Z = 0, C = 0, nb = 0, G = 0, Gp = 0, A = 0;
@@ -84,6 +87,15 @@ struct umt_batch {
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   long long unknowns = 0;
+  // Persistent sweep: each angle block's levels in one launch of a grid the
+  // device holds at once, with a grid barrier between levels.
+  bool persistent = false;
+  unsigned spinLimit = 0;            // polls at a barrier before a workgroup gives up
+  int persistentBlocks = 0;          // workgroups of that launch (0: no resident grid)
+  long* levelItemD = nullptr;        // levelItem, levelSeg, levelTasks on the device
+  int* levelSegD = nullptr;
+  long* levelTasksD = nullptr;
+  umt::GridBarrier* bar = nullptr;
   size_t rsize() const { return precision == 32 ? sizeof(float) : sizeof(double); }
 };
 
@@ -238,6 +250,25 @@ int layout(umt_batch* b, int n, int K) {
   b->levelSeg = levelSeg;
   b->levelTasks = levelTasks;
   b->blockLevel = blockLevel;
+  dfree(b->levelItemD);
+  dfree(b->levelSegD);
+  dfree(b->levelTasksD);
+  if ((rc = dalloc(&b->levelItemD, levelItem.size())) || (rc = dalloc(&b->levelSegD, levelSeg.size())) ||
+      (rc = dalloc(&b->levelTasksD, std::max<size_t>(1, levelTasks.size()))))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(b->levelItemD, levelItem.data(), levelItem.size() * sizeof(long),
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->levelSegD, levelSeg.data(), levelSeg.size() * sizeof(int),
+                         hipMemcpyHostToDevice, b->stream));
+  if (!levelTasks.empty())
+    HIP_TRY(hipMemcpyAsync(b->levelTasksD, levelTasks.data(), levelTasks.size() * sizeof(long),
+                           hipMemcpyHostToDevice, b->stream));
+  // The persistent grid: as many workgroups as the device holds at once,
+  // and no more than the widest level has work for (32 tasks each).
+  long widest = 1;
+  for (long t : levelTasks) widest = std::max(widest, (t + 31) / 32);
+  const int resident = umt::batch_persistent_resident_blocks(b->precision, K);
+  b->persistentBlocks = (int)std::min<long>(resident, widest);
   b->anglesPerBlock = n;
   b->K = K;
   if ((rc = b->precision == 32 ? upload_descs<float>(b) : upload_descs<double>(b))) return rc;
@@ -254,15 +285,32 @@ int launch_iteration(umt_batch* b, bool withResidual) {
                                             b->stream));
   for (int blk = 0; blk + 1 < (int)b->blockLevel.size(); ++blk) {
     const int a0 = blk * b->anglesPerBlock;
-    for (int h = b->blockLevel[blk]; h < b->blockLevel[blk + 1]; ++h) {
-      umt::BatchLevelArgs<Real> a{};
+    if (b->persistent) {
+      umt::BatchPersistentArgs<Real> a{};
       a.doms = (const umt::BatchDomain<Real>*)b->descs;
-      a.items = b->items + b->levelItem[h];
-      a.segs = b->segs + b->levelSeg[h];
-      a.nSegs = b->levelSeg[h + 1] - b->levelSeg[h];
-      a.nTasks = b->levelTasks[h];
+      a.items = b->items;
+      a.segs = b->segs;
+      a.levelItem = b->levelItemD;
+      a.levelSeg = b->levelSegD;
+      a.levelTasks = b->levelTasksD;
+      a.h0 = b->blockLevel[blk];
+      a.h1 = b->blockLevel[blk + 1];
       a.angle0 = a0;
-      HIP_TRY(umt::launch_sweep_batch_level<Real>(a, b->K, b->stream));
+      a.bar = b->bar;
+      a.spinLimit = b->spinLimit;
+      if (a.h1 > a.h0)
+        HIP_TRY(umt::launch_sweep_batch_persistent<Real>(a, b->K, b->persistentBlocks, b->stream));
+    } else {
+      for (int h = b->blockLevel[blk]; h < b->blockLevel[blk + 1]; ++h) {
+        umt::BatchLevelArgs<Real> a{};
+        a.doms = (const umt::BatchDomain<Real>*)b->descs;
+        a.items = b->items + b->levelItem[h];
+        a.segs = b->segs + b->levelSeg[h];
+        a.nSegs = b->levelSeg[h + 1] - b->levelSeg[h];
+        a.nTasks = b->levelTasks[h];
+        a.angle0 = a0;
+        HIP_TRY(umt::launch_sweep_batch_level<Real>(a, b->K, b->stream));
+      }
     }
     for (Domain& d : b->dom) {
       if (a0 >= d.A) continue;
@@ -302,6 +350,21 @@ int iteration(umt_batch* b, bool withResidual) {
   }
   HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   return 0;
+}
+
+// After a persistent sweep: fails (and clears the barrier for the next
+// launch) if a workgroup gave up waiting at a grid barrier.
+int barrier_status(umt_batch* b) {
+  if (!b->persistent) return 0;
+  umt::GridBarrier h{};
+  HIP_TRY(hipMemcpyAsync(&h, b->bar, sizeof h, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!h.err) return 0;
+  HIP_TRY(hipMemsetAsync(b->bar, 0, sizeof(umt::GridBarrier), b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return fail(60, "batch: a workgroup of the persistent sweep gave up after %u polls at a grid "
+                  "barrier (the grid was not resident at once); this sweep's results are invalid: "
+                  "use per-level launches (umt_batch_set_persistent(b, 0, 0))", b->spinLimit);
 }
 
 void free_domain(Domain& d) {
@@ -362,6 +425,7 @@ void umt_batch_destroy(umt_batch* b) {
   drop_graph(b);
   for (Domain& d : b->dom) free_domain(d);
   dfree(b->descs); dfree(b->items); dfree(b->segs); dfree(b->bits);
+  dfree(b->levelItemD); dfree(b->levelSegD); dfree(b->levelTasksD); dfree(b->bar);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -394,8 +458,11 @@ int umt_batch_create(int nDomains, const umt_problem* const* problems, const int
   int rc;
   if ((rc = dalloc((char**)&b->descs, (size_t)nDomains * (precision == 32 ? sizeof(umt::BatchDomain<float>)
                                                                           : sizeof(umt::BatchDomain<double>)))) ||
-      (rc = dalloc(&b->bits, 2 * (size_t)nDomains)))
+      (rc = dalloc(&b->bits, 2 * (size_t)nDomains)) || (rc = dalloc(&b->bar, 1)))
     return bail(rc);
+  if (hipMemsetAsync(b->bar, 0, sizeof(umt::GridBarrier), b->stream) != hipSuccess)
+    return bail(fail(2, "hipMemsetAsync failed"));
+  b->spinLimit = kBarrierPolls;
   // Start with blocks of at most 32 angles (psi for all angles of every
   // domain may not fit); the autotuner tries larger blocks where they fit.
   if ((rc = layout(b, std::min(b->maxA, 32), 4))) return bail(rc);
@@ -407,7 +474,7 @@ int umt_batch_dims(const umt_batch* b, int64_t* d) {
   if (!b || !d) return fail(1, "null argument");
   d[0] = b->D;
   d[1] = b->unknowns;
-  d[2] = (int64_t)b->levelTasks.size();
+  d[2] = b->persistent ? (int64_t)b->blockLevel.size() - 1 : (int64_t)b->levelTasks.size();
   d[3] = b->K;
   d[4] = b->anglesPerBlock;
   d[5] = (int64_t)b->blockLevel.size() - 1;
@@ -464,6 +531,24 @@ int umt_batch_sweep(umt_batch* b) {
   if (!b) return fail(1, "null batch");
   if (int rc = iteration(b, false)) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
+  return barrier_status(b);
+}
+
+int umt_batch_set_persistent(umt_batch* b, int on, int maxPolls) {
+  if (!b) return fail(1, "null batch");
+  if (on && b->persistentBlocks < 1)
+    return fail(61, "batch: the device reports no resident grid for the persistent sweep");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  drop_graph(b);
+  b->persistent = on != 0;
+  b->spinLimit = maxPolls > 0 ? (unsigned)maxPolls : maxPolls < 0 ? 0u : kBarrierPolls;
+  return 0;
+}
+
+int umt_batch_get_persistent(const umt_batch* b, int* on, int* blocks) {
+  if (!b) return fail(1, "null batch");
+  if (on) *on = b->persistent ? 1 : 0;
+  if (blocks) *blocks = b->persistentBlocks;
   return 0;
 }
 
@@ -500,6 +585,7 @@ int umt_batch_source_iteration(umt_batch* b, double tol, int maxIters, int* iter
     HIP_TRY(hipMemcpyAsync(bits.data(), b->bits, bits.size() * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    if (int rc = barrier_status(b)) return rc;
     ++it;
     all = true;
     for (int i = 0; i < b->D; ++i) {
@@ -532,7 +618,7 @@ int umt_batch_time_iterations(umt_batch* b, int warmup, int reps, double* msPerI
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   if (msPerIteration) *msPerIteration = ms / reps;
-  return 0;
+  return barrier_status(b);
 }
 
 int umt_batch_autotune(umt_batch* b, int reps, int* bestK, int* bestAnglesPerBlock, double* msBest) {
@@ -543,6 +629,7 @@ int umt_batch_autotune(umt_batch* b, int reps, int* bestK, int* bestAnglesPerBlo
     if (n <= b->maxA && std::find(blocks.begin(), blocks.end(), n) == blocks.end()) blocks.push_back(n);
   double best = HUGE_VAL;
   int bk = b->K, bn = b->anglesPerBlock;
+  if (int rc = umt_batch_set_persistent(b, 0, 0)) return rc;  // candidates with per-level launches
   for (int n : blocks) {
     // Skip block sizes whose psi buffers would not fit in free memory.
     size_t need = 0;
@@ -564,6 +651,14 @@ int umt_batch_autotune(umt_batch* b, int reps, int* bestK, int* bestAnglesPerBlo
     }
   }
   if (int rc = layout(b, bn, bk)) return rc;
+  // The persistent sweep at that layout, kept if faster.
+  if (b->persistentBlocks > 0) {
+    if (int rc = umt_batch_set_persistent(b, 1, 0)) return rc;
+    double ms = 0.0;
+    if (int rc = umt_batch_time_iterations(b, 1, reps, &ms)) return rc;
+    if (ms < best) best = ms;
+    else if (int rc = umt_batch_set_persistent(b, 0, 0)) return rc;
+  }
   if (int rc = umt_batch_reset(b)) return rc;
   if (bestK) *bestK = bk;
   if (bestAnglesPerBlock) *bestAnglesPerBlock = bn;

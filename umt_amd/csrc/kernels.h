@@ -97,6 +97,33 @@ struct BatchLevelArgs {
 };
 template <typename Real>
 hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStream_t s);
+// Barrier state of the persistent batch kernel (device memory, zeroed before
+// first use; count returns to 0 after every completed barrier).
+struct GridBarrier {
+  unsigned count;           // workgroups arrived at the current barrier
+  unsigned gen;             // barriers completed
+  unsigned err;             // nonzero: a workgroup gave up waiting (grid not resident)
+  unsigned pad;
+};
+// All levels [h0, h1) of one angle block of a batch in one launch.
+template <typename Real>
+struct BatchPersistentArgs {
+  const BatchDomain<Real>* doms;
+  const int2* items;        // every level's items
+  const BatchSeg* segs;     // every level's segments
+  const long* levelItem;    // first item of level h
+  const int* levelSeg;      // first segment of level h (h + 1: end)
+  const long* levelTasks;   // padded tasks of level h
+  int h0, h1;
+  int angle0;
+  GridBarrier* bar;
+  unsigned spinLimit;       // polls a workgroup waits at a barrier before giving up
+};
+template <typename Real>
+hipError_t launch_sweep_batch_persistent(const BatchPersistentArgs<Real>& b, int K, int blocks,
+                                         hipStream_t s);
+// Workgroups of the persistent batch kernel the device holds at once.
+int batch_persistent_resident_blocks(int precision, int K);
 // Nonzero when the sweep kernels can carry K groups per lane for G groups.
 int groups_per_lane_supported(int G, int K);
 // Layout code of the xyz zone layout (ucb_sweep_level_zone): all eight

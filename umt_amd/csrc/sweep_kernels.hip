@@ -749,8 +749,7 @@ int zone_lanes_for(int G) {
 // way, and runs the same corner code as the one-domain kernel; padding tasks
 // compute the segment's first item and store nothing.
 template <typename Real, int K>
-__global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real> b) {
-  const unsigned task = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+__device__ __forceinline__ void batch_task(const BatchLevelArgs<Real>& b, unsigned task) {
   const unsigned waveTask = __builtin_amdgcn_readfirstlane(task);
   int lo = 0, hi = b.nSegs - 1;
   while (lo < hi) {
@@ -783,6 +782,68 @@ __global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real
   const int item = sg.itemStart + (int)q;
   const int g0 = K * (int)(local - q * blocksPerItem);
   sweep_corner<Real, K, false>(p, item, g0, valid);
+}
+
+template <typename Real, int K>
+__global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real> b) {
+  batch_task<Real, K>(b, (blockIdx.x * blockDim.x + threadIdx.x) >> 3);
+}
+
+// Grid-wide barrier of the persistent batch kernel: every workgroup
+// publishes its writes (agent-scope release), arrives on the counter, and
+// the last to arrive resets it and starts the next generation; the others
+// wait for that (acquire). The wait is bounded: after spinLimit polls (0:
+// at once), or when another workgroup has already given up, the workgroup
+// sets the error flag and returns false, so every wave leaves the kernel
+// whether or not the whole grid was resident. The host checks the flag.
+__device__ __forceinline__ bool grid_barrier(GridBarrier* bar, unsigned spinLimit) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();
+    const unsigned gen = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived =
+        __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (arrived == gridDim.x) {
+      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned polls = 0;
+      while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (polls++ >= spinLimit ||
+            __hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+          __hip_atomic_store(&bar->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Every level of one angle block of a batch in one launch: a resident grid
+// walks the levels in order, each workgroup taking 32 tasks (256 lanes) at a
+// time with a grid stride, with a grid barrier between levels. The per-task
+// work is the per-level kernel's, so both give the same bits.
+template <typename Real, int K>
+__global__ __launch_bounds__(256) void ucb_sweep_batch_persistent(BatchPersistentArgs<Real> b) {
+  for (int h = b.h0; h < b.h1; ++h) {
+    BatchLevelArgs<Real> a{};
+    a.doms = b.doms;
+    a.items = b.items + b.levelItem[h];
+    a.segs = b.segs + b.levelSeg[h];
+    a.nSegs = b.levelSeg[h + 1] - b.levelSeg[h];
+    a.nTasks = b.levelTasks[h];
+    a.angle0 = b.angle0;
+    for (long base = 32L * blockIdx.x; base < a.nTasks; base += 32L * gridDim.x)
+      batch_task<Real, K>(a, (unsigned)(base + (threadIdx.x >> 3)));
+    if (h + 1 < b.h1 && !grid_barrier(b.bar, b.spinLimit)) break;
+  }
 }
 
 // Incident boundary fluxes into the boundary tails of an angle block's psi
@@ -988,6 +1049,38 @@ int groups_per_lane_supported(int G, int K) {
 }
 
 template <typename Real>
+hipError_t launch_sweep_batch_persistent(const BatchPersistentArgs<Real>& b, int K, int blocks,
+                                         hipStream_t s) {
+  if (blocks < 1) return hipErrorInvalidValue;
+  const dim3 grid(blocks), block(256);
+  switch (K) {
+    case 4: hipLaunchKernelGGL((ucb_sweep_batch_persistent<Real, 4>), grid, block, 0, s, b); break;
+    case 2: hipLaunchKernelGGL((ucb_sweep_batch_persistent<Real, 2>), grid, block, 0, s, b); break;
+    default: hipLaunchKernelGGL((ucb_sweep_batch_persistent<Real, 1>), grid, block, 0, s, b); break;
+  }
+  return hipGetLastError();
+}
+
+template <typename Real, int K>
+static int resident_blocks_k() {
+  int perCU = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, ucb_sweep_batch_persistent<Real, K>, 256, 0) !=
+          hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return perCU * cus;
+}
+
+int batch_persistent_resident_blocks(int precision, int K) {
+  if (precision == 32)
+    return K == 4 ? resident_blocks_k<float, 4>() : K == 2 ? resident_blocks_k<float, 2>()
+                                                           : resident_blocks_k<float, 1>();
+  return K == 4 ? resident_blocks_k<double, 4>() : K == 2 ? resident_blocks_k<double, 2>()
+                                                          : resident_blocks_k<double, 1>();
+}
+
+template <typename Real>
 hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStream_t s) {
   const long threads = 8L * b.nTasks;
   if (threads == 0) return hipSuccess;
@@ -1127,6 +1220,10 @@ hipError_t launch_convert(const Src* s_, Dst* d, long n, hipStream_t s) {
 
 template hipError_t launch_sweep_batch_level<double>(const BatchLevelArgs<double>&, int, hipStream_t);
 template hipError_t launch_sweep_batch_level<float>(const BatchLevelArgs<float>&, int, hipStream_t);
+template hipError_t launch_sweep_batch_persistent<double>(const BatchPersistentArgs<double>&, int,
+                                                         int, hipStream_t);
+template hipError_t launch_sweep_batch_persistent<float>(const BatchPersistentArgs<float>&, int, int,
+                                                        hipStream_t);
 template hipError_t launch_sweep_level<double>(const SweepLevelArgs<double>&, bool, hipStream_t);
 template hipError_t launch_sweep_level<float>(const SweepLevelArgs<float>&, bool, hipStream_t);
 template hipError_t launch_phi_from_psi<double>(const double*, const double*, int, long, long,
