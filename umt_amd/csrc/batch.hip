@@ -264,9 +264,10 @@ int layout(umt_batch* b, int n, int K) {
     HIP_TRY(hipMemcpyAsync(b->levelTasksD, levelTasks.data(), levelTasks.size() * sizeof(long),
                            hipMemcpyHostToDevice, b->stream));
   // The persistent grid: as many workgroups as the device holds at once,
-  // and no more than the widest level has work for (32 tasks each).
+  // and no more than the widest level has work for.
+  constexpr long kWgTasks = umt::kPersistentThreads / 8;
   long widest = 1;
-  for (long t : levelTasks) widest = std::max(widest, (t + 31) / 32);
+  for (long t : levelTasks) widest = std::max(widest, (t + kWgTasks - 1) / kWgTasks);
   const int resident = umt::batch_persistent_resident_blocks(b->precision, K);
   b->persistentBlocks = (int)std::min<long>(resident, widest);
   b->anglesPerBlock = n;

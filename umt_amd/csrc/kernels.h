@@ -100,11 +100,15 @@ hipError_t launch_sweep_batch_level(const BatchLevelArgs<Real>& b, int K, hipStr
 // Barrier state of the persistent batch kernel (device memory, zeroed before
 // first use; count returns to 0 after every completed barrier).
 struct GridBarrier {
-  unsigned count;           // workgroups arrived at the current barrier
+  unsigned count;           // groups of workgroups arrived at the current barrier
   unsigned gen;             // barriers completed
   unsigned err;             // nonzero: a workgroup gave up waiting (grid not resident)
-  unsigned pad;
+  unsigned pad[29];
+  unsigned sub[8][32];      // workgroups w with w mod 8 = i arrived: sub[i][0] (one cache line each)
 };
+// Threads of a workgroup of the persistent batch kernel (one workgroup per
+// CU at its occupancy, so that few workgroups meet at each grid barrier).
+constexpr int kPersistentThreads = 1024;
 // All levels [h0, h1) of one angle block of a batch in one launch.
 template <typename Real>
 struct BatchPersistentArgs {

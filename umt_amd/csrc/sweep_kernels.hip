@@ -789,13 +789,16 @@ __global__ __launch_bounds__(256) void ucb_sweep_batch_level(BatchLevelArgs<Real
   batch_task<Real, K>(b, (blockIdx.x * blockDim.x + threadIdx.x) >> 3);
 }
 
-// Grid-wide barrier of the persistent batch kernel: every workgroup
-// publishes its writes (agent-scope release), arrives on the counter, and
-// the last to arrive resets it and starts the next generation; the others
-// wait for that (acquire). The wait is bounded: after spinLimit polls (0:
-// at once), or when another workgroup has already given up, the workgroup
-// sets the error flag and returns false, so every wave leaves the kernel
-// whether or not the whole grid was resident. The host checks the flag.
+// Grid-wide barrier of the persistent batch kernel. Every workgroup
+// publishes its writes (agent-scope release) and arrives on the counter of
+// its group (workgroups w with the same w mod 8); the last of a group
+// arrives on the top counter, and the last there resets it and starts the
+// next generation, which the others wait for (acquire). Arrivals on eight
+// lines, not one, keep them from queueing on a single address. The wait is
+// bounded: after spinLimit polls (0: at once), or when another workgroup
+// has already given up, the workgroup sets the error flag and returns
+// false, so every wave leaves the kernel whether or not the whole grid was
+// resident. The host checks the flag.
 __device__ __forceinline__ bool grid_barrier(GridBarrier* bar, unsigned spinLimit) {
   __shared__ int ok;
   __syncthreads();
@@ -803,9 +806,17 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* bar, unsigned spinLimi
     ok = 1;
     __threadfence();
     const unsigned gen = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived =
-        __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-    if (arrived == gridDim.x) {
+    const unsigned grp = blockIdx.x & 7u;
+    const unsigned inGroup = (gridDim.x - grp + 7u) >> 3;   // workgroups w with w mod 8 = grp
+    const unsigned groups = gridDim.x < 8u ? gridDim.x : 8u;
+    bool last = false;
+    if (__hip_atomic_fetch_add(&bar->sub[grp][0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
+        inGroup) {
+      __hip_atomic_store(&bar->sub[grp][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
+             groups;
+    }
+    if (last) {
       __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&bar->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -817,7 +828,7 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* bar, unsigned spinLimi
           ok = 0;
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(8);
       }
     }
     __threadfence();
@@ -827,11 +838,13 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* bar, unsigned spinLimi
 }
 
 // Every level of one angle block of a batch in one launch: a resident grid
-// walks the levels in order, each workgroup taking 32 tasks (256 lanes) at a
-// time with a grid stride, with a grid barrier between levels. The per-task
-// work is the per-level kernel's, so both give the same bits.
+// walks the levels in order, each workgroup taking kPersistentThreads / 8
+// tasks at a time with a grid stride, with a grid barrier between levels.
+// The per-task work is the per-level kernel's, so both give the same bits.
 template <typename Real, int K>
-__global__ __launch_bounds__(256) void ucb_sweep_batch_persistent(BatchPersistentArgs<Real> b) {
+__global__ __launch_bounds__(kPersistentThreads) void ucb_sweep_batch_persistent(
+    BatchPersistentArgs<Real> b) {
+  constexpr long kTasks = kPersistentThreads / 8;
   for (int h = b.h0; h < b.h1; ++h) {
     BatchLevelArgs<Real> a{};
     a.doms = b.doms;
@@ -840,7 +853,7 @@ __global__ __launch_bounds__(256) void ucb_sweep_batch_persistent(BatchPersisten
     a.nSegs = b.levelSeg[h + 1] - b.levelSeg[h];
     a.nTasks = b.levelTasks[h];
     a.angle0 = b.angle0;
-    for (long base = 32L * blockIdx.x; base < a.nTasks; base += 32L * gridDim.x)
+    for (long base = kTasks * blockIdx.x; base < a.nTasks; base += kTasks * gridDim.x)
       batch_task<Real, K>(a, (unsigned)(base + (threadIdx.x >> 3)));
     if (h + 1 < b.h1 && !grid_barrier(b.bar, b.spinLimit)) break;
   }
@@ -1052,7 +1065,7 @@ template <typename Real>
 hipError_t launch_sweep_batch_persistent(const BatchPersistentArgs<Real>& b, int K, int blocks,
                                          hipStream_t s) {
   if (blocks < 1) return hipErrorInvalidValue;
-  const dim3 grid(blocks), block(256);
+  const dim3 grid(blocks), block(kPersistentThreads);
   switch (K) {
     case 4: hipLaunchKernelGGL((ucb_sweep_batch_persistent<Real, 4>), grid, block, 0, s, b); break;
     case 2: hipLaunchKernelGGL((ucb_sweep_batch_persistent<Real, 2>), grid, block, 0, s, b); break;
@@ -1064,7 +1077,8 @@ hipError_t launch_sweep_batch_persistent(const BatchPersistentArgs<Real>& b, int
 template <typename Real, int K>
 static int resident_blocks_k() {
   int perCU = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, ucb_sweep_batch_persistent<Real, K>, 256, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, ucb_sweep_batch_persistent<Real, K>,
+                                                    kPersistentThreads, 0) !=
           hipSuccess ||
       hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
