@@ -5,7 +5,7 @@ the host, with the inputs drawn as bench.py draws them:
   - RZ: 512x512 quads, 4x4 RZ quadrature (72 swept directions), 16 groups,
     1.2e9 unknowns per sweep.
 FP64 phi within 1e-12 and FP32 within 2e-5 of the oracle, elementwise over
-every corner and group. The oracle is fed the product's host arrays (mesh,
+every corner and group (xyz: for the corner and the zone layout). The oracle is fed the product's host arrays (mesh,
 geometry, quadrature, schedule), which the oracle tests show bit-identical
 to its own at smaller sizes."""
 import os
@@ -34,15 +34,19 @@ def test_full_size_xyz_sweep_matches_oracle():
           f"{THREADS} threads", flush=True)
     ref, _ = O.sweep_all(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), S, st,
                          nthreads=THREADS, fast=True)
-    for precision, tol in ((64, 1e-12), (32, 2e-5)):
-        ctx = U.SweepContext(p, G, precision=precision)
-        ctx.set_sigt(st)
-        ctx.set_source(S)
-        ctx.sweep()
-        err = rel_err(ctx.phi(), ref)
-        ctx.close()
-        print(f"FP{precision}: max rel phi error {err:.2e}", flush=True)
-        assert err < tol, (precision, err)
+    # The corner layout (4 groups per lane) and the zone layout, which the
+    # autotuner picks at this size.
+    for layout in (4, U.ZONE_LAYOUT):
+        for precision, tol in ((64, 1e-12), (32, 2e-5)):
+            ctx = U.SweepContext(p, G, precision=precision)
+            ctx.set_groups_per_lane(layout)
+            ctx.set_sigt(st)
+            ctx.set_source(S)
+            ctx.sweep()
+            err = rel_err(ctx.phi(), ref)
+            ctx.close()
+            print(f"layout {layout} FP{precision}: max rel phi error {err:.2e}", flush=True)
+            assert err < tol, (layout, precision, err)
 
 
 def test_full_size_rz_sweep_matches_oracle():
