@@ -123,6 +123,27 @@ def test_zone_layout_on_cyclic_meshes(mesh):
     assert rel_err(phis[ZONE], ref) < 1e-12
 
 
+def test_zone_layout_fp32_on_a_cyclic_mesh():
+    """FP32 sweeps with cyclic zones and cycle lists: the corner layout's phi
+    to rounding, the oracle's within the FP32 bound of the cyclic tests."""
+    p = cyclic_problem(*CYCLIC_MESHES[2])
+    G = 32
+    st, ss, qq = U.draw_cross_sections(18, p.nzones, G, (0.5, 50.0), (0.0, 0.9), (0.0, 1.0))
+    phis = {}
+    for layout in (4, ZONE):
+        ctx = U.SweepContext(p, G, precision=32)
+        ctx.set_groups_per_lane(layout)
+        ctx.set_sigt(st)
+        ctx.set_scattering(ss, qq)
+        ctx.source_iteration(tol=0.0, max_iters=3)
+        phis[layout] = ctx.phi()
+        ctx.close()
+    _same(phis[ZONE], phis[4], "phi", 1e-5)
+    ref, _, _, _ = O.source_iteration(p.mesh(), p.geometry(), p.quadrature(), p.schedule(), st, ss,
+                                      qq, 0.0, 3)
+    assert rel_err(phis[ZONE], ref) < 1e-4
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_zone_time_cycles_and_incident_flux(precision):
     """Two time-dependent cycles (tau Psi source, savePsi sweep) with

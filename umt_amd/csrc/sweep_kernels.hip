@@ -600,6 +600,15 @@ __global__ __launch_bounds__(256) void ucb_sweep_level_zone(SweepLevelArgs<Real>
       psf[c][f] = psiA[R.off[c][f] + lane];
     });
   });
+  // The reciprocals need no upwind psi: in FP64 (four waves per SIMD either
+  // way) they are formed while its loads are in flight; FP32 forms them
+  // last, which keeps it at 8 waves.
+  Real rden[8];
+  if constexpr (sizeof(Real) == 8)
+    static_for<8>([&](auto C) {
+      constexpr int c = C;
+      rden[c] = pinned(quot<Real>(1, R.outArea[c] + sig * R.vol[c]));
+    });
   // The slot's record is re-read from LDS in each phase below rather than
   // held in registers across phases.
   reread_lds();
@@ -683,10 +692,10 @@ __global__ __launch_bounds__(256) void ucb_sweep_level_zone(SweepLevelArgs<Real>
 
   // Corner solve psi_c = (src_c + sum over upstream EZ neighbours of
   // |aez| psi_n) / den_c, den_c = outgoing areas + sigma V_c.
-  Real rden[8], psi[8];
+  Real psi[8];
   static_for<8>([&](auto C) {
     constexpr int c = C;
-    rden[c] = quot<Real>(1, R.outArea[c] + sig * R.vol[c]);
+    if constexpr (sizeof(Real) == 4) rden[c] = quot<Real>(1, R.outArea[c] + sig * R.vol[c]);
     psi[c] = src[c] * rden[c];
   });
   if (cyclic) {
