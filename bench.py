@@ -341,9 +341,11 @@ def solve(p, G, xs, accel, mixed, tol, max_iters, apb, net=None, warm_iters=2):
             ctx.set_angles_per_block(apb[precision])
         ctx.set_sigt(st)
         ctx.set_scattering(ss, qq)
-        ctx.autotune(1)
+        k, _ = ctx.autotune(1)
         if accel.startswith("gta"):
-            ctx.autotune_grey(2)
+            k2, ms2 = ctx.autotune_grey(2)
+            log(f"FP{precision} {accel}: S_N layout {k}, S2 layout {k2} "
+                f"(ms per S2 sweep {', '.join(f'{kk}: {t:.2f}' for kk, t in ms2.items())})")
         if precision == 32 and accel.startswith("gta") and "rel_tol" in FP32_GREY:
             ctx.set_grey_controls(rel_tol=FP32_GREY["rel_tol"])
         ctx.set_graphs(GRAPHS[0])
@@ -758,8 +760,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 sweeps")
     ap.add_argument("--methods", default="",
                     help="comma-separated time-to-convergence methods of the headline workload "
-                         "(default: fp64_plain, fp32_then_fp64_gta, fp32_then_fp64_gta_learned "
-                         "for large; all for single)")
+                         f"(default: {', '.join(LARGE_METHODS)} for large; "
+                         f"{', '.join(SINGLE_METHODS)} for single)")
     ap.add_argument("--dropin-angles", type=int, default=4,
                     help="angles timed through the per-angle drop-in with host arrays (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

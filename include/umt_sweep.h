@@ -272,9 +272,14 @@ int umt_ctx_enable_gta(umt_ctx *ctx, int nangles, const double *omega, const dou
  * correction (umt_ctx_set_learner); 3: both, the learned correction applied to
  * the GTA-corrected iterates (GTA is then the fallback and the arbiter). */
 int umt_ctx_set_acceleration(umt_ctx *ctx, int mode);
-/* Autotuner of the S2 grey sweeps' groups per lane (their levels are much
- * smaller than the S_N sweep's, so the best K can differ): times K = 1, 2,
- * 4, 8 (msPerK[0..3], -1 when unsupported), keeps the fastest. Needs the cross sections. No counterpart in the reference. */
+/* Autotuner of the S2 grey sweeps' layout (their levels are much smaller
+ * than the S_N sweep's, so the best layout can differ): times groups per
+ * lane K = 1, 2, 4, 8 and UMT_ZONE_LAYOUT (msPerLayout[0..4], -1 when
+ * unsupported), keeps the fastest. Needs the cross sections. No
+ * counterpart in the reference. */
+int umt_ctx_autotune_grey_layouts(umt_ctx *ctx, int reps, int *bestK, double *msPerLayout);
+/* The same, reporting the four corner-layout timings only (msPerK[0..3]);
+ * *bestK can still be UMT_ZONE_LAYOUT. */
 int umt_ctx_autotune_grey(umt_ctx *ctx, int reps, int *bestK, double *msPerK);
 /* Cumulative BiCGStab iterations and S2 sweeps done by GTA on this context. */
 int umt_ctx_gta_stats(const umt_ctx *ctx, int64_t *greyIterations, int64_t *greySweeps);

@@ -214,3 +214,30 @@ def test_autotuner_times_the_zone_layout():
     _, times = ctx.autotune(1)
     assert set(times) == {1, 2, 4, 8}
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_grey_autotuner_times_the_zone_layout(precision):
+    """The S2 grey sweeps' autotuner times the zone layout beside the corner
+    layouts; GTA then converges with the S2 layout it kept, to the answer of
+    a context whose S2 sweeps keep the default layout."""
+    p = U.TiledProblem(2, 2, 2, 2, 2)
+    G = 32
+    st, ss, qq = _xs(p, G, 11, sigt=(0.1, 100.0), c=(0.9, 0.999))
+    tol = 1e-8 if precision == 64 else 1e-5
+    runs = {}
+    for tune in (False, True):
+        ctx = U.SweepContext(p, G, precision=precision)
+        ctx.set_sigt(st)
+        ctx.set_scattering(ss, qq)
+        if tune:
+            best, times = ctx.autotune_grey(1)
+            print(f"grey autotune FP{precision}: best {best}, ms {times}", flush=True)
+            assert set(times) == {1, 2, 4, 8, ZONE} and best in times
+        ctx.set_acceleration("gta")
+        ok, it, _, _ = ctx.source_iteration(tol=tol, max_iters=200)
+        runs[tune] = (ok, it, ctx.phi())
+        ctx.close()
+    assert runs[True][0] and runs[False][0]
+    assert abs(runs[True][1] - runs[False][1]) <= 1, (runs[True][1], runs[False][1])
+    _same(runs[True][2], runs[False][2], "phi", 1e-6 if precision == 64 else 1e-4)

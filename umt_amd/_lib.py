@@ -85,6 +85,7 @@ SIGNATURES = {
     "umt_ctx_gta_stats": (_i, [_vp, _pi64, _pi64]),
     "umt_ctx_grey_sweep": (_i, [_vp, _pd, _pd, _pd]),
     "umt_ctx_autotune_grey": (_i, [_vp, _i, _pi, _pd]),
+    "umt_ctx_autotune_grey_layouts": (_i, [_vp, _i, _pi, _pd]),
     "umt_ctx_grey_solve": (_i, [_vp, _pd, _pd, _d, _pd, _pi64]),
     "umt_ctx_set_gta_solver": (_i, [_vp, _i]),
     "umt_ctx_set_grey_controls": (_i, [_vp, _d, _i]),

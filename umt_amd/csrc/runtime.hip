@@ -1818,7 +1818,7 @@ int learn_record(umt_ctx* x, double res) {
 
 extern "C" {
 
-int umt_ctx_autotune_grey(umt_ctx* x, int reps, int* bestK, double* msPerK) {
+int umt_ctx_autotune_grey_layouts(umt_ctx* x, int reps, int* bestK, double* msPerLayout) {
   if (!x || reps < 1) return fail(1, "bad argument");
   if (!x->gta.enabled) return fail(37, "GTA needs umt_ctx_enable_gta first");
   if (int rc = gta_prepare(x)) return rc;
@@ -1826,12 +1826,12 @@ int umt_ctx_autotune_grey(umt_ctx* x, int reps, int* bestK, double* msPerK) {
   const long GC = (long)x->G * x->C;
   if (x->precision == 32) HIP_TRY(umt::launch_fill_zero<float>((float*)g.S2, GC, x->stream));
   else HIP_TRY(umt::launch_fill_zero<double>((double*)g.S2, GC, x->stream));
-  static const int cand[4] = {1, 2, 4, 8};
+  static const int cand[5] = {1, 2, 4, 8, umt::kZoneLayout};
   double best = 1e300;
   int bk = x->s2.groupsPerLane;
-  for (int i = 0; i < 4; ++i) {
-    if (msPerK) msPerK[i] = -1.0;
-    if (!umt::groups_per_lane_supported(x->G, cand[i])) continue;
+  for (int i = 0; i < 5; ++i) {
+    if (msPerLayout) msPerLayout[i] = -1.0;
+    if (!umt::sweep_layout_supported(x->G, cand[i])) continue;
     x->s2.groupsPerLane = cand[i];
     if (int rc = sweep_set(x, x->s2, g.S2, g.sigt2, g.As, false, false)) return rc;  // warm-up
     HIP_TRY(hipEventRecord(x->ev0, x->stream));
@@ -1841,7 +1841,7 @@ int umt_ctx_autotune_grey(umt_ctx* x, int reps, int* bestK, double* msPerK) {
     HIP_TRY(hipEventSynchronize(x->ev1));
     float t = 0.f;
     HIP_TRY(hipEventElapsedTime(&t, x->ev0, x->ev1));
-    if (msPerK) msPerK[i] = t / reps;
+    if (msPerLayout) msPerLayout[i] = t / reps;
     if (t / reps < best) {
       best = t / reps;
       bk = cand[i];
@@ -1849,6 +1849,14 @@ int umt_ctx_autotune_grey(umt_ctx* x, int reps, int* bestK, double* msPerK) {
   }
   x->s2.groupsPerLane = bk;
   if (bestK) *bestK = bk;
+  return 0;
+}
+
+int umt_ctx_autotune_grey(umt_ctx* x, int reps, int* bestK, double* msPerK) {
+  double ms[5];
+  if (int rc = umt_ctx_autotune_grey_layouts(x, reps, bestK, ms)) return rc;
+  if (msPerK)
+    for (int i = 0; i < 4; ++i) msPerK[i] = ms[i];
   return 0;
 }
 

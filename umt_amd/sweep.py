@@ -241,10 +241,10 @@ class SweepContext:
     def autotune_grey(self, reps=2):
         """The same for the S2 grey sweeps of GTA. Returns (best K, {K: ms per S2 sweep})."""
         k = C.c_int(0)
-        ms = np.zeros(4)
-        check("umt_ctx_autotune_grey",
-              _lib.load().umt_ctx_autotune_grey(self._h, reps, C.byref(k), _dp(ms)))
-        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4, 8), ms) if t >= 0}
+        ms = np.zeros(5)
+        check("umt_ctx_autotune_grey_layouts",
+              _lib.load().umt_ctx_autotune_grey_layouts(self._h, reps, C.byref(k), _dp(ms)))
+        return k.value, {kk: float(t) for kk, t in zip((1, 2, 4, 8, ZONE_LAYOUT), ms) if t >= 0}
 
     def incoming_faces(self):
         v = C.c_double(0.0)
