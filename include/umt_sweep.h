@@ -29,12 +29,17 @@ extern "C" {
  * corner fluxes, Phi += quadwt * Psi1, exiting boundary fluxes go to PsiB, and
  * Psi(:,:,Angle) = Psi1 when *savePsi != 0. Outputs are complete when the call
  * returns (gpu_streamsynchronize is kept for the caller's sequence).
- * Supported: 3D hex zones in the corner-face order setTetonZone produces,
- * acyclic schedules and schedules with cycle lists (*numCycles corners of
- * cycleList from *cycleOffSet load Psi1 from cyclePsi before the sweep and
- * store back after it, control/constructDynMemory.F90:118-189; cyclic zones
- * come as negative nextZ entries), no reflecting boundary (*nBdyElem == 0 or
- * *Mref < 1). Anything else leaves the outputs untouched and sets a nonzero
+ * Supported: 3D meshes of any polyhedral zones, up to 32 corners per zone
+ * and 8 corner faces per corner (*maxCorner <= 32, 3 <= *maxcf <= 8). Hex
+ * zones in the corner-face order setTetonZone produces (*maxcf 3,
+ * *maxCorner 8) take the hex-tuned kernels; any other mesh takes a general
+ * kernel that follows nextC within each zone and reproduces the serial
+ * FP64 sweep bit for bit. Acyclic schedules and schedules with cycle lists
+ * (*numCycles corners of cycleList from *cycleOffSet load Psi1 from cyclePsi
+ * before the sweep and store back after it,
+ * control/constructDynMemory.F90:118-189; cyclic zones come as negative
+ * nextZ entries), no reflecting boundary (*nBdyElem == 0 or *Mref < 1).
+ * Anything else leaves the outputs untouched and sets a nonzero
  * gpu_sweepucbxyz_status().
  * ---------------------------------------------------------------------- */
 void gpu_sweepucbxyz(int *Angle, int *nHyperPlanes, int *nZonesInPlane, int *nextZ, int *nextC,

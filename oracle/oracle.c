@@ -14,9 +14,18 @@
 #include <omp.h>
 #endif
 
+/* Leading dimensions of the corner-face, corner and zone-face arrays
+ * (Teton's maxcf, maxCorner, maxFaces). The hex build takes 3, 8, 6; the
+ * polyhedral build (liboracle_poly.so) is compiled with larger values. */
+#ifndef MAXCF
 #define MAXCF 3
+#endif
+#ifndef MAXCORNER
 #define MAXCORNER 8
+#endif
+#ifndef MAXFACES
 #define MAXFACES 6
+#endif
 
 /* 1-based column-major accessors */
 #define I2(a, i, j, n1) (a)[((i)-1) + (size_t)(n1) * ((j)-1)]
@@ -37,7 +46,7 @@ int oracle_geometry(int nzones, int ncorner, const int *numCorner, const int *cO
                     const int *zoneFaces, const int *nCFaces, const int *cFP, const int *cEZ,
                     const int *CToFace, const int *zoneOpp, const double *px, double *A_fp,
                     double *A_ez, double *Volume) {
-  memset(A_fp, 0, sizeof(double) * 9 * (size_t)ncorner);
+  memset(A_fp, 0, sizeof(double) * 3 * MAXCF * (size_t)ncorner);
   for (int zone = 1; zone <= nzones; ++zone) {
     const int nFaces = zoneFaces[zone - 1];
     const int nCorner = numCorner[zone - 1];

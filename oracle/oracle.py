@@ -44,9 +44,15 @@ _SIG = {
 _libs = {}
 
 
-def load(fast=False):
-    """liboracle.so (parity checker) or liboracle_fast.so (timed CPU baseline)."""
-    name = "liboracle_fast.so" if fast else "liboracle.so"
+# Leading dimensions (corner faces, zone faces) of the polyhedral build.
+POLY_MAXCF, POLY_MAXCORNER, POLY_MAXFACES = 6, 12, 8
+
+
+def load(fast=False, poly=False):
+    """liboracle.so (parity checker), liboracle_fast.so (timed CPU baseline)
+    or liboracle_poly.so (the checker for polyhedral zones: corner-face and
+    zone-face arrays with leading dimensions POLY_MAXCF and POLY_MAXFACES)."""
+    name = "liboracle_poly.so" if poly else ("liboracle_fast.so" if fast else "liboracle.so")
     if name in _libs:
         return _libs[name]
     path = os.path.join(_HERE, name)
@@ -72,12 +78,24 @@ def _dp(a):
     return a.ctypes.data_as(_pd)
 
 
+def _lib_for(mesh):
+    """The build whose array dimensions the mesh arrays have: hex (corner
+    faces 3, zone faces 6) or polyhedral (POLY_MAXCF, POLY_MAXFACES)."""
+    maxcf, maxfaces = mesh["cFP"].shape[1], mesh["zoneOpp"].shape[1]
+    if (maxcf, maxfaces) == (3, 6):
+        return load()
+    assert (maxcf, maxfaces) == (POLY_MAXCF, POLY_MAXFACES), (maxcf, maxfaces)
+    assert int(mesh["numCorner"].max()) <= POLY_MAXCORNER
+    return load(poly=True)
+
+
 def geometry(mesh):
-    """A_fp (C,3,3), A_ez (C,3,3), Volume (C) from mesh arrays."""
+    """A_fp (C,maxcf,3), A_ez (C,maxcf,3), Volume (C) from mesh arrays."""
     Cn = mesh["nCFaces"].shape[0]
     Z = mesh["numCorner"].shape[0]
-    A_fp, A_ez, V = np.zeros((Cn, 3, 3)), np.zeros((Cn, 3, 3)), np.zeros(Cn)
-    load().oracle_geometry(Z, Cn, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]),
+    maxcf = mesh["cFP"].shape[1]
+    A_fp, A_ez, V = np.zeros((Cn, maxcf, 3)), np.zeros((Cn, maxcf, 3)), np.zeros(Cn)
+    _lib_for(mesh).oracle_geometry(Z, Cn, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]),
                            _ip(mesh["zoneFaces"]), _ip(mesh["nCFaces"]), _ip(mesh["cFP"]),
                            _ip(mesh["cEZ"]), _ip(mesh["CToFace"]), _ip(mesh["zoneOpp"]),
                            _dp(mesh["px"]), _dp(A_fp), _dp(A_ez), _dp(V))
@@ -110,12 +128,13 @@ def schedule_angle(mesh, A_fp, A_ez, omega, par_tol=0.0, cycles=False):
             _ip(mesh["nCFaces"]), _ip(mesh["cEZ"]), _ip(mesh["CToFace"]), _ip(mesh["zoneOpp"]),
             _ip(mesh["faceOpp"]), _dp(A_fp), _dp(A_ez), _dp(np.ascontiguousarray(omega, np.float64)),
             par_tol, _ip(nextZ), _ip(nextC), _ip(zip_), C.byref(nhp))
+    lib = _lib_for(mesh)
     if not cycles:
-        rc = load().oracle_schedule_angle_tol(*args)
+        rc = lib.oracle_schedule_angle_tol(*args)
         return rc, nextZ, nextC, zip_[:nhp.value].copy()
     cyc = np.zeros(Cn, np.int32)
     nc = C.c_int(0)
-    rc = load().oracle_schedule_angle_cyc(*args, _ip(cyc), C.byref(nc))
+    rc = lib.oracle_schedule_angle_cyc(*args, _ip(cyc), C.byref(nc))
     return rc, nextZ, nextC, zip_[:nhp.value].copy(), cyc[:nc.value].copy()
 
 
@@ -166,7 +185,7 @@ def sweep_angle(mesh, geom, STotal, Sigt, omega, quadwt, nextZ, nextC, zonesInPl
     n = 0 if cycle_list is None else len(cycle_list)
     if n:
         assert cycle_psi.shape == (n, G) and cycle_psi.flags["C_CONTIGUOUS"]
-    load().oracle_sweep_angle_cyc(
+    _lib_for(mesh).oracle_sweep_angle_cyc(
         Z, Cn, nb, G, _ip(mesh["numCorner"]), _ip(mesh["cOffSet"]), _ip(mesh["nCFaces"]),
         _ip(mesh["cFP"]), _ip(mesh["cEZ"]), _dp(geom["A_fp"]), _dp(geom["A_ez"]),
         _dp(geom["Volume"]), _dp(STotal), _dp(Sigt), tau, _dp(Psi),

@@ -148,3 +148,57 @@ def test_dropin_rejects_bad_schedule_and_cycle_list_without_touching_outputs(pro
         s["cycleList"] = np.zeros(0, np.int32)
     assert status == 42
     assert np.all(Phi == 3.0)
+
+
+def test_dropin_refuses_polyhedral_arrays_it_cannot_take():
+    """A mesh that is not the canonical hex goes to the polyhedral kernel
+    after its arrays are checked; arrays it cannot take leave the outputs
+    alone and set the status (before any device call)."""
+    from polymesh import PolyProblem, hex_pyramid_mesh
+
+    quad = U.TiledProblem(1, 1, 1, 2, 2).quadrature()
+    p = PolyProblem(hex_pyramid_mesh(2, 2, 1), quad)
+    G = 2
+    st = np.ones((p.nzones, G))
+    S = np.ones((p.ncorner, G))
+    m, sched = p.mesh(), p.schedule()
+
+    class Patched:
+        ncorner, nbelem, nangles = p.ncorner, p.nbelem, p.nangles
+
+        def __init__(self, mesh=None, schedule=None):
+            self._m = dict(m, **(mesh or {}))
+            self._s = dict(sched, **(schedule or {}))
+
+        def mesh(self):
+            return self._m
+
+        def schedule(self):
+            return self._s
+
+        geometry = staticmethod(p.geometry)
+        quadrature = staticmethod(p.quadrature)
+
+    def status(q):
+        Psi1 = np.zeros((p.ncorner + p.nbelem, G))
+        Phi = np.full((p.ncorner, G), 7.0)
+        rc = U.sweep_ucbxyz(q, 1, S, st, Psi1, np.zeros((p.nbelem, G)), Phi)
+        assert np.all(Phi == 7.0)
+        return rc, U.load().umt_last_error().decode()
+
+    nextC = sched["nextC"].copy()
+    nextC[0, 1] = nextC[0, 0]  # not an order of the zone's corners
+    rc, msg = status(Patched(schedule=dict(nextC=nextC)))
+    assert rc == 47 and "nextC" in msg
+    cEZ = m["cEZ"].copy()
+    cEZ[3, 0] = m["numCorner"][0] + 1  # EZ neighbour outside its zone
+    rc, msg = status(Patched(mesh=dict(cEZ=cEZ)))
+    assert rc == 13 and "EZ neighbour" in msg
+    nCF = m["nCFaces"].copy()
+    nCF[2] = 7  # more corner faces than maxcf
+    rc, msg = status(Patched(mesh=dict(nCFaces=nCF)))
+    assert rc == 12 and "corner faces" in msg
+    cFP = m["cFP"].copy()
+    cFP[5, 1] = p.ncorner + p.nbelem + 1
+    rc, msg = status(Patched(mesh=dict(cFP=cFP)))
+    assert rc == 46

@@ -61,6 +61,37 @@ struct SweepLevelArgs {
 template <typename Real>
 hipError_t launch_sweep_level(const SweepLevelArgs<Real>& a, bool singleAngle, hipStream_t s);
 
+// One hyperplane of the drop-in's single-angle FP64 sweep on a mesh of
+// general polyhedral zones (corner arrays in Teton's layout, not
+// zone-blocked). One thread per (zone, group).
+constexpr int kPolyMaxCorner = 32;  // corners per zone
+constexpr int kPolyMaxCF = 8;       // corner faces per corner
+struct PolySweepArgs {
+  const int2* items;        // (zone, 0) of this hyperplane, 0-based; a zone on the
+  int nItems;               // cycle list comes as ~zone
+  int G;
+  int C;                    // corners; cFP values above C are boundary elements
+  int maxcf;                // leading dimension of the corner-face arrays
+  const int* numCorner;     // (Z)
+  const int* cOffSet;       // (Z)
+  const int* nCFaces;       // (C)
+  const int* cFP;           // (maxcf, C), 1-based
+  const int* cEZ;           // (maxcf, C), 1-based corner within the zone
+  const int* nextC;         // (C): corner order of each zone, 1-based within the zone
+  const double* Afp;        // (3, maxcf, C)
+  const double* Aez;        // (3, maxcf, C)
+  const double* vol;        // (C)
+  const double* S;          // (G, C)
+  const double* sigt;       // (G, Z)
+  const double* omega;      // (3)
+  const double* tauPsi;     // null, or source += tau * Psi (G, C)
+  double tau;
+  double* psi;              // (G, C + nbelem): corner fluxes, then the boundary tail
+  double* phi;              // (G, C), phi += quadwt * psi
+  double quadwt;
+};
+hipError_t launch_sweep_poly(const PolySweepArgs& a, hipStream_t s);
+
 // Many domains in one level launch (batches of uneven domains).
 template <typename Real>
 struct BatchDomain {

@@ -2460,16 +2460,20 @@ int umt_ctx_learn_stats(const umt_ctx* x, int64_t* corrections, int* fellBackAt)
 namespace {
 
 // Device buffers of one drop-in call: the host arrays' device copies.
+// Corner arrays are (3, maxcf, C) and (maxcf, C) in size; the hex kernels
+// take them zone-blocked with maxcf = 3, the polyhedral kernel in Teton's
+// layout, with cEZ, nCFaces, nextC, numCorner and cOffSet beside them.
 struct DropBufs {
   double *Afp = nullptr, *Aez = nullptr, *vol = nullptr, *S = nullptr, *sigt = nullptr,
          *omega = nullptr, *psi1 = nullptr, *phi = nullptr, *tauPsi = nullptr;
-  int* cFP = nullptr;
+  int *cFP = nullptr, *cEZ = nullptr, *nCF = nullptr, *nextC = nullptr, *numCorner = nullptr,
+      *cOffSet = nullptr;
   int2* items = nullptr;
 };
 
 struct DropIn : DropBufs {
   hipStream_t stream = nullptr;
-  size_t capC = 0, capZ = 0, capG = 0, capNb = 0, capItems = 0;
+  size_t capC = 0, capZ = 0, capG = 0, capNb = 0, capCF = 0, capItems = 0;
 };
 
 constexpr int kMaxStreams = 80;
@@ -2505,17 +2509,20 @@ int scratch_address(void** out) {
   return 0;
 }
 
-// Bytes of the device buffers of a call, each rounded up to 256 bytes.
-size_t dropin_bytes(size_t C, size_t Z, size_t G, size_t nb, bool tau) {
+// Bytes of the device buffers of a call, each rounded up to 256 bytes
+// (mcf: corner faces per corner of the arrays; poly: the polyhedral path's
+// extra arrays).
+size_t dropin_bytes(size_t C, size_t Z, size_t G, size_t nb, bool tau, size_t mcf, bool poly) {
   auto r = [](size_t n) { return (n + 255) / 256 * 256; };
-  return r(9 * C * 8) * 2 + r(C * 8) + r(G * C * 8) + r(G * Z * 8) + r(3 * 8) +
-         r(G * (C + nb) * 8) + r(G * C * 8) + (tau ? r(G * C * 8) : 0) + r(3 * C * 4) +
-         r(Z * sizeof(int2));
+  return r(3 * mcf * C * 8) * 2 + r(C * 8) + r(G * C * 8) + r(G * Z * 8) + r(3 * 8) +
+         r(G * (C + nb) * 8) + r(G * C * 8) + (tau ? r(G * C * 8) : 0) + r(mcf * C * 4) +
+         r(Z * sizeof(int2)) + (poly ? r(mcf * C * 4) + 2 * r(C * 4) + 2 * r(Z * 4) : 0);
 }
 
 // Carves the buffers of a call from the no-alloc scratch.
-int dropin_carve(DropBufs& b, size_t C, size_t Z, size_t G, size_t nb, bool tau) {
-  const size_t need = dropin_bytes(C, Z, G, nb, tau);
+int dropin_carve(DropBufs& b, size_t C, size_t Z, size_t G, size_t nb, bool tau, size_t mcf,
+                 bool poly) {
+  const size_t need = dropin_bytes(C, Z, G, nb, tau, mcf, poly);
   if (need > kNoAllocScratchBytes)
     return fail(53, "gpu_sweepucbxyz_noalloc: the call needs %zu bytes of device scratch, more "
                     "than its %zu (use gpu_sweepucbxyz)", need, kNoAllocScratchBytes);
@@ -2526,8 +2533,8 @@ int dropin_carve(DropBufs& b, size_t C, size_t Z, size_t G, size_t nb, bool tau)
     dst = (std::remove_reference_t<decltype(dst)>)(void*)p;
     p += (bytes + 255) / 256 * 256;
   };
-  take(b.Afp, 9 * C * 8);
-  take(b.Aez, 9 * C * 8);
+  take(b.Afp, 3 * mcf * C * 8);
+  take(b.Aez, 3 * mcf * C * 8);
   take(b.vol, C * 8);
   take(b.S, G * C * 8);
   take(b.sigt, G * Z * 8);
@@ -2536,40 +2543,133 @@ int dropin_carve(DropBufs& b, size_t C, size_t Z, size_t G, size_t nb, bool tau)
   take(b.phi, G * C * 8);
   if (tau) take(b.tauPsi, G * C * 8);
   else b.tauPsi = nullptr;
-  take(b.cFP, 3 * C * 4);
+  take(b.cFP, mcf * C * 4);
   take(b.items, Z * sizeof(int2));
+  if (poly) {
+    take(b.cEZ, mcf * C * 4);
+    take(b.nCF, C * 4);
+    take(b.nextC, C * 4);
+    take(b.numCorner, Z * 4);
+    take(b.cOffSet, Z * 4);
+  } else {
+    b.cEZ = b.nCF = b.nextC = b.numCorner = b.cOffSet = nullptr;
+  }
   return 0;
 }
 
-int dropin_alloc(DropIn& d, size_t C, size_t Z, size_t G, size_t nb) {
-  if (C <= d.capC && Z <= d.capZ && G <= d.capG && nb <= d.capNb) return 0;
+int dropin_alloc(DropIn& d, size_t C, size_t Z, size_t G, size_t nb, size_t mcf) {
+  if (C <= d.capC && Z <= d.capZ && G <= d.capG && nb <= d.capNb && mcf <= d.capCF) return 0;
   dfree(d.Afp); dfree(d.Aez); dfree(d.vol); dfree(d.S); dfree(d.sigt); dfree(d.omega);
   dfree(d.psi1); dfree(d.phi); dfree(d.tauPsi); dfree(d.cFP); dfree(d.items);
+  dfree(d.cEZ); dfree(d.nCF); dfree(d.nextC); dfree(d.numCorner); dfree(d.cOffSet);
   d.capC = std::max(C, d.capC); d.capZ = std::max(Z, d.capZ);
   d.capG = std::max(G, d.capG); d.capNb = std::max(nb, d.capNb);
-  C = d.capC; Z = d.capZ; G = d.capG; nb = d.capNb;
+  d.capCF = std::max(mcf, d.capCF);
+  C = d.capC; Z = d.capZ; G = d.capG; nb = d.capNb; mcf = d.capCF;
   int rc;
-  if ((rc = dalloc(&d.Afp, 9 * C)) || (rc = dalloc(&d.Aez, 9 * C)) || (rc = dalloc(&d.vol, C)) ||
-      (rc = dalloc(&d.S, G * C)) || (rc = dalloc(&d.sigt, G * Z)) || (rc = dalloc(&d.omega, 3)) ||
-      (rc = dalloc(&d.psi1, G * (C + nb))) || (rc = dalloc(&d.phi, G * C)) ||
-      (rc = dalloc(&d.tauPsi, G * C)) || (rc = dalloc(&d.cFP, 3 * C)) || (rc = dalloc(&d.items, Z))) {
-    d.capC = d.capZ = d.capG = d.capNb = 0;
+  if ((rc = dalloc(&d.Afp, 3 * mcf * C)) || (rc = dalloc(&d.Aez, 3 * mcf * C)) ||
+      (rc = dalloc(&d.vol, C)) || (rc = dalloc(&d.S, G * C)) || (rc = dalloc(&d.sigt, G * Z)) ||
+      (rc = dalloc(&d.omega, 3)) || (rc = dalloc(&d.psi1, G * (C + nb))) ||
+      (rc = dalloc(&d.phi, G * C)) || (rc = dalloc(&d.tauPsi, G * C)) ||
+      (rc = dalloc(&d.cFP, mcf * C)) || (rc = dalloc(&d.items, Z)) ||
+      (rc = dalloc(&d.cEZ, mcf * C)) || (rc = dalloc(&d.nCF, C)) || (rc = dalloc(&d.nextC, C)) ||
+      (rc = dalloc(&d.numCorner, Z)) || (rc = dalloc(&d.cOffSet, Z))) {
+    d.capC = d.capZ = d.capG = d.capNb = d.capCF = 0;
     return rc;
   }
   d.capItems = Z;
   return 0;
 }
 
-int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const double* STotal,
-                 double tau, double* Psi, int G, const double* Volume, const double* Sigt,
-                 const int* nCFaces, int ndim, int maxcf, int C, const double* A_fp,
-                 const double* omega, const int* cFP, double* Psi1, int nb, const double* A_ez,
-                 const int* cEZ, double quadwt, double* Phi, double* PsiB, int maxCorner,
-                 int streamId, int savePsi, int numCycles, int cycleOffSet, double* cyclePsi,
-                 const int* cycleList, int nBdyElem, int Mref, const int* numCorner,
-                 const int* cOffSet, bool noAlloc) {
+// Checks the corner arrays of a mesh that is not the canonical hex for the
+// polyhedral kernel: every zone's corners inside (0, C], at most maxCorner of
+// them, 3..maxcf corner faces per corner, EZ neighbours within the zone and
+// nextC a permutation of the zone's corners. cornerZone receives each
+// corner's zone.
+int check_poly_topology(int Z, int C, int maxcf, int maxCorner, int nb, const int* numCorner,
+                        const int* cOffSet, const int* nCFaces, const int* cEZ, const int* cFP,
+                        const int* nextC, std::vector<int>& cornerZone) {
+  cornerZone.assign(C, -1);
+  for (int z = 0; z < Z; ++z) {
+    const int n = numCorner[z], c0 = cOffSet[z];
+    if (n < 1 || n > maxCorner || c0 < 0 || (long)c0 + n > C)
+      return fail(11, "zone %d: %d corners at offset %d (maxCorner %d, %d corners)", z + 1, n, c0,
+                  maxCorner, C);
+    unsigned long long seen = 0;
+    for (int c = 0; c < n; ++c) {
+      const long cc = (long)c0 + c;
+      if (cornerZone[cc] >= 0) return fail(11, "corner %ld belongs to two zones", cc + 1);
+      cornerZone[cc] = z;
+      if (nCFaces[cc] < 3 || nCFaces[cc] > maxcf)
+        return fail(12, "corner %ld has %d corner faces (3 to maxcf = %d)", cc + 1, nCFaces[cc], maxcf);
+      for (int f = 0; f < nCFaces[cc]; ++f) {
+        const int e = cEZ[cc * maxcf + f];
+        if (e < 1 || e > n || e == c + 1)
+          return fail(13, "corner %ld face %d: EZ neighbour %d outside its zone's %d corners", cc + 1,
+                      f + 1, e, n);
+        if (cFP[cc * maxcf + f] < 1 || cFP[cc * maxcf + f] > C + nb)
+          return fail(46, "cFP(%d, %ld) = %d out of range", f + 1, cc + 1, cFP[cc * maxcf + f]);
+      }
+      const int k = nextC[cc];
+      if (k < 1 || k > n || (seen >> (k - 1) & 1))
+        return fail(47, "zone %d: nextC is not an order of its %d corners", z + 1, n);
+      seen |= 1ull << (k - 1);
+    }
+  }
+  return 0;
+}
+
+// split_cyclic_planes for one angle of a polyhedral mesh: a zone goes one
+// sub-level after the latest earlier zone of its plane that it shares a face
+// with, so every read of a neighbour's psi sees what the serial sweep sees.
+void split_cyclic_planes_poly(int Z, int C, int maxcf, const int* cFP, const int* nCFaces,
+                              const int* numCorner, const int* cOffSet,
+                              const std::vector<int>& cornerZone, std::vector<int2>& items,
+                              std::vector<int>& planeOffset) {
+  std::vector<int2> out;
+  std::vector<int> offs(1, 0), mark(Z, -1), sub;
+  const int L = (int)planeOffset.size() - 1;
+  for (int h = 0; h < L; ++h) {
+    const int i0 = planeOffset[h], i1 = planeOffset[h + 1];
+    sub.assign(i1 - i0, 0);
+    int maxSub = 0;
+    for (int k = i0; k < i1; ++k) {
+      const int z = items[k].x < 0 ? ~items[k].x : items[k].x;
+      int sl = 0;
+      for (int c = cOffSet[z]; c < cOffSet[z] + numCorner[z]; ++c)
+        for (int f = 0; f < nCFaces[c]; ++f) {
+          const int nb = cFP[(size_t)c * maxcf + f] - 1;
+          if (nb >= C) continue;
+          const int zn = cornerZone[nb];
+          if (zn != z && zn >= 0 && mark[zn] >= 0) sl = std::max(sl, sub[mark[zn]] + 1);
+        }
+      sub[k - i0] = sl;
+      mark[z] = k - i0;
+      maxSub = std::max(maxSub, sl);
+    }
+    for (int sLevel = 0; sLevel <= maxSub; ++sLevel) {
+      for (int k = i0; k < i1; ++k)
+        if (sub[k - i0] == sLevel) out.push_back(items[k]);
+      offs.push_back((int)out.size());
+    }
+    for (int k = i0; k < i1; ++k) mark[items[k].x < 0 ? ~items[k].x : items[k].x] = -1;
+  }
+  items.swap(out);
+  planeOffset.swap(offs);
+}
+
+int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const int* nextC,
+                 const double* STotal, double tau, double* Psi, int G, const double* Volume,
+                 const double* Sigt, const int* nCFaces, int ndim, int maxcf, int C,
+                 const double* A_fp, const double* omega, const int* cFP, double* Psi1, int nb,
+                 const double* A_ez, const int* cEZ, double quadwt, double* Phi, double* PsiB,
+                 int maxCorner, int streamId, int savePsi, int numCycles, int cycleOffSet,
+                 double* cyclePsi, const int* cycleList, int nBdyElem, int Mref,
+                 const int* numCorner, const int* cOffSet, bool noAlloc) {
   if (ndim != 3) return fail(40, "only 3D (ndim = 3) is supported, got %d", ndim);
-  if (maxcf != 3 || maxCorner != 8) return fail(41, "maxcf = %d, maxCorner = %d (need 3 and 8)", maxcf, maxCorner);
+  if (maxcf < 3 || maxcf > umt::kPolyMaxCF || maxCorner < 1 || maxCorner > umt::kPolyMaxCorner)
+    return fail(41, "maxcf = %d, maxCorner = %d (supported: maxcf 3 to %d, maxCorner up to %d)",
+                maxcf, maxCorner, umt::kPolyMaxCF, umt::kPolyMaxCorner);
   if (numCycles < 0 || cycleOffSet < 0 || (numCycles > 0 && (!cyclePsi || !cycleList)))
     return fail(42, "angle %d: bad cycle list (numCycles %d, cycleOffSet %d)", Angle, numCycles, cycleOffSet);
   if (nBdyElem > 0 && Mref >= 1) return fail(43, "reflecting boundaries are not supported");
@@ -2577,15 +2677,25 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   long Zl = 0;
   for (int h = 0; h < nHP; ++h) Zl += zip[h];
   const int Z = (int)Zl;
-  if (int rc = check_hex_topology(Z, C, numCorner, cOffSet, nCFaces, cEZ)) return rc;
+  // Canonical hex zones take the hex kernels; anything else the polyhedral
+  // kernel (check_hex_topology's message is dropped when it refuses).
+  const bool hex = maxcf == 3 && maxCorner == 8 && check_hex_topology(Z, C, numCorner, cOffSet, nCFaces, cEZ) == 0;
+  std::vector<int> cornerZone;
+  if (!hex) {
+    if (!nextC) return fail(47, "the polyhedral sweep needs nextC");
+    if (int rc = check_poly_topology(Z, C, maxcf, maxCorner, nb, numCorner, cOffSet, nCFaces, cEZ,
+                                     cFP, nextC, cornerZone))
+      return rc;
+  }
   std::vector<int2> items(Z);
   for (int i = 0; i < Z; ++i) {
     const int z = nextZ[i];
     if (z == 0 || z > Z || z < -Z) return fail(45, "nextZ(%d) = %d out of range", i + 1, z);
     items[i] = make_int2(z > 0 ? z - 1 : ~(-z - 1), 0);  // cyclic zone: ~zone
   }
-  for (long i = 0; i < 3L * C; ++i)
-    if (cFP[i] < 1 || cFP[i] > C + nb) return fail(46, "cFP entry %ld out of range", i + 1);
+  if (hex)
+    for (long i = 0; i < 3L * C; ++i)
+      if (cFP[i] < 1 || cFP[i] > C + nb) return fail(46, "cFP entry %ld out of range", i + 1);
   for (int m = 0; m < numCycles; ++m)
     if (cycleList[cycleOffSet + m] < 1 || cycleList[cycleOffSet + m] > C)
       return fail(42, "angle %d: cycle list entry %d = corner %d out of range", Angle,
@@ -2593,33 +2703,48 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   std::vector<int> planeOffset(nHP + 1, 0);
   for (int h = 0; h < nHP; ++h) planeOffset[h + 1] = planeOffset[h] + zip[h];
   if (numCycles > 0) {
-    const int one = numCycles;
-    split_cyclic_planes(Z, C, cFP, 1, std::vector<int>(1, one), items, planeOffset);
+    if (hex)
+      split_cyclic_planes(Z, C, cFP, 1, std::vector<int>(1, numCycles), items, planeOffset);
+    else
+      split_cyclic_planes_poly(Z, C, maxcf, cFP, nCFaces, numCorner, cOffSet, cornerZone, items,
+                               planeOffset);
   }
   const int nLevels = (int)planeOffset.size() - 1;
 
   // Device buffers: the stream's own allocation (grown as needed), or for
   // the no-alloc entry point the static scratch on the null stream.
   const bool useTau = tau != 0.0;
+  const size_t mcf = hex ? 3 : (size_t)maxcf;
   DropBufs d;
   hipStream_t s = nullptr;
   if (noAlloc) {
-    if (int rc = dropin_carve(d, C, Z, G, nb, useTau)) return rc;
+    if (int rc = dropin_carve(d, C, Z, G, nb, useTau, mcf, !hex)) return rc;
   } else {
     DropIn& di = g_drop[streamId % kMaxStreams];
     if (!di.stream) HIP_TRY(hipStreamCreate(&di.stream));
-    if (int rc = dropin_alloc(di, C, Z, G, nb)) return rc;
+    if (int rc = dropin_alloc(di, C, Z, G, nb, mcf)) return rc;
     d = di;
     s = di.stream;
   }
   const size_t GC = (size_t)G * C;
-  const std::vector<double> afpDev = umt::zone_blocked(A_fp, 9, (size_t)C);
-  const std::vector<double> aezDev = umt::zone_blocked(A_ez, 9, (size_t)C);
-  const std::vector<int> cfpDev = umt::zone_blocked(cFP, 3, (size_t)C);
-  HIP_TRY(hipMemcpyAsync(d.Afp, afpDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(d.Aez, aezDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
+  if (hex) {
+    const std::vector<double> afpDev = umt::zone_blocked(A_fp, 9, (size_t)C);
+    const std::vector<double> aezDev = umt::zone_blocked(A_ez, 9, (size_t)C);
+    const std::vector<int> cfpDev = umt::zone_blocked(cFP, 3, (size_t)C);
+    HIP_TRY(hipMemcpyAsync(d.Afp, afpDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.Aez, aezDev.data(), 9 * (size_t)C * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.cFP, cfpDev.data(), 3 * (size_t)C * 4, hipMemcpyHostToDevice, s));
+  } else {
+    HIP_TRY(hipMemcpyAsync(d.Afp, A_fp, 3 * mcf * C * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.Aez, A_ez, 3 * mcf * C * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.cFP, cFP, mcf * C * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.cEZ, cEZ, mcf * C * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.nCF, nCFaces, (size_t)C * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.nextC, nextC, (size_t)C * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.numCorner, numCorner, (size_t)Z * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d.cOffSet, cOffSet, (size_t)Z * 4, hipMemcpyHostToDevice, s));
+  }
   HIP_TRY(hipMemcpyAsync(d.vol, Volume, (size_t)C * 8, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(d.cFP, cfpDev.data(), 3 * (size_t)C * 4, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.S, STotal, GC * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.sigt, Sigt, (size_t)G * Z * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d.omega, omega, 3 * 8, hipMemcpyHostToDevice, s));
@@ -2634,27 +2759,56 @@ int dropin_sweep(int Angle, int nHP, const int* zip, const int* nextZ, const dou
   HIP_TRY(hipMemcpyAsync(d.psi1, Psi1, (GC + (size_t)G * nb) * 8, hipMemcpyHostToDevice, s));
   if (useTau) HIP_TRY(hipMemcpyAsync(d.tauPsi, Psi, GC * 8, hipMemcpyHostToDevice, s));
 
-  umt::SweepLevelArgs<double> a{};
-  a.G = G;
-  a.groupsPerLane = umt::groups_per_lane_supported(G, 2) ? 2 : 1;
-  a.C = C;
-  a.Afp = d.Afp;
-  a.Aez = d.Aez;
-  a.vol = d.vol;
-  a.cFP = d.cFP;
-  a.S = d.S;
-  a.sigt = d.sigt;
-  a.omega = d.omega;
-  a.psi = d.psi1;
-  a.psiAngleStride = 0;
-  a.phi = d.phi;
-  a.quadwt = quadwt;
-  a.tauPsi = useTau ? d.tauPsi : nullptr;
-  a.tau = tau;
-  for (int h = 0; h < nLevels; ++h) {
-    a.items = d.items + planeOffset[h];
-    a.nItems = planeOffset[h + 1] - planeOffset[h];
-    HIP_TRY(umt::launch_sweep_level<double>(a, true, s));
+  if (hex) {
+    umt::SweepLevelArgs<double> a{};
+    a.G = G;
+    a.groupsPerLane = umt::groups_per_lane_supported(G, 2) ? 2 : 1;
+    a.C = C;
+    a.Afp = d.Afp;
+    a.Aez = d.Aez;
+    a.vol = d.vol;
+    a.cFP = d.cFP;
+    a.S = d.S;
+    a.sigt = d.sigt;
+    a.omega = d.omega;
+    a.psi = d.psi1;
+    a.psiAngleStride = 0;
+    a.phi = d.phi;
+    a.quadwt = quadwt;
+    a.tauPsi = useTau ? d.tauPsi : nullptr;
+    a.tau = tau;
+    for (int h = 0; h < nLevels; ++h) {
+      a.items = d.items + planeOffset[h];
+      a.nItems = planeOffset[h + 1] - planeOffset[h];
+      HIP_TRY(umt::launch_sweep_level<double>(a, true, s));
+    }
+  } else {
+    umt::PolySweepArgs a{};
+    a.G = G;
+    a.C = C;
+    a.maxcf = (int)mcf;
+    a.numCorner = d.numCorner;
+    a.cOffSet = d.cOffSet;
+    a.nCFaces = d.nCF;
+    a.cFP = d.cFP;
+    a.cEZ = d.cEZ;
+    a.nextC = d.nextC;
+    a.Afp = d.Afp;
+    a.Aez = d.Aez;
+    a.vol = d.vol;
+    a.S = d.S;
+    a.sigt = d.sigt;
+    a.omega = d.omega;
+    a.tauPsi = useTau ? d.tauPsi : nullptr;
+    a.tau = tau;
+    a.psi = d.psi1;
+    a.phi = d.phi;
+    a.quadwt = quadwt;
+    for (int h = 0; h < nLevels; ++h) {
+      a.items = d.items + planeOffset[h];
+      a.nItems = planeOffset[h + 1] - planeOffset[h];
+      HIP_TRY(umt::launch_sweep_poly(a, s));
+    }
   }
   HIP_TRY(hipMemcpyAsync(Psi1, d.psi1, GC * 8, hipMemcpyDeviceToHost, s));
   if (nb > 0) HIP_TRY(hipMemcpyAsync(PsiB, d.psi1 + GC, (size_t)G * nb * 8, hipMemcpyDeviceToHost, s));
@@ -2678,9 +2832,9 @@ void gpu_sweepucbxyz(int* Angle, int* nHyperPlanes, int* nZonesInPlane, int* nex
                      int* savePsi, int* numCycles, int* cycleOffSet, double* cyclePsi,
                      int* cycleList, int* b0, int* nBdyElem, double* PsiBMref, int* Mref,
                      int* Geom_numCorner, int* Geom_cOffSet) {
-  (void)nextC; (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)b0; (void)PsiBMref;
+  (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)b0; (void)PsiBMref;
   std::lock_guard<std::mutex> lock(g_dropMutex);
-  g_dropin_status = dropin_sweep(*Angle, *nHyperPlanes, nZonesInPlane, nextZ, STotal, *tau, Psi,
+  g_dropin_status = dropin_sweep(*Angle, *nHyperPlanes, nZonesInPlane, nextZ, nextC, STotal, *tau, Psi,
                                  *Groups, Volume, Sigt, nCFacesArray, *ndim, *maxcf, *ncorner, A_fp,
                                  omega, cFP, Psi1, *nbelem, A_ez, cEZ, *quadwt, Phi, PsiB,
                                  *maxCorner, *streamIdPtr, *savePsi, *numCycles, *cycleOffSet,
@@ -2699,9 +2853,9 @@ void gpu_sweepucbxyz_noalloc(int* Angle, int* nHyperPlanes, int* nZonesInPlane, 
                              int* numCycles, int* cycleOffSet, double* cyclePsi, int* cycleList,
                              int* b0, int* nBdyElem, double* PsiBMref, int* Mref,
                              int* Geom_numCorner, int* Geom_cOffSet) {
-  (void)nextC; (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)b0; (void)PsiBMref;
+  (void)NumAngles; (void)mem0solve1; (void)totalStreams; (void)b0; (void)PsiBMref;
   std::lock_guard<std::mutex> lock(g_dropMutex);
-  g_dropin_status = dropin_sweep(*Angle, *nHyperPlanes, nZonesInPlane, nextZ, STotal, *tau, Psi,
+  g_dropin_status = dropin_sweep(*Angle, *nHyperPlanes, nZonesInPlane, nextZ, nextC, STotal, *tau, Psi,
                                  *Groups, Volume, Sigt, nCFacesArray, *ndim, *maxcf, *ncorner, A_fp,
                                  omega, cFP, Psi1, *nbelem, A_ez, cEZ, *quadwt, Phi, PsiB,
                                  *maxCorner, *streamIdPtr, *savePsi, *numCycles, *cycleOffSet,

@@ -505,9 +505,13 @@ def sweep_ucbxyz(problem, angle, STotal, Sigt, Psi1, PsiB_angle, Phi, tau=0.0, P
     if cycle_psi is None:
         cycle_psi = np.zeros((max(len(s["cycleList"]), 1), STotal.shape[1]))
     assert cycle_psi.dtype == np.float64 and cycle_psi.flags["C_CONTIGUOUS"]
+    # Teton's maxcf and maxCorner: the corner-face arrays' leading dimension
+    # and the most corners of a zone (3 and 8 on hex meshes).
+    maxcf = int(m["cFP"].shape[1])
+    max_corner = int(np.max(m["numCorner"]))
     ints = {k: C.c_int(v) for k, v in dict(
-        Angle=angle, nHP=nhp, Groups=G, ndim=3, maxcf=3, ncorner=problem.ncorner,
-        nbelem=problem.nbelem, NumAngles=problem.nangles, maxCorner=8, mem0solve1=0,
+        Angle=angle, nHP=nhp, Groups=G, ndim=3, maxcf=maxcf, ncorner=problem.ncorner,
+        nbelem=problem.nbelem, NumAngles=problem.nangles, maxCorner=max_corner, mem0solve1=0,
         streamId=stream_id, totalStreams=1, savePsi=int(save_psi), numCycles=int(ncyc[a]),
         cycleOffSet=int(np.sum(ncyc[:a])), b0=0, nBdyElem=0, Mref=-1).items()}
     tau_c = C.c_double(tau)
